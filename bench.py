@@ -1,0 +1,148 @@
+"""Benchmark: swap trials/s of the lattice-sampling hot path on one MI355X.
+
+Workload (one "step"): the quaternary bcc anneal -- bcc 32^3 conventional
+cells (65536 sites), 4 equiatomic species, 4 shells with V^(n) ~ N(0, 30/n
+meV) (zero-mean per shell), T from 3000 K to 100 K in 50 linear steps of 200
+sweeps each, lattice-wide swaps, energy recounted after every sweep.  One
+step = the whole anneal (>= 655.36 M swap trials) from the same synthetic
+start, on an HBM-resident lattice, through the C-ABI.
+
+Prints ONE JSON line.  With --gpus N > 1 it exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def bytes_per_trial(z: int) -> int:
+    # int8 occupations of both sites and their z neighbours, plus two writes
+    return 2 * (z + 1) + 2
+
+
+def cpu_baseline(chain, budget_s: float):
+    """The oracle (single-core C restatement of the reference's serial
+    path: per-trial pair_energy, exp acceptance, full energy after every
+    sweep) on a bounded sample of the same anneal: the first temperature
+    step, as many sweeps as fit the time budget."""
+    from oracle import oracle as orc
+    orc.build()
+    ref = orc.OracleLattice(chain.lattice, chain.n, chain.v_ex)
+    gen = orc.Generator(110179)
+    cfg = ref.initial_setup(gen, species_numbers=chain.counts)
+    N = ref.n_atoms
+    # calibrate on one sweep, then run a sample of ~budget_s seconds
+    t0 = time.perf_counter()
+    ref.anneal(cfg, gen, T=chain.T, n_mc_steps=N, n_sample_steps=N)
+    one = time.perf_counter() - t0
+    sweeps = max(1, int(budget_s / max(one, 1e-6)))
+    t0 = time.perf_counter()
+    ref.anneal(cfg, gen, T=chain.T, n_mc_steps=sweeps * N, n_sample_steps=N)
+    dt = time.perf_counter() - t0
+    trials = sweeps * N
+    return {"value": trials / dt, "unit": "swap trials/s", "cores": 1, "kind": "port",
+            "sample": f"first T step (3000 K) of the same anneal, {sweeps} sweeps = {trials} "
+                      f"trials with the reference's per-sweep energy recount, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(the multi-GPU layout is described in DESIGN.md)"}))
+        sys.exit(2)
+
+    from brawl_amd import Lattice, workloads
+    from brawl_amd import _lib
+
+    chain = workloads.config2()
+    log(f"[bench] workload {chain.name}: N={chain.n_atoms} S={chain.S} R={chain.R}")
+    # start configuration: equiatomic random occupation (reference initial_setup)
+    lat = Lattice(chain.lattice, chain.n, chain.v_ex)
+    lat.seed(110179)
+    start = lat.initial_setup(species_numbers=chain.counts)
+    z = {1: [12, 6, 24, 12, 24, 8], 2: [8, 6, 12, 24, 8, 6, 24, 24, 24, 32]}[chain.lattice]
+    z = sum(z[: chain.R])
+    bpt = bytes_per_trial(z)
+
+    def step(seed):
+        lat.set_config(start)
+        return lat.metropolis_simulated_annealing(T=chain.T, delta_T=chain.delta_T,
+                                                  T_steps=chain.T_steps, n_sweeps=chain.n_sweeps,
+                                                  seed=seed)
+
+    for w in range(args.warmup):
+        r = step(1000 + w)
+        log(f"[bench] warmup {w}: {r.trials} trials, kernel {r.kernel_ms:.1f} ms")
+    total_trials = 0
+    kernel_ms = []
+    results = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        r = step(k + 1)
+        total_trials += r.trials
+        kernel_ms.append(r.kernel_ms)
+        results.append(r)
+        log(f"[bench] step {k}: {r.trials} trials, kernel {r.kernel_ms:.1f} ms")
+    dt = time.perf_counter() - t0
+    value = total_trials / dt
+    avg_kernel_s = np.mean(kernel_ms) / 1e3
+    trials_per_launch = results[0].trials
+    achieved = trials_per_launch * bpt / avg_kernel_s / 1e9
+    peak = 8000.0
+    last = results[-1]
+    out = {
+        "metric": "swap trials/s on 1 MI355X (plain vs batched-served); p50/p99 run latency",
+        "value": value,
+        "unit": "swap trials/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "plain: quaternary bcc anneal, bcc 32^3 cells (65536 sites), "
+                               "S=4 equiatomic, 4 shells, 3000->100 K in 50 steps x 200 sweeps",
+                   "form": "plain (one chain per call)", "trials_per_step": trials_per_launch,
+                   "bytes_per_trial": bpt},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "k_metro_par", "kernel_ms": float(np.mean(kernel_ms))},
+        "results": {"E_per_atom_first_T": float(last.energies_of_T[0]),
+                    "E_per_atom_last_T": float(last.energies_of_T[-1]),
+                    "C_max": float(np.max(last.C_of_T)),
+                    "acceptance_first_last": [float(last.acceptance_of_T[0]),
+                                              float(last.acceptance_of_T[-1])]},
+    }
+    lat.close()
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(chain, args.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,104 @@
+"""ctypes binding of libbrawl_hip.so (the C-ABI declared in include/brawl_hip.h).
+
+The shared library is built in-tree (brawl_amd/libbrawl_hip.so) by
+__graft_entry__.build() / `make -C brawl_amd/csrc`.  There is no fallback:
+if the library is missing, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbrawl_hip.so")
+
+BWL_OK = 0
+BWL_ERR_ARG = -1
+BWL_ERR_UNSUPPORTED = -2
+BWL_ERR_DEVICE = -3
+BWL_ERR_NOMEM = -4
+BWL_ERR_STATE = -5
+FCC, BCC = 1, 2
+
+# Every symbol include/brawl_hip.h declares.
+EXPORTS = (
+    "bwl_last_error", "bwl_device_count", "bwl_create", "bwl_destroy", "bwl_n_atoms",
+    "bwl_set_config", "bwl_get_config", "bwl_pair_histogram", "bwl_total_energy",
+    "bwl_site_energy", "bwl_pair_energy", "bwl_genrand_seed", "bwl_genrand_get_state",
+    "bwl_genrand_set_state", "bwl_initial_setup", "bwl_mc_steps", "bwl_metropolis_anneal",
+)
+
+
+class LatticeDesc(C.Structure):
+    _fields_ = [("lattice", C.c_int32), ("n1", C.c_int32), ("n2", C.c_int32),
+                ("n3", C.c_int32), ("n_species", C.c_int32), ("n_shells", C.c_int32)]
+
+
+class AnnealParams(C.Structure):
+    _fields_ = [("T", C.c_double), ("delta_T", C.c_double), ("T_steps", C.c_int32),
+                ("n_sweeps", C.c_int32), ("burn_in_sweeps", C.c_int32),
+                ("reserved", C.c_int32), ("seed", C.c_uint64)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("trials", C.c_int64), ("accepted", C.c_int64), ("kernel_ms", C.c_double)]
+
+
+class BrawlError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                          "or `make -C brawl_amd/csrc`")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp, i8p, i32p, i64p, u32p, dp = (C.c_void_p, P(C.c_int8), P(C.c_int32), P(C.c_int64),
+                                     P(C.c_uint32), P(C.c_double))
+    sig = {
+        "bwl_last_error": (C.c_char_p, []),
+        "bwl_device_count": (C.c_int, [i32p]),
+        "bwl_create": (C.c_int, [P(LatticeDesc), dp, P(vp)]),
+        "bwl_destroy": (C.c_int, [vp]),
+        "bwl_n_atoms": (C.c_int, [vp, i32p]),
+        "bwl_set_config": (C.c_int, [vp, i8p]),
+        "bwl_get_config": (C.c_int, [vp, i8p]),
+        "bwl_pair_histogram": (C.c_int, [vp, i64p]),
+        "bwl_total_energy": (C.c_int, [vp, dp]),
+        "bwl_site_energy": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp]),
+        "bwl_pair_energy": (C.c_int, [vp, i32p, i32p, dp]),
+        "bwl_genrand_seed": (C.c_int, [vp, C.c_uint32]),
+        "bwl_genrand_get_state": (C.c_int, [vp, u32p, i32p]),
+        "bwl_genrand_set_state": (C.c_int, [vp, u32p, C.c_int32]),
+        "bwl_initial_setup": (C.c_int, [vp, i64p, dp]),
+        "bwl_mc_steps": (C.c_int, [vp, C.c_double, C.c_int32, C.c_int64, C.c_int64, dp, i64p]),
+        "bwl_metropolis_anneal": (C.c_int, [vp, P(AnnealParams), dp, dp, dp, P(RunStats)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != BWL_OK:
+        msg = load().bwl_last_error()
+        raise BrawlError(rc, msg.decode() if msg else "")
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    load().bwl_device_count(C.byref(n))
+    return int(n.value)

@@ -1,0 +1,435 @@
+// bwl_api.hip -- the C-ABI (include/brawl_hip.h) over the gfx950 kernels.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/brawl_hip.h"
+#include "bwl_host.h"
+#include "bwl_kernels.h"
+
+using namespace bwl;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_OK(call)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(BWL_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+}  // namespace
+
+struct bwl_ctx {
+  bwl_lattice_desc desc;
+  Geo g;
+  std::vector<int32_t> off;
+  std::vector<double> vex;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int8_t *d_compact = nullptr;
+  int8_t *d_ref = nullptr;
+  int32_t *d_off = nullptr;
+  double *d_vex = nullptr;
+  void *d_scratch = nullptr;  // small per-call outputs
+  size_t scratch_bytes = 0;
+  bool has_config = false;
+  MT19937 mt;
+  unsigned long long phase = 0;  // parallel-sampler round counter
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+int ensure_scratch(bwl_ctx *c, size_t bytes) {
+  if (bytes <= c->scratch_bytes) return BWL_OK;
+  if (c->d_scratch) HIP_OK(hipFree(c->d_scratch));
+  c->d_scratch = nullptr;
+  c->scratch_bytes = 0;
+  if (hipMalloc(&c->d_scratch, bytes) != hipSuccess) return fail(BWL_ERR_NOMEM, "device allocation failed");
+  c->scratch_bytes = bytes;
+  return BWL_OK;
+}
+
+int check_ctx(bwl_ctx *c, bool need_config) {
+  if (!c) return fail(BWL_ERR_ARG, "null context");
+  if (need_config && !c->has_config) return fail(BWL_ERR_STATE, "no configuration uploaded");
+  HIP_OK(hipSetDevice(c->device));
+  return BWL_OK;
+}
+
+int lds_limit() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return kLdsBytes;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess && v > 0)
+    return v;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+    return v;
+  return kLdsBytes;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *bwl_last_error(void) { return g_err.c_str(); }
+
+int bwl_device_count(int32_t *count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  if (count) *count = n;
+  return e == hipSuccess ? BWL_OK : fail(BWL_ERR_DEVICE, hipGetErrorString(e));
+}
+
+int bwl_create(const bwl_lattice_desc *desc, const double *v_ex, bwl_ctx **out) {
+  if (!desc || !v_ex || !out) return fail(BWL_ERR_ARG, "null argument");
+  *out = nullptr;
+  bwl_ctx *c = new bwl_ctx();
+  c->desc = *desc;
+  int rc = make_geo(*desc, c->g, c->off, g_err);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  const Geo &g = c->g;
+  c->vex.assign(v_ex, v_ex + (size_t)g.S * g.S * g.R);
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+    delete c;
+    return fail(BWL_ERR_DEVICE, "no HIP device available");
+  }
+  if (hipGetDevice(&c->device) != hipSuccess) c->device = 0;
+  size_t ref_bytes = (size_t)g.m[0] * g.m[1] * g.m[2];
+  bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipMalloc(&c->d_compact, g.compact_bytes) == hipSuccess &&
+            hipMalloc(&c->d_ref, ref_bytes) == hipSuccess &&
+            hipMalloc(&c->d_off, sizeof(int32_t) * c->off.size()) == hipSuccess &&
+            hipMalloc(&c->d_vex, sizeof(double) * c->vex.size()) == hipSuccess &&
+            hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
+  if (!ok) {
+    bwl_destroy(c);
+    return fail(BWL_ERR_NOMEM, "device allocation failed");
+  }
+  if (hipMemcpy(c->d_off, c->off.data(), sizeof(int32_t) * c->off.size(), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(c->d_vex, c->vex.data(), sizeof(double) * c->vex.size(), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    bwl_destroy(c);
+    return fail(BWL_ERR_DEVICE, "upload failed");
+  }
+  c->mt.seed(110179u);  // f90_init_genrand static seed, rank 0 (mt19937ar.c:138)
+  *out = c;
+  return BWL_OK;
+}
+
+int bwl_destroy(bwl_ctx *c) {
+  if (!c) return BWL_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->d_compact);
+  (void)hipFree(c->d_ref);
+  (void)hipFree(c->d_off);
+  (void)hipFree(c->d_vex);
+  (void)hipFree(c->d_scratch);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return BWL_OK;
+}
+
+int bwl_n_atoms(const bwl_ctx *c, int32_t *n_atoms) {
+  if (!c || !n_atoms) return fail(BWL_ERR_ARG, "null argument");
+  *n_atoms = c->g.n_atoms;
+  return BWL_OK;
+}
+
+int bwl_set_config(bwl_ctx *c, const int8_t *config) {
+  int rc = check_ctx(c, false);
+  if (rc) return rc;
+  if (!config) return fail(BWL_ERR_ARG, "null config");
+  rc = check_config(c->g, config, g_err);
+  if (rc) return rc;
+  size_t ref_bytes = (size_t)c->g.m[0] * c->g.m[1] * c->g.m[2];
+  HIP_OK(hipMemcpyAsync(c->d_ref, config, ref_bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(launch_pack(c->d_ref, c->d_compact, c->g, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  c->has_config = true;
+  return BWL_OK;
+}
+
+int bwl_get_config(bwl_ctx *c, int8_t *config) {
+  int rc = check_ctx(c, true);
+  if (rc) return rc;
+  if (!config) return fail(BWL_ERR_ARG, "null config");
+  size_t ref_bytes = (size_t)c->g.m[0] * c->g.m[1] * c->g.m[2];
+  HIP_OK(launch_unpack(c->d_compact, c->d_ref, c->g, c->stream));
+  HIP_OK(hipMemcpyAsync(config, c->d_ref, ref_bytes, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return BWL_OK;
+}
+
+int bwl_pair_histogram(bwl_ctx *c, int64_t *hist) {
+  int rc = check_ctx(c, true);
+  if (rc) return rc;
+  if (!hist) return fail(BWL_ERR_ARG, "null hist");
+  const Geo &g = c->g;
+  const int threads = 512;
+  if (hist_lds_bytes(g, threads) > (size_t)lds_limit())
+    return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
+  const size_t hbytes = sizeof(long long) * g.S * g.S * g.R;
+  rc = ensure_scratch(c, sizeof(HistArgs) + hbytes + 64);
+  if (rc) return rc;
+  HistArgs a;
+  a.g = g;
+  a.compact = c->d_compact;
+  a.off = c->d_off;
+  a.hist = (long long *)((char *)c->d_scratch + ((sizeof(HistArgs) + 63) & ~63));
+  HIP_OK(hipMemcpyAsync(c->d_scratch, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
+  HIP_OK(launch_histogram((const HistArgs *)c->d_scratch, 1, g, threads, c->stream));
+  HIP_OK(hipMemcpyAsync(hist, a.hist, hbytes, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return BWL_OK;
+}
+
+int bwl_total_energy(bwl_ctx *c, double *energy) {
+  if (!c || !energy) return fail(BWL_ERR_ARG, "null argument");
+  std::vector<int64_t> h((size_t)c->g.S * c->g.S * c->g.R);
+  int rc = bwl_pair_histogram(c, h.data());
+  if (rc) return rc;
+  const int S = c->g.S;
+  double e = 0.0;
+  for (int n = 0; n < c->g.R; ++n)
+    for (int b = 0; b < S; ++b)
+      for (int a = 0; a < S; ++a) {
+        size_t i = a + (size_t)S * (b + (size_t)S * n);
+        e += (double)h[i] * c->vex[i];
+      }
+  *energy = 0.5 * e;
+  return BWL_OK;
+}
+
+static int site_pair(bwl_ctx *c, const int32_t *s1, const int32_t *s2, double *out2) {
+  int rc = check_ctx(c, true);
+  if (rc) return rc;
+  const Geo &g = c->g;
+  if (site_lds_bytes(g) > (size_t)lds_limit())
+    return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
+  const int32_t *sites[2] = {s1, s2};
+  SiteArgs a;
+  a.g = g;
+  for (int w = 0; w < 2; ++w) {
+    const int32_t *s = sites[w];
+    if (s[0] != 1) return fail(BWL_ERR_ARG, "basis index must be 1");
+    int x = s[1] - 1, y = s[2] - 1, z = s[3] - 1;
+    if (x < 0 || y < 0 || z < 0 || x >= g.m[0] || y >= g.m[1] || z >= g.m[2])
+      return fail(BWL_ERR_ARG, "site index out of range");
+    bool occ = g.lattice == BWL_FCC ? (((x + y + z) & 1) == 0)
+                                    : ((x & 1) == (y & 1) && (y & 1) == (z & 1));
+    if (!occ) return fail(BWL_ERR_ARG, "site is not a lattice site");
+    a.x[w] = x; a.y[w] = y; a.z[w] = z;
+  }
+  rc = ensure_scratch(c, 64);
+  if (rc) return rc;
+  a.compact = c->d_compact;
+  a.off = c->d_off;
+  a.vex = c->d_vex;
+  a.out = (double *)c->d_scratch;
+  HIP_OK(launch_site_energy(a, c->stream));
+  HIP_OK(hipMemcpyAsync(out2, a.out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return BWL_OK;
+}
+
+int bwl_site_energy(bwl_ctx *c, int32_t site_b, int32_t site_i, int32_t site_j, int32_t site_k,
+                    double *energy) {
+  if (!energy) return fail(BWL_ERR_ARG, "null energy");
+  int32_t s[4] = {site_b, site_i, site_j, site_k};
+  double o[2];
+  int rc = site_pair(c, s, s, o);
+  if (rc) return rc;
+  *energy = o[0];
+  return BWL_OK;
+}
+
+int bwl_pair_energy(bwl_ctx *c, const int32_t *site1, const int32_t *site2, double *energy) {
+  if (!energy || !site1 || !site2) return fail(BWL_ERR_ARG, "null argument");
+  double o[2];
+  int rc = site_pair(c, site1, site2, o);
+  if (rc) return rc;
+  *energy = o[1];
+  return BWL_OK;
+}
+
+int bwl_genrand_seed(bwl_ctx *c, uint32_t seed) {
+  if (!c) return fail(BWL_ERR_ARG, "null context");
+  c->mt.seed(seed);
+  return BWL_OK;
+}
+
+int bwl_genrand_get_state(bwl_ctx *c, uint32_t *mt, int32_t *mti) {
+  if (!c || !mt || !mti) return fail(BWL_ERR_ARG, "null argument");
+  memcpy(mt, c->mt.mt, sizeof(c->mt.mt));
+  *mti = c->mt.mti;
+  return BWL_OK;
+}
+
+int bwl_genrand_set_state(bwl_ctx *c, const uint32_t *mt, int32_t mti) {
+  if (!c || !mt) return fail(BWL_ERR_ARG, "null argument");
+  if (mti < 0 || mti > 625) return fail(BWL_ERR_ARG, "mti out of range");
+  memcpy(c->mt.mt, mt, sizeof(c->mt.mt));
+  c->mt.mti = mti;
+  return BWL_OK;
+}
+
+int bwl_initial_setup(bwl_ctx *c, const int64_t *species_numbers, const double *concentrations) {
+  int rc = check_ctx(c, false);
+  if (rc) return rc;
+  std::vector<int8_t> cfg((size_t)c->g.m[0] * c->g.m[1] * c->g.m[2]);
+  rc = initial_setup_host(c->g, species_numbers, concentrations, c->mt, cfg.data(), g_err);
+  if (rc) return rc;
+  return bwl_set_config(c, cfg.data());
+}
+
+int bwl_mc_steps(bwl_ctx *c, double beta, int32_t nbr_swap, int64_t n_trials, int64_t sample_every,
+                 double *energies, int64_t *accepted) {
+  int rc = check_ctx(c, true);
+  if (rc) return rc;
+  if (n_trials < 0) return fail(BWL_ERR_ARG, "negative trial count");
+  if (energies && sample_every <= 0) return fail(BWL_ERR_ARG, "sample_every must be positive");
+  const Geo &g = c->g;
+  if (seq_lds_bytes(g) > (size_t)lds_limit())
+    return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
+  if (2 * g.R > 64) return fail(BWL_ERR_UNSUPPORTED, "too many shells");
+  // the MT state must be initialised (mti == 625 means "seed with 5489")
+  if (c->mt.mti == 625) c->mt.seed(5489u);
+  const long long n_samples = energies ? n_trials / sample_every : 0;
+  const size_t need = 624 * 4 + 64 + sizeof(long long) + sizeof(double) * (size_t)n_samples + 256;
+  rc = ensure_scratch(c, need);
+  if (rc) return rc;
+  char *base = (char *)c->d_scratch;
+  SeqArgs a;
+  a.g = g;
+  a.compact = c->d_compact;
+  a.off = c->d_off;
+  a.vex = c->d_vex;
+  a.mt = (uint32_t *)base;
+  a.mt_index = (int32_t *)(base + 624 * 4);
+  a.accepted = (long long *)(base + 624 * 4 + 16);
+  a.energies = n_samples ? (double *)(base + 624 * 4 + 64) : nullptr;
+  a.beta = beta;
+  a.nbr_swap = nbr_swap ? 1 : 0;
+  a.n_trials = n_trials;
+  a.sample_every = energies ? sample_every : 0;
+  int32_t mti = c->mt.mti;
+  HIP_OK(hipMemcpyAsync(a.mt, c->mt.mt, 624 * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(a.mt_index, &mti, 4, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(launch_metro_seq(a, c->stream));
+  long long acc = 0;
+  HIP_OK(hipMemcpyAsync(c->mt.mt, a.mt, 624 * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(&mti, a.mt_index, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(&acc, a.accepted, sizeof(acc), hipMemcpyDeviceToHost, c->stream));
+  if (n_samples)
+    HIP_OK(hipMemcpyAsync(energies, a.energies, sizeof(double) * n_samples, hipMemcpyDeviceToHost,
+                          c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  c->mt.mti = mti;
+  if (accepted) *accepted = acc;
+  return BWL_OK;
+}
+
+int bwl_metropolis_anneal(bwl_ctx *c, const bwl_anneal_params *p, double *energies_of_T,
+                          double *C_of_T, double *acceptance_of_T, bwl_run_stats *stats) {
+  int rc = check_ctx(c, true);
+  if (rc) return rc;
+  if (!p || !energies_of_T || !C_of_T || !acceptance_of_T)
+    return fail(BWL_ERR_ARG, "null argument");
+  if (p->T_steps < 1 || p->n_sweeps < 1 || p->burn_in_sweeps < 0)
+    return fail(BWL_ERR_ARG, "T_steps and n_sweeps must be positive");
+  const Geo &g = c->g;
+  ParPlan plan;
+  rc = plan_parallel(g, plan, g_err);
+  if (rc) return rc;
+  int threads = ((plan.npairs + 63) / 64) * 64;
+  if (threads > 1024) threads = 1024;
+  if (threads < 64) threads = 64;
+  size_t lds = par_lds_bytes(g, threads);
+  if (lds > (size_t)lds_limit()) return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
+  const bool wide = g.S > 5;
+  const size_t T = (size_t)p->T_steps;
+  size_t off_args = 0;
+  size_t off_muls = 256 * ((sizeof(ParArgs) + 255) / 256);
+  size_t off_out = off_muls + 256 * ((plan.muls.size() * 4 + 255) / 256);
+  size_t off_cnt = off_out + 3 * T * sizeof(double);
+  size_t need = off_cnt + 64;
+  rc = ensure_scratch(c, need);
+  if (rc) return rc;
+  char *base = (char *)c->d_scratch;
+  ParArgs a;
+  memset(&a, 0, sizeof(a));
+  a.g = g;
+  a.compact = c->d_compact;
+  a.off = c->d_off;
+  a.vex = c->d_vex;
+  for (int k = 0; k < 3; ++k) {
+    a.K[k] = plan.K[k];
+    a.stride[k] = plan.stride[k];
+  }
+  a.A = plan.A;
+  a.npairs = plan.npairs;
+  a.phases_per_sweep = plan.phases_per_sweep;
+  a.n_mul = (int32_t)plan.muls.size();
+  a.muls = (const uint32_t *)(base + off_muls);
+  a.T0 = p->T;
+  a.dT = p->delta_T;
+  a.T_steps = p->T_steps;
+  a.n_sweeps = p->n_sweeps;
+  a.burn_sweeps = p->burn_in_sweeps;
+  a.seed_lo = (uint32_t)p->seed;
+  a.seed_hi = (uint32_t)(p->seed >> 32);
+  a.chain_id = 0;
+  a.phase0 = 0;
+  a.energies_of_T = (double *)(base + off_out);
+  a.C_of_T = a.energies_of_T + T;
+  a.acc_of_T = a.C_of_T + T;
+  a.e_trace = nullptr;
+  a.counters = (long long *)(base + off_cnt);
+  long long cnt0[2] = {0, 0};
+  HIP_OK(hipMemcpyAsync(base + off_args, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(base + off_muls, plan.muls.data(), plan.muls.size() * 4,
+                        hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(a.counters, cnt0, sizeof(cnt0), hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipEventRecord(c->ev0, c->stream));
+  HIP_OK(launch_metro_par((const ParArgs *)(base + off_args), 1, lds, threads, wide, c->stream));
+  HIP_OK(hipEventRecord(c->ev1, c->stream));
+  std::vector<double> out(3 * T);
+  HIP_OK(hipMemcpyAsync(out.data(), a.energies_of_T, 3 * T * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  long long cnt[2];
+  HIP_OK(hipMemcpyAsync(cnt, a.counters, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  memcpy(energies_of_T, out.data(), T * sizeof(double));
+  memcpy(C_of_T, out.data() + T, T * sizeof(double));
+  memcpy(acceptance_of_T, out.data() + 2 * T, T * sizeof(double));
+  if (stats) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    stats->trials = (long long)p->T_steps * (p->n_sweeps + p->burn_in_sweeps) *
+                    (long long)plan.phases_per_sweep * plan.npairs;
+    stats->accepted = cnt[1];
+    stats->kernel_ms = ms;
+  }
+  return BWL_OK;
+}
+
+}  // extern "C"

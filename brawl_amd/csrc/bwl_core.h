@@ -1,0 +1,113 @@
+// bwl_core.h -- shared host/device definitions for the gfx950 lattice kernels.
+//
+// Data layout (HBM and LDS)
+//   The reference stores occupations on a doubled cubic grid
+//   config(1, 2*n1, 2*n2, 2*n3), of which only 1/4 of the cells are sites
+//   (shared_data.f90:30; fcc occupied iff x+y+z even, bcc iff x=y=z mod 2,
+//   0-based doubled coordinates).  Here every lattice is stored per
+//   sublattice: a site at doubled coordinates (x,y,z) has parity bits
+//   (x&1, y&1, z&1) -> sublattice s, and cell (x>>1, y>>1, z>>1).
+//     fcc: 4 sublattices (000, 011, 101, 110), s = 2*(x&1) + (y&1)
+//     bcc: 2 sublattices (000, 111),           s = x&1
+//   HBM ("compact"): int8 [s][cz][cy][cx], n1*n2*n3 bytes per sublattice.
+//   LDS ("padded"):  int8 [s][cz+h][cy+h][cx+h] with a periodic halo of h
+//   cells on every face, so every neighbour of an interior site is at a
+//   fixed signed byte offset from it (no wrap arithmetic in the hot loop).
+#pragma once
+
+#include <stdint.h>
+
+namespace bwl {
+
+constexpr int kMaxShells = 10;
+constexpr int kMaxSpecies = 8;
+constexpr int kMaxNbr = 168;        // bcc, 10 shells
+constexpr int kLdsBytes = 160 * 1024;  // gfx950 LDS per workgroup
+
+// Geometry of one lattice, computed on the host, passed by value.
+struct Geo {
+  int32_t lattice;          // 1 fcc, 2 bcc
+  int32_t n[3];             // cells per axis
+  int32_t m[3];             // doubled-grid extent 2*n
+  int32_t h;                // halo width (cells)
+  int32_t P[3];             // padded extents n + 2h
+  int32_t nsub;             // 4 (fcc) or 2 (bcc)
+  int32_t sub_stride;       // P0*P1*P2
+  int32_t lds_bytes;        // nsub*sub_stride rounded up to 16
+  int32_t compact_bytes;    // nsub*n0*n1*n2
+  int32_t S, R;             // species, shells
+  int32_t z;                // neighbours over shells 1..R
+  int32_t rcut;             // largest |offset component| over shells 1..R
+  int32_t n_atoms;
+  int32_t shell_start[kMaxShells + 1];
+};
+
+// Parity bits of each sublattice.
+__host__ __device__ inline void sub_bits(int lattice, int s, int &bx, int &by, int &bz) {
+  if (lattice == 1) {
+    // 000, 011, 101, 110
+    bx = s >> 1;
+    by = s & 1;
+    bz = bx ^ by;
+  } else {
+    bx = by = bz = s;
+  }
+}
+
+__host__ __device__ inline int sub_of(int lattice, int bx, int by) {
+  return lattice == 1 ? 2 * bx + by : bx;
+}
+
+// Padded LDS address of the interior cell (s, cx, cy, cz).
+__host__ __device__ inline int padded_addr(const Geo &g, int s, int cx, int cy, int cz) {
+  return s * g.sub_stride + ((cz + g.h) * g.P[1] + (cy + g.h)) * g.P[0] + (cx + g.h);
+}
+
+// Padded address of the site at 0-based doubled coordinates (x,y,z).
+__host__ __device__ inline int padded_addr_xyz(const Geo &g, int x, int y, int z) {
+  return padded_addr(g, sub_of(g.lattice, x & 1, y & 1), x >> 1, y >> 1, z >> 1);
+}
+
+__host__ __device__ inline int compact_addr(const Geo &g, int s, int cx, int cy, int cz) {
+  return ((s * g.n[2] + cz) * g.n[1] + cy) * g.n[0] + cx;
+}
+
+// ------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): counter-based RNG,
+// written from the published round function and constants.
+// ------------------------------------------------------------------
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__host__ __device__ inline U4 philox4x32(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    U4 n;
+    n.x = hi1 ^ c.y ^ k0;
+    n.y = lo1;
+    n.z = hi0 ^ c.w ^ k1;
+    n.w = lo0;
+    c = n;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// 53-bit uniform in [0,1) from two words
+__host__ __device__ inline double u01_53(uint32_t a, uint32_t b) {
+  uint64_t v = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);
+  return (double)v * (1.0 / 9007199254740992.0);
+}
+
+// uniform integer in [0, m) from a 32-bit word (m < 2^31)
+__host__ __device__ inline uint32_t below(uint32_t r, uint32_t m) {
+  return (uint32_t)(((uint64_t)r * m) >> 32);
+}
+
+}  // namespace bwl

@@ -1,0 +1,267 @@
+// bwl_host.cpp -- host-side logic (geometry, MT19937, initial configuration,
+// parallel round plan).  Plain C++, no device calls.
+#include "bwl_host.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+
+#include "bwl_tables.h"
+
+namespace bwl {
+
+namespace {
+
+int floor_div2(int v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); }
+
+}  // namespace
+
+int make_geo(const bwl_lattice_desc &d, Geo &g, std::vector<int32_t> &off, std::string &err) {
+  memset(&g, 0, sizeof(g));
+  if (d.lattice != BWL_FCC && d.lattice != BWL_BCC) {
+    err = "lattice must be BWL_FCC or BWL_BCC (simple_cubic is not supported)";
+    return d.lattice == 0 ? BWL_ERR_ARG : BWL_ERR_UNSUPPORTED;
+  }
+  if (d.n1 < 1 || d.n2 < 1 || d.n3 < 1 || d.n1 > 512 || d.n2 > 512 || d.n3 > 512) {
+    err = "cell counts n1, n2, n3 must be in 1..512";
+    return BWL_ERR_ARG;
+  }
+  if (d.n_species < 1 || d.n_species > kMaxSpecies) {
+    err = "n_species must be in 1..8";
+    return BWL_ERR_UNSUPPORTED;
+  }
+  const int max_shells = d.lattice == BWL_FCC ? 6 : 10;
+  if (d.n_shells < 1 || d.n_shells > max_shells) {
+    err = "unsupported number of shells";  // initialise.F90:168, 206, 237
+    return BWL_ERR_UNSUPPORTED;
+  }
+  g.lattice = d.lattice;
+  g.n[0] = d.n1; g.n[1] = d.n2; g.n[2] = d.n3;
+  for (int a = 0; a < 3; ++a) g.m[a] = 2 * g.n[a];
+  g.S = d.n_species;
+  g.R = d.n_shells;
+  g.nsub = d.lattice == BWL_FCC ? 4 : 2;
+  g.n_atoms = g.nsub * d.n1 * d.n2 * d.n3;
+  const int8_t(*tab)[3] = d.lattice == BWL_FCC ? kFccOffsets : kBccOffsets;
+  const int *sz = d.lattice == BWL_FCC ? kFccShellSize : kBccShellSize;
+  int z = 0;
+  g.shell_start[0] = 0;
+  for (int n = 0; n < g.R; ++n) {
+    z += sz[n];
+    g.shell_start[n + 1] = z;
+  }
+  g.z = z;
+  int rcut = 0, h = 0;
+  for (int t = 0; t < z; ++t)
+    for (int a = 0; a < 3; ++a) {
+      rcut = std::max(rcut, (int)std::abs(tab[t][a]));
+      for (int b = 0; b <= 1; ++b) h = std::max(h, std::abs(floor_div2(b + tab[t][a])));
+    }
+  g.rcut = rcut;
+  g.h = h;
+  for (int a = 0; a < 3; ++a) g.P[a] = g.n[a] + 2 * h;
+  g.sub_stride = g.P[0] * g.P[1] * g.P[2];
+  long long lds = (long long)g.nsub * g.sub_stride;
+  g.lds_bytes = (int)((lds + 15) & ~15LL);
+  g.compact_bytes = g.nsub * d.n1 * d.n2 * d.n3;
+  off.assign((size_t)g.nsub * z, 0);
+  for (int s = 0; s < g.nsub; ++s) {
+    int bx, by, bz;
+    sub_bits(g.lattice, s, bx, by, bz);
+    for (int t = 0; t < z; ++t) {
+      int ex = bx + tab[t][0], ey = by + tab[t][1], ez = bz + tab[t][2];
+      int s2 = sub_of(g.lattice, ex & 1, ey & 1);
+      int dcx = floor_div2(ex), dcy = floor_div2(ey), dcz = floor_div2(ez);
+      off[(size_t)s * z + t] =
+          (s2 - s) * g.sub_stride + (dcz * g.P[1] + dcy) * g.P[0] + dcx;
+    }
+  }
+  return BWL_OK;
+}
+
+// ---------------------------------------------------------------------------
+void MT19937::seed(uint32_t s) {
+  mt[0] = s;
+  for (int i = 1; i < 624; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
+  mti = 624;
+}
+
+uint32_t MT19937::next() {
+  if (mti >= 624) {
+    if (mti == 625) seed(5489u);
+    for (int k = 0; k < 624; ++k) {
+      uint32_t y = (mt[k] & 0x80000000u) | (mt[(k + 1) % 624] & 0x7fffffffu);
+      mt[k] = mt[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+    }
+    mti = 0;
+  }
+  uint32_t y = mt[mti++];
+  y ^= y >> 11;
+  y ^= (y << 7) & 0x9d2c5680u;
+  y ^= (y << 15) & 0xefc60000u;
+  y ^= y >> 18;
+  return y;
+}
+
+// ---------------------------------------------------------------------------
+// initial_setup: species counts from numbers (REAL(4) concentrations,
+// initialise.F90:468-472) or from concentrations (nint + round-robin
+// correction, :480-504); then rejection sampling site by site, in the
+// reference's loop order, against inclusive cumulative bounds.
+int initial_setup_host(const Geo &g, const int64_t *species_numbers, const double *concentrations,
+                       MT19937 &mt, int8_t *config, std::string &err) {
+  const int S = g.S;
+  const int n_sites = g.n_atoms;
+  std::vector<int64_t> count(S, 0), placed(S, 0);
+  std::vector<double> conc(S + 1, 0.0);
+  int64_t total_numbers = 0;
+  if (species_numbers)
+    for (int i = 0; i < S; ++i) total_numbers += species_numbers[i];
+  if (species_numbers && total_numbers == n_sites) {
+    for (int i = 0; i < S; ++i) {
+      count[i] = species_numbers[i];
+      conc[i + 1] = (double)((float)count[i] / (float)n_sites);
+    }
+  } else {
+    if (!concentrations) {
+      err = "species numbers do not sum to the number of sites and no concentrations given";
+      return BWL_ERR_ARG;
+    }
+    int64_t total = 0;
+    for (int i = 0; i < S; ++i) {
+      conc[i + 1] = concentrations[i];
+      count[i] = (int64_t)llround((double)(float)n_sites * concentrations[i]);
+      total += count[i];
+    }
+    int64_t pending = total - n_sites;
+    const int64_t step = pending > 0 ? -1 : 1;
+    int guard = 0;
+    while (pending != 0) {
+      bool moved = false;
+      for (int j = 0; j < S && pending != 0; ++j) {
+        if (count[j] == 0) continue;
+        count[j] += step;
+        pending += step;
+        moved = true;
+      }
+      if (!moved || ++guard > (1 << 24)) {
+        err = "cannot correct species counts";
+        return BWL_ERR_ARG;
+      }
+    }
+  }
+  int64_t sum = std::accumulate(count.begin(), count.end(), (int64_t)0);
+  if (sum != n_sites) {
+    err = "Species counts do not sum to the number of lattice sites";
+    return BWL_ERR_ARG;
+  }
+  for (int i = 0; i < S; ++i)
+    if (count[i] < 0) {
+      err = "negative species count";
+      return BWL_ERR_ARG;
+    }
+  // bounds sum(conc(0:l)), accumulated from element 0 each time
+  std::vector<double> bound(S + 1);
+  for (int l = 0; l <= S; ++l) {
+    double acc = 0.0;
+    for (int q = 0; q <= l; ++q) acc = acc + conc[q];
+    bound[l] = acc;
+  }
+  const size_t cells = (size_t)g.m[0] * g.m[1] * g.m[2];
+  memset(config, 0, cells);
+  auto fill = [&](size_t idx) -> bool {
+    long long draws = 0;
+    while (config[idx] == 0) {
+      double r = mt.genrand();
+      for (int l = 1; l <= S; ++l) {
+        if (r >= bound[l - 1] && r <= bound[l] && placed[l - 1] < count[l - 1]) {
+          config[idx] = (int8_t)l;
+          placed[l - 1]++;
+        }
+      }
+      if (++draws > (1LL << 40)) return false;
+    }
+    return true;
+  };
+  const int m0 = g.m[0], m1 = g.m[1], m2 = g.m[2];
+  for (int k = 1; k <= m2; ++k) {
+    if (g.lattice == BWL_BCC) {
+      // sites (2i - mod(k,2), 2j - mod(k,2), k), initialise.F90:540-570
+      const int sh = k & 1;
+      for (int j = 1; j <= m1 / 2; ++j)
+        for (int i = 1; i <= m0 / 2; ++i) {
+          size_t idx = (size_t)(2 * i - sh - 1) + (size_t)m0 * ((2 * j - sh - 1) + (size_t)m1 * (k - 1));
+          if (!fill(idx)) { err = "initial_setup did not terminate"; return BWL_ERR_ARG; }
+        }
+    } else {
+      // fcc, initialise.F90:572-611
+      for (int j = 1; j <= m1; ++j)
+        for (int i = 1; i <= m0 / 2; ++i) {
+          int x = 2 * i - (k & 1) * (j & 1) - ((k + 1) & 1) * ((j + 1) & 1);
+          size_t idx = (size_t)(x - 1) + (size_t)m0 * ((j - 1) + (size_t)m1 * (k - 1));
+          if (!fill(idx)) { err = "initial_setup did not terminate"; return BWL_ERR_ARG; }
+        }
+    }
+  }
+  return BWL_OK;
+}
+
+// ---------------------------------------------------------------------------
+int plan_parallel(const Geo &g, ParPlan &p, std::string &err) {
+  const int need = g.rcut + 1;
+  const int s = (need % 2) ? need + 1 : need;  // smallest even >= rcut+1
+  int A = 1;
+  for (int a = 0; a < 3; ++a) {
+    const int m = g.m[a];
+    if (m < need) {
+      err = "lattice too small for simultaneous swaps (extent below interaction range)";
+      return BWL_ERR_UNSUPPORTED;
+    }
+    int K = (m % s == 0) ? m / s : (m - need) / s + 1;
+    if (K < 1) K = 1;
+    p.K[a] = K;
+    p.stride[a] = s;
+    A *= K;
+  }
+  if (A < 2) {
+    err = "lattice too small for simultaneous swaps (fewer than two active sites)";
+    return BWL_ERR_UNSUPPORTED;
+  }
+  p.A = A;
+  p.npairs = A / 2;
+  p.phases_per_sweep = (g.n_atoms + p.npairs - 1) / p.npairs;
+  p.muls.clear();
+  // multipliers coprime with A, spread over [1, A)
+  uint64_t x = 0x9E3779B97F4A7C15ull;
+  for (int tries = 0; tries < 4096 && p.muls.size() < 64; ++tries) {
+    x = x * 6364136223846793005ull + 1442695040888963407ull;
+    uint32_t c = (uint32_t)((x >> 33) % (uint64_t)A);
+    if (c == 0) continue;
+    if (std::gcd(c, (uint32_t)A) != 1) continue;
+    if (std::find(p.muls.begin(), p.muls.end(), c) != p.muls.end()) continue;
+    p.muls.push_back(c);
+  }
+  if (p.muls.empty()) p.muls.push_back(1);
+  return BWL_OK;
+}
+
+int check_config(const Geo &g, const int8_t *config, std::string &err) {
+  const int m0 = g.m[0], m1 = g.m[1], m2 = g.m[2];
+  for (int z = 0; z < m2; ++z)
+    for (int y = 0; y < m1; ++y)
+      for (int x = 0; x < m0; ++x) {
+        int v = config[x + (size_t)m0 * (y + (size_t)m1 * z)];
+        bool occ = g.lattice == BWL_FCC ? (((x + y + z) & 1) == 0)
+                                        : ((x & 1) == (y & 1) && (y & 1) == (z & 1));
+        if (occ ? (v < 1 || v > g.S) : (v != 0)) {
+          err = "config holds an invalid occupation at (" + std::to_string(x + 1) + "," +
+                std::to_string(y + 1) + "," + std::to_string(z + 1) + ")";
+          return BWL_ERR_ARG;
+        }
+      }
+  return BWL_OK;
+}
+
+}  // namespace bwl

@@ -1,0 +1,80 @@
+// bwl_kernels.h -- kernel argument blocks and launchers (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "bwl_core.h"
+
+namespace bwl {
+
+struct HistArgs {
+  Geo g;
+  const int8_t *compact;  // lattice, compact layout
+  const int32_t *off;     // [nsub][z] padded-layout neighbour offsets
+  long long *hist;        // H(S,S,R), Fortran layout
+};
+
+struct SiteArgs {
+  Geo g;
+  const int8_t *compact;
+  const int32_t *off;
+  const double *vex;  // V_ex(S,S,R)
+  int x[2], y[2], z[2];  // 0-based doubled coordinates of two sites
+  double *out;           // out[0] = e(site 0), out[1] = e(site 0) + e(site 1)
+};
+
+struct SeqArgs {
+  Geo g;
+  int8_t *compact;
+  const int32_t *off;
+  const double *vex;
+  uint32_t *mt;        // 624 words, in/out
+  int32_t *mt_index;   // in/out
+  double beta;
+  int32_t nbr_swap;
+  long long n_trials;
+  long long sample_every;
+  double *energies;    // optional, n_trials / sample_every values
+  long long *accepted;
+};
+
+struct ParArgs {
+  Geo g;
+  int8_t *compact;       // in/out lattice
+  const int32_t *off;
+  const double *vex;
+  // active-site grid of a round
+  int32_t K[3];          // active positions per axis
+  int32_t stride[3];     // even strides, doubled-grid units
+  int32_t A;             // K0*K1*K2
+  int32_t npairs;        // A/2 pair trials per round
+  int32_t phases_per_sweep;
+  int32_t n_mul;
+  const uint32_t *muls;  // multipliers coprime with A
+  // schedule
+  double T0, dT;
+  int32_t T_steps, n_sweeps, burn_sweeps;
+  uint32_t seed_lo, seed_hi, chain_id;
+  unsigned long long phase0;  // first round counter (resumable streams)
+  // outputs
+  double *energies_of_T, *C_of_T, *acc_of_T;
+  double *e_trace;       // optional [T_steps][n_sweeps]
+  long long *counters;   // [0] = next round counter, [1] += accepted
+};
+
+hipError_t launch_pack(const int8_t *ref, int8_t *compact, const Geo &g, hipStream_t s);
+hipError_t launch_unpack(const int8_t *compact, int8_t *ref, const Geo &g, hipStream_t s);
+size_t hist_lds_bytes(const Geo &g, int threads);
+hipError_t launch_histogram(const HistArgs *args_dev, int n, const Geo &g_max, int threads,
+                            hipStream_t s);
+size_t site_lds_bytes(const Geo &g);
+hipError_t launch_site_energy(const SiteArgs &a, hipStream_t s);
+size_t seq_lds_bytes(const Geo &g);
+hipError_t launch_metro_seq(const SeqArgs &a, hipStream_t s);
+size_t par_lds_bytes(const Geo &g, int threads);
+hipError_t launch_metro_par(const ParArgs *args_dev, int n_chains, size_t lds, int threads,
+                            bool wide, hipStream_t s);
+
+}  // namespace bwl

@@ -1,0 +1,185 @@
+"""Host-side mirror of the reference's energy and sampler interface, over the
+HIP C-ABI.
+
+`Lattice` plays the role of one `run_params` plus its module-global
+`config` and `V_ex` (derived_types.f90:56-100, shared_data.f90:30-37).  Its
+methods carry the reference's procedure names:
+
+    full_energy / total_energy   setup%full_energy   (bw_hamiltonian.f90:58-81)
+    nbr_energy                    setup%nbr_energy    (bw_hamiltonian.f90:898-1910)
+    pair_energy                   pair_energy         (bw_hamiltonian.f90:99-114)
+    initial_setup                 initial_setup       (initialise.F90:434-617)
+    mc_step / mc_steps            setup%mc_step       (metropolis.F90:751-891)
+    metropolis_simulated_annealing  the temperature loop (metropolis.F90:193-437)
+
+Arrays use the reference's Fortran layouts (config (2n1,2n2,2n3) int8,
+V_ex (S,S,R) float64, both order='F').  Errors raise BrawlError with the
+library's status code; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import AnnealParams, BrawlError, LatticeDesc, RunStats, check
+
+FCC, BCC = _lib.FCC, _lib.BCC
+K_B_IN_RY = 8.167333262e-5 / 13.605693122990   # constants.f90:45-46 (typo kept)
+RY_TO_EV = 13.605693122                        # constants.f90:49
+LATTICE_NAMES = {"fcc": FCC, "bcc": BCC}
+
+
+def beta_of(temp: float) -> float:
+    """beta = 1/(T k_B) as metropolis.F90:204-206 computes it."""
+    return 1.0 / (temp * K_B_IN_RY)
+
+
+def _p(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+@dataclass
+class AnnealResult:
+    energies_of_T: np.ndarray   # <E>/N per T (Ry/atom)
+    C_of_T: np.ndarray          # heat capacity per atom
+    acceptance_of_T: np.ndarray
+    trials: int
+    accepted: int
+    kernel_ms: float
+
+
+class Lattice:
+    def __init__(self, lattice, n, v_ex):
+        if isinstance(lattice, str):
+            lattice = LATTICE_NAMES[lattice]
+        v = np.asfortranarray(np.asarray(v_ex, dtype=np.float64))
+        if v.ndim != 3 or v.shape[0] != v.shape[1]:
+            raise ValueError("v_ex must have shape (S, S, R)")
+        self.v_ex = v
+        self.S, _, self.R = v.shape
+        self.n = tuple(int(x) for x in n)
+        self.lattice = int(lattice)
+        self.grid = tuple(2 * x for x in self.n)
+        L = _lib.load()
+        d = LatticeDesc(self.lattice, self.n[0], self.n[1], self.n[2], self.S, self.R)
+        h = C.c_void_p()
+        check(L.bwl_create(C.byref(d), _p(self.v_ex, C.c_double), C.byref(h)))
+        self._h = h
+        self._L = L
+        na = C.c_int32()
+        check(L.bwl_n_atoms(h, C.byref(na)))
+        self.n_atoms = int(na.value)
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.bwl_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- configuration ----------------------------------------------------
+    def _cfg(self, cfg):
+        cfg = np.asfortranarray(cfg, dtype=np.int8)
+        if cfg.shape != self.grid:
+            raise ValueError(f"config must have shape {self.grid}")
+        return cfg
+
+    def set_config(self, cfg):
+        cfg = self._cfg(cfg)
+        check(self._L.bwl_set_config(self._h, _p(cfg, C.c_int8)))
+
+    def get_config(self) -> np.ndarray:
+        out = np.zeros(self.grid, dtype=np.int8, order="F")
+        check(self._L.bwl_get_config(self._h, _p(out, C.c_int8)))
+        return out
+
+    # -- Hamiltonian ------------------------------------------------------
+    def pair_histogram(self) -> np.ndarray:
+        h = np.zeros((self.S, self.S, self.R), dtype=np.int64, order="F")
+        check(self._L.bwl_pair_histogram(self._h, _p(h, C.c_int64)))
+        return h
+
+    def total_energy(self) -> float:
+        e = C.c_double()
+        check(self._L.bwl_total_energy(self._h, C.byref(e)))
+        return e.value
+
+    full_energy = total_energy
+
+    def nbr_energy(self, site_b, site_i, site_j, site_k) -> float:
+        e = C.c_double()
+        check(self._L.bwl_site_energy(self._h, site_b, site_i, site_j, site_k, C.byref(e)))
+        return e.value
+
+    def pair_energy(self, idx1, idx2) -> float:
+        a = np.ascontiguousarray(idx1, dtype=np.int32)
+        b = np.ascontiguousarray(idx2, dtype=np.int32)
+        e = C.c_double()
+        check(self._L.bwl_pair_energy(self._h, _p(a, C.c_int32), _p(b, C.c_int32), C.byref(e)))
+        return e.value
+
+    # -- reference random stream ------------------------------------------
+    def seed(self, seed: int) -> None:
+        check(self._L.bwl_genrand_seed(self._h, seed))
+
+    def genrand_state(self):
+        mt = np.zeros(624, dtype=np.uint32)
+        mti = C.c_int32()
+        check(self._L.bwl_genrand_get_state(self._h, _p(mt, C.c_uint32), C.byref(mti)))
+        return mt, int(mti.value)
+
+    def set_genrand_state(self, mt, mti) -> None:
+        mt = np.ascontiguousarray(mt, dtype=np.uint32)
+        check(self._L.bwl_genrand_set_state(self._h, _p(mt, C.c_uint32), int(mti)))
+
+    def initial_setup(self, species_numbers=None, concentrations=None) -> np.ndarray:
+        sn = None if species_numbers is None else np.ascontiguousarray(species_numbers, dtype=np.int64)
+        cc = None if concentrations is None else np.ascontiguousarray(concentrations, dtype=np.float64)
+        check(self._L.bwl_initial_setup(self._h, None if sn is None else _p(sn, C.c_int64),
+                                        None if cc is None else _p(cc, C.c_double)))
+        return self.get_config()
+
+    # -- sequential (bit-exact) sampler -------------------------------------
+    def mc_steps(self, beta: float, n_trials: int, *, nbr_swap=False, sample_every=0):
+        """n_trials reference trials; returns (accepted, energies or None)."""
+        acc = C.c_int64()
+        energies = None
+        ep = None
+        if sample_every:
+            energies = np.zeros(n_trials // sample_every)
+            ep = _p(energies, C.c_double)
+        check(self._L.bwl_mc_steps(self._h, beta, int(nbr_swap), n_trials, sample_every,
+                                   ep, C.byref(acc)))
+        return int(acc.value), energies
+
+    def mc_step(self, beta: float, nbr_swap=False) -> int:
+        return self.mc_steps(beta, 1, nbr_swap=nbr_swap)[0]
+
+    # -- parallel sampler -------------------------------------------------
+    def metropolis_simulated_annealing(self, *, T, delta_T=0.0, T_steps=1, n_sweeps,
+                                       burn_in_sweeps=0, seed=20261202) -> AnnealResult:
+        p = AnnealParams(float(T), float(delta_T), int(T_steps), int(n_sweeps),
+                         int(burn_in_sweeps), 0, int(seed) & (2**64 - 1))
+        e = np.zeros(T_steps); c = np.zeros(T_steps); a = np.zeros(T_steps)
+        st = RunStats()
+        check(self._L.bwl_metropolis_anneal(self._h, C.byref(p), _p(e, C.c_double),
+                                            _p(c, C.c_double), _p(a, C.c_double), C.byref(st)))
+        return AnnealResult(e, c, a, int(st.trials), int(st.accepted), float(st.kernel_ms))
+
+
+__all__ = ["Lattice", "BrawlError", "AnnealResult", "beta_of", "FCC", "BCC", "K_B_IN_RY",
+           "RY_TO_EV"]

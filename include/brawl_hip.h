@@ -1,0 +1,150 @@
+/*
+ * brawl_hip.h -- C-ABI of the MI355X (gfx950) lattice-sampling library.
+ *
+ * Plain C types only (no torch, no HIP types).  Every entry point returns an
+ * int status (BWL_OK == 0); bwl_last_error() gives the message of the last
+ * failure on the calling thread.  A context (bwl_ctx) is one lattice chain
+ * with its device-resident state and its own HIP stream; contexts are not
+ * re-entrant (one host thread per context at a time).
+ *
+ * Layouts at the boundary are the reference's own:
+ *   config  int8 config(1, 2*n1, 2*n2, 2*n3), Fortran column-major
+ *           (shared_data.f90:30, allocated at initialise.F90:295);
+ *           element (1,i,j,k), 1-based, at (i-1) + 2*n1*((j-1) + 2*n2*(k-1)).
+ *   v_ex    FP64 V_ex(S, S, R), Fortran column-major, Rydberg
+ *           (shared_data.f90:37, read at io.f90:399-402).
+ *   hist    int64 H(S, S, R), same layout: ordered (site, listed neighbour)
+ *           pairs with species (a, b) on shell n.
+ *
+ * Each declaration names the reference interface it replaces.
+ */
+#ifndef BRAWL_HIP_H
+#define BRAWL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BWL_OK 0
+#define BWL_ERR_ARG (-1)         /* invalid argument or descriptor */
+#define BWL_ERR_UNSUPPORTED (-2) /* lattice/shell/species combination not built */
+#define BWL_ERR_DEVICE (-3)      /* HIP runtime error */
+#define BWL_ERR_NOMEM (-4)       /* device allocation failed */
+#define BWL_ERR_STATE (-5)       /* call order (e.g. no configuration uploaded) */
+
+#define BWL_FCC 1
+#define BWL_BCC 2
+
+/* Lattice descriptor: the fields of run_params the path reads
+ * (derived_types.f90:56-100: lattice, n_1..n_3, n_species,
+ * interaction_range). */
+typedef struct {
+  int32_t lattice;   /* BWL_FCC or BWL_BCC */
+  int32_t n1, n2, n3; /* conventional cells per axis */
+  int32_t n_species;  /* S, 1..8 */
+  int32_t n_shells;   /* R: fcc 1..6, bcc 1..10 */
+} bwl_lattice_desc;
+
+typedef struct bwl_ctx bwl_ctx;
+
+/* ---------------------------------------------------------------- */
+/* set-up                                                            */
+/* ---------------------------------------------------------------- */
+const char *bwl_last_error(void);
+int bwl_device_count(int32_t *count);
+
+/* Replaces initialise_interaction + initialise_function_pointers +
+ * initialise_local_arrays (initialise.F90:153-298): allocates the device
+ * lattice for `desc` and uploads V_ex. */
+int bwl_create(const bwl_lattice_desc *desc, const double *v_ex, bwl_ctx **out);
+int bwl_destroy(bwl_ctx *ctx);
+/* Number of occupied sites (run_params%n_atoms, initialise.F90:174-212) */
+int bwl_n_atoms(const bwl_ctx *ctx, int32_t *n_atoms);
+/* Upload / download the reference-layout config (config array,
+ * shared_data.f90:30). */
+int bwl_set_config(bwl_ctx *ctx, const int8_t *config);
+int bwl_get_config(bwl_ctx *ctx, int8_t *config);
+
+/* ---------------------------------------------------------------- */
+/* Hamiltonian (bw_hamiltonian.f90)                                  */
+/* ---------------------------------------------------------------- */
+/* Integer pair-count histogram over the reference neighbour lists,
+ * counted on the device. */
+int bwl_pair_histogram(bwl_ctx *ctx, int64_t *hist);
+/* `hamiltonian` interface, bound as setup%full_energy (derived_types.f90:
+ * 373-384; total_energy, bw_hamiltonian.f90:58-81): 0.5*sum H*V_ex
+ * contracted in FP64 from the device histogram. */
+int bwl_total_energy(bwl_ctx *ctx, double *energy);
+/* `neighbour` interface, bound as setup%nbr_energy (derived_types.f90:
+ * 401-413; {fcc,bcc}_energy_Rshells, bw_hamiltonian.f90:898-1910).
+ * Evaluated on the device in the reference's FP64 summation order, so the
+ * result is bit-identical.  site_b/i/j/k are 1-based. */
+int bwl_site_energy(bwl_ctx *ctx, int32_t site_b, int32_t site_i,
+                    int32_t site_j, int32_t site_k, double *energy);
+/* pair_energy (bw_hamiltonian.f90:99-114); sites are int(4) (b,i,j,k). */
+int bwl_pair_energy(bwl_ctx *ctx, const int32_t *site1, const int32_t *site2,
+                    double *energy);
+
+/* ---------------------------------------------------------------- */
+/* Reference random stream (mt19937ar.c:58-142) -- sequential mode   */
+/* ---------------------------------------------------------------- */
+/* init_genrand(seed) on the context's MT19937 state. */
+int bwl_genrand_seed(bwl_ctx *ctx, uint32_t seed);
+/* Copy the 624-word state and index (the globals mt[], mti of
+ * mt19937ar.c:58-59) in or out, so a host that also draws numbers stays in
+ * step with the device. */
+int bwl_genrand_get_state(bwl_ctx *ctx, uint32_t *mt, int32_t *mti);
+int bwl_genrand_set_state(bwl_ctx *ctx, const uint32_t *mt, int32_t mti);
+/* initial_setup (initialise.F90:434-617) from the context's stream: uses
+ * species_numbers (length S) when they sum to n_atoms, else
+ * concentrations (length S).  Result is uploaded as the configuration. */
+int bwl_initial_setup(bwl_ctx *ctx, const int64_t *species_numbers,
+                      const double *concentrations);
+
+/* `monte_carlo` interface, bound as setup%mc_step (derived_types.f90:
+ * 474-486; monte_carlo_step_lattice metropolis.F90:751-813, or
+ * monte_carlo_step_nbr :829-891 when nbr_swap != 0): n_trials sequential
+ * trials on the device consuming the context's MT19937 stream exactly as
+ * the reference does.  accepted counts trials returning 1.  If energies is
+ * not NULL, the total energy is evaluated after every `sample_every`
+ * trials (metropolis.F90:357-368) and written there
+ * (n_trials / sample_every values). */
+int bwl_mc_steps(bwl_ctx *ctx, double beta, int32_t nbr_swap, int64_t n_trials,
+                 int64_t sample_every, double *energies, int64_t *accepted);
+
+/* ---------------------------------------------------------------- */
+/* Parallel Metropolis sampler (statistical parity)                  */
+/* ---------------------------------------------------------------- */
+/* Replaces the temperature loop of metropolis_simulated_annealing
+ * (metropolis.F90:193-437) for lattice-wide swaps.  Each sweep makes at
+ * least n_atoms swap trials as rounds of simultaneous, non-interacting
+ * pair trials (sites of a round are further apart than the interaction
+ * range); proposals and acceptances use Philox4x32-10 keyed by `seed`.
+ * The energy is recounted after every sweep; outputs per T step follow
+ * metropolis.F90:414-437 (<E>/N, C, acceptance rate). */
+typedef struct {
+  double T;            /* start temperature (K) */
+  double delta_T;      /* step (K) */
+  int32_t T_steps;
+  int32_t n_sweeps;    /* sweeps per T step (samples per T step) */
+  int32_t burn_in_sweeps; /* unsampled sweeps before each T step's samples */
+  int32_t reserved;
+  uint64_t seed;
+} bwl_anneal_params;
+
+typedef struct {
+  int64_t trials;        /* swap trials made (all T steps) */
+  int64_t accepted;      /* accepted, equal species counted as accepted */
+  double kernel_ms;      /* device time of the sampler kernel */
+} bwl_run_stats;
+
+int bwl_metropolis_anneal(bwl_ctx *ctx, const bwl_anneal_params *p,
+                          double *energies_of_T, double *C_of_T,
+                          double *acceptance_of_T, bwl_run_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

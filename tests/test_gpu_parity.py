@@ -1,0 +1,228 @@
+"""HIP path vs the oracle, through the C-ABI.  Integer work (histograms,
+configurations, accepted counts) must be bit-exact; energies summed in the
+reference's order must be bit-exact; energies contracted from integer
+counts must agree with the oracle's reference-order sum to 1e-12 relative
+(or the reference sum's own rounding bound, whichever is larger)."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def hip():
+    import brawl_amd
+    from brawl_amd import _lib
+    _lib.load()
+    if _lib.device_count() < 1:
+        pytest.fail("no GPU visible to the HIP library")
+    return brawl_amd
+
+
+def _energy_tol(L, cfg, e_ref):
+    """1e-12 relative, or the rounding noise of the reference's own
+    left-to-right sum of N*z terms (~ N*z*eps*max|V|) when that is larger."""
+    z = sum(oracle_shell_sizes(L))
+    noise = 8.0 * L.n_atoms * z * np.abs(L.v_ex).max() * 1.1e-16
+    return max(1e-12 * abs(e_ref), noise)
+
+
+def oracle_shell_sizes(L):
+    from oracle import oracle as orc
+    return [orc.lib().bwo_shell_size(L.lattice, s) for s in range(1, L.R + 1)]
+
+
+CASES = [
+    # lattice, n, S, R
+    (1, (4, 4, 4), 5, 6),
+    (2, (4, 4, 4), 4, 6),
+    (1, (3, 3, 3), 5, 4),
+    (1, (8, 8, 8), 2, 2),
+    (2, (5, 3, 4), 3, 10),
+    (1, (2, 3, 5), 8, 5),
+    (2, (1, 2, 2), 2, 3),
+    (2, (16, 16, 16), 4, 2),
+    (2, (32, 32, 32), 4, 4),
+    (1, (24, 24, 24), 5, 3),
+]
+
+
+@pytest.mark.parametrize("lat,n,S,R", CASES)
+def test_histogram_energy_and_layout(hip, oracle, lat, n, S, R):
+    from brawl_amd import workloads
+    _say("case", lat, n, S, R)
+    v = workloads.random_v(S, [20.0 / k for k in range(1, R + 1)], 17 + R)
+    ref = oracle.OracleLattice(lat, n, v)
+    N = ref.n_atoms
+    cfg = ref.initial_setup(oracle.Generator(110179),
+                            species_numbers=workloads.dirichlet_counts(N, S, np.random.default_rng(N))
+                            if N >= S else None,
+                            concentrations=np.full(S, 1.0 / S))
+    with hip.Lattice(lat, n, v) as L:
+        L.set_config(cfg)
+        assert np.array_equal(L.get_config(), cfg)
+        h = L.pair_histogram()
+        assert np.array_equal(h, ref.histogram(cfg))
+        e = L.total_energy()
+        e_hist = ref.energy_from_histogram(h)
+        assert e == e_hist                        # same integers, same contraction order
+        e_ref = ref.total_energy(cfg)             # reference order
+        assert abs(e - e_ref) <= _energy_tol(ref, cfg, e_ref)
+
+
+@pytest.mark.parametrize("lat,n,S,R", CASES[:7])
+def test_site_and_pair_energy_bit_exact(hip, oracle, lat, n, S, R):
+    from brawl_amd import workloads
+    _say("case", lat, n, S, R)
+    v = workloads.random_v(S, [20.0 / k for k in range(1, R + 1)], 5 + R)
+    ref = oracle.OracleLattice(lat, n, v)
+    gen = oracle.Generator(7)
+    cfg = ref.initial_setup(gen, concentrations=np.full(S, 1.0 / S))
+    with hip.Lattice(lat, n, v) as L:
+        L.set_config(cfg)
+        for _ in range(40):
+            a = ref.random_site(gen)
+            b = ref.random_site(gen)
+            assert L.nbr_energy(1, *a) == ref.site_energy(cfg, *a)
+            assert L.pair_energy((1,) + a, (1,) + b) == ref.pair_energy(cfg, a, b)
+
+
+@pytest.mark.parametrize("lat,n,S", [(1, (4, 4, 4), 5), (2, (4, 4, 4), 4), (1, (5, 3, 4), 3),
+                                     (2, (8, 8, 8), 4)])
+def test_initial_setup_bit_exact(hip, oracle, lat, n, S):
+    from brawl_amd import workloads
+    v = workloads.random_v(S, [10.0], 3)
+    ref = oracle.OracleLattice(lat, n, v)
+    N = ref.n_atoms
+    for kind in ("numbers", "conc"):
+        gen = oracle.Generator(110179 + 11)
+        kw = (dict(species_numbers=workloads.dirichlet_counts(N, S, np.random.default_rng(1)))
+              if kind == "numbers" else dict(concentrations=np.full(S, 1.0 / S)))
+        cfg = ref.initial_setup(gen, **kw)
+        with hip.Lattice(lat, n, v) as L:
+            L.seed(110179 + 11)
+            got = L.initial_setup(**kw)
+            assert np.array_equal(got, cfg)
+            mt, mti = L.genrand_state()
+            mt_ref, mti_ref = gen.state_arrays()
+            assert mti == mti_ref and np.array_equal(mt, mt_ref)
+
+
+def test_sequential_mode_reproduces_reference_golden_fcc(hip, oracle):
+    """tests/99_ref/01 through the device: the 256-trial energy trajectory
+    (f20.10), the final configuration and the MT19937 stream position."""
+    g = load_golden("ref01_serial_metropolis.npz")
+    with hip.Lattice(int(g["lattice"]), g["n"], g["v_ex"]) as L:
+        L.seed(int(g["seed"]))
+        cfg0 = L.initial_setup(species_numbers=g["species_numbers"])
+        assert np.array_equal(cfg0, g["initial_config"])
+        e0 = L.total_energy()
+        acc, traj = L.mc_steps(hip.beta_of(float(g["T"])), int(g["n_mc_steps"]), sample_every=1)
+        energies = np.concatenate([[e0], traj])
+        ours = ["%20.10f" % e for e in energies]
+        gold = ["%20.10f" % e for e in g["traj_energy"]]
+        assert ours == gold
+        assert np.array_equal(L.get_config(), g["final_config"])
+
+
+@pytest.mark.parametrize("rank", [0, 1, 2, 3])
+def test_sequential_mode_reproduces_reference_golden_bcc(hip, oracle, rank):
+    g = load_golden("ref02_parallel_metropolis.npz")
+    with hip.Lattice(int(g["lattice"]), g["n"], g["v_ex"]) as L:
+        L.seed(int(g[f"seed{rank}"]))
+        cfg0 = L.initial_setup(concentrations=g["concentrations"])
+        assert np.array_equal(cfg0, g[f"initial_config{rank}"])
+        e0 = L.total_energy()
+        acc, traj = L.mc_steps(hip.beta_of(300.0), int(g["n_mc_steps"]), sample_every=1)
+        energies = np.concatenate([[e0], traj])
+        assert ["%20.10f" % e for e in energies] == ["%20.10f" % e for e in g[f"traj_energy{rank}"]]
+        assert np.array_equal(L.get_config(), g[f"final_config{rank}"])
+        assert "%6.4f" % (acc / 128.0) == "%6.4f" % g[f"diagnostics{rank}"][3]
+
+
+@pytest.mark.parametrize("nbr_swap", [False, True])
+def test_sequential_mode_bit_exact_config1(hip, oracle, nbr_swap):
+    """The small config (fcc 8^3, 2 species, 2 shells, 600 K): 3 sweeps of
+    2048 trials each, configuration, accepted count and RNG position
+    identical to the oracle."""
+    from brawl_amd import workloads
+    c = workloads.config1()
+    ref = oracle.OracleLattice(c.lattice, c.n, c.v_ex)
+    gen = oracle.Generator(110179)
+    cfg = ref.initial_setup(gen, species_numbers=c.counts)
+    beta = hip.beta_of(c.T)
+    with hip.Lattice(c.lattice, c.n, c.v_ex) as L:
+        L.seed(110179)
+        L.initial_setup(species_numbers=c.counts)
+        for sweep in range(3):
+            acc, e = L.mc_steps(beta, c.n_atoms, nbr_swap=nbr_swap, sample_every=c.n_atoms)
+            acc_ref = sum(ref.mc_step(cfg, beta, gen, nbr_swap) for _ in range(c.n_atoms))
+            assert acc == acc_ref
+            assert np.array_equal(L.get_config(), cfg)
+            e_ref = ref.total_energy(cfg)
+            assert abs(e[0] - e_ref) <= 1e-12 * max(1.0, abs(e_ref))
+        mt, mti = L.genrand_state()
+        mt_ref, mti_ref = gen.state_arrays()
+        assert mti == mti_ref and np.array_equal(mt, mt_ref)
+
+
+def test_parallel_sampler_conserves_composition_and_energy(hip, oracle):
+    """Full-size config 2 lattice (bcc 32^3, 65536 sites): a short anneal
+    keeps species counts, and the last recounted energy equals the oracle's
+    energy of the downloaded configuration."""
+    from brawl_amd import workloads
+    c = workloads.config2()
+    ref = oracle.OracleLattice(c.lattice, c.n, c.v_ex)
+    cfg = ref.initial_setup(oracle.Generator(110179), species_numbers=c.counts)
+    with hip.Lattice(c.lattice, c.n, c.v_ex) as L:
+        L.set_config(cfg)
+        res = L.metropolis_simulated_annealing(T=1500.0, n_sweeps=1, seed=5)
+        out = L.get_config()
+        assert np.array_equal(np.bincount(out.ravel(), minlength=5), np.bincount(cfg.ravel(), minlength=5))
+        assert np.array_equal((out > 0), (cfg > 0))
+        e_gpu = res.energies_of_T[0] * c.n_atoms       # one sample: <E> = E_last
+        e_ref = ref.total_energy(out)
+        assert abs(e_gpu - e_ref) <= 1e-11 * max(1.0, abs(e_ref))
+        h = L.pair_histogram()
+        assert np.array_equal(h, ref.histogram(out))
+        assert res.trials >= c.n_atoms
+        assert 0 < res.accepted <= res.trials
+
+
+def test_parallel_sampler_statistics_match_reference_sampler(hip, oracle):
+    """The small config at 600 K: <E>/N from the parallel sampler agrees with
+    the reference's sequential sampler (the oracle) within 2 sigma of the
+    difference, both equilibrated from the same start."""
+    from brawl_amd import workloads
+    c = workloads.config1()
+    ref = oracle.OracleLattice(c.lattice, c.n, c.v_ex)
+    beta = hip.beta_of(c.T)
+    # reference sampler (oracle): 6 chains x (200 burn-in + 300 sampled sweeps)
+    e_ref = []
+    for k in range(6):
+        gen = oracle.Generator(110179 + 11 * k)
+        cfg = ref.initial_setup(gen, species_numbers=c.counts)
+        r = ref.anneal(cfg, gen, T=c.T, n_mc_steps=300 * c.n_atoms, n_sample_steps=c.n_atoms,
+                       burn_in_start=True, n_burn_in_steps=200 * c.n_atoms)
+        e_ref.append(r["energies"][0])
+    e_gpu = []
+    for k in range(8):
+        cfg = ref.initial_setup(oracle.Generator(999 + k), species_numbers=c.counts)
+        with hip.Lattice(c.lattice, c.n, c.v_ex) as L:
+            L.set_config(cfg)
+            r = L.metropolis_simulated_annealing(T=c.T, n_sweeps=300, burn_in_sweeps=200, seed=k + 1)
+            e_gpu.append(r.energies_of_T[0])
+    e_ref, e_gpu = np.array(e_ref), np.array(e_gpu)
+    diff = e_gpu.mean() - e_ref.mean()
+    sigma = np.sqrt(e_gpu.var(ddof=1) / len(e_gpu) + e_ref.var(ddof=1) / len(e_ref))
+    _say("<E>/N gpu", e_gpu.mean(), "ref", e_ref.mean(), "diff/sigma", diff / sigma)
+    assert abs(diff) <= 2.0 * sigma + 1e-9

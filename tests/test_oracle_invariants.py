@@ -1,0 +1,211 @@
+"""Checks that pin the oracle independently of its own formulas: published
+known answers, brute-force enumeration, and invariants a wrong sign, index or
+table entry would break.  CPU only."""
+import itertools
+
+import numpy as np
+import pytest
+
+from brawl_amd import workloads
+
+# coordination shells by squared distance (doubled-grid units): these are the
+# complete lattice shells, which the reference's lists reproduce for fcc 1-6
+# and bcc 1-7, 9
+FCC_NORMS = [2, 4, 6, 8, 10, 12]
+BCC_NORMS = [3, 4, 8, 11, 12, 16, 19, 20, 24, 27]
+FCC_Z = [12, 6, 24, 12, 24, 8]
+BCC_Z = [8, 6, 12, 24, 8, 6, 24, 24, 24, 32]
+
+
+def test_mt19937_known_answers(oracle):
+    g = oracle.Generator(5489)
+    first = g.u32()
+    assert first == 3499211612           # C++11 [rand.predef]: mt19937 first output
+    for _ in range(9998):
+        g.u32()
+    assert g.u32() == 4123659995         # C++11: 10000th output of default mt19937
+
+
+def test_mt19937_matches_numpy_legacy_seeding(oracle):
+    # numpy's RandomState seeds with init_genrand for 32-bit integer seeds
+    for seed in (110179, 110190, 12345):
+        rs = np.random.RandomState(seed)
+        want = rs.randint(0, 2**32, size=2000, dtype=np.uint64)
+        g = oracle.Generator(seed)
+        got = np.array([g.u32() for _ in range(2000)], dtype=np.uint64)
+        assert np.array_equal(got, want)
+    assert oracle.lib().bwo_static_seed(3) == 110179 + 33
+
+
+def _sites(lat, n):
+    m = [2 * x for x in n]
+    pts = []
+    for z in range(m[2]):
+        for y in range(m[1]):
+            for x in range(m[0]):
+                occ = ((x + y + z) % 2 == 0) if lat == 1 else (x % 2 == y % 2 == z % 2)
+                if occ:
+                    pts.append((x, y, z))
+    return np.array(pts), m
+
+
+def _brute_energy(lat, n, v, cfg):
+    """E = 1/2 sum_i sum_j V(s_i, s_j, shell(|d_ij|^2)), minimum image."""
+    pts, m = _sites(lat, n)
+    sp = np.array([cfg[x, y, z] for x, y, z in pts])
+    d = pts[:, None, :] - pts[None, :, :]
+    m = np.array(m)
+    d = d - m * np.round(d / m)
+    r2 = (d ** 2).sum(-1).astype(int)
+    norms = FCC_NORMS if lat == 1 else BCC_NORMS
+    e = 0.0
+    for s, nr in enumerate(norms[: v.shape[2]]):
+        ii, jj = np.nonzero(r2 == nr)
+        e += v[sp[ii] - 1, sp[jj] - 1, s].sum()
+    return 0.5 * e
+
+
+@pytest.mark.parametrize("lat,R,n", [(1, 6, (4, 4, 4)), (1, 3, (3, 4, 4)), (2, 7, (5, 5, 5)),
+                                     (2, 4, (4, 4, 5))])
+def test_energy_matches_brute_force_distance_shells(oracle, lat, R, n):
+    S = 3
+    v = workloads.random_v(S, [20.0 / (k + 1) for k in range(R)], 99 + R)
+    v[:, :, :] += np.random.default_rng(1).normal(0, 1e-4, v.shape)  # break V symmetry too
+    L = oracle.OracleLattice(lat, n, v)
+    N = L.n_atoms
+    cfg = L.initial_setup(oracle.Generator(5), species_numbers=workloads.equal_counts(N, S))
+    e = L.total_energy(cfg)
+    eb = _brute_energy(lat, n, v, cfg)
+    assert abs(e - eb) <= 1e-12 * max(1.0, abs(eb)) * 10
+    h = L.histogram(cfg)
+    z = FCC_Z if lat == 1 else BCC_Z
+    for s in range(R):
+        assert h[:, :, s].sum() == N * z[s]
+    assert abs(L.energy_from_histogram(h) - e) <= 1e-12 * max(1.0, abs(e))
+
+
+@pytest.mark.parametrize("lat,maxR", [(1, 6), (2, 10)])
+def test_shell_tables(oracle, lat, maxR):
+    norms = FCC_NORMS if lat == 1 else BCC_NORMS
+    z = FCC_Z if lat == 1 else BCC_Z
+    for s in range(1, maxR + 1):
+        k = oracle.lib().bwo_shell_size(lat, s)
+        assert k == z[s - 1]
+        p = oracle.lib().bwo_shell_offsets(lat, s)
+        offs = np.array([p[i] for i in range(3 * k)]).reshape(k, 3)
+        r2 = (offs ** 2).sum(1)
+        if lat == 2 and s == 8:
+            # reference quirk: entries 15-16 are (0,4,0), (-4,0,0)
+            assert sorted(r2.tolist()) == [16, 16] + [20] * 22
+            assert offs[14].tolist() == [0, 4, 0] and offs[15].tolist() == [-4, 0, 0]
+            continue
+        assert np.all(r2 == norms[s - 1])
+        if lat == 2 and s == 10:
+            # reference quirk: 4 duplicated entries, so 4 (+-1,5,+-1)-type
+            # vectors are missing
+            assert len({tuple(o) for o in offs}) == 28
+            continue
+        assert len({tuple(o) for o in offs}) == k
+        # closed under negation
+        assert {tuple(o) for o in offs} == {tuple(-o) for o in offs}
+        # every offset is a lattice vector
+        if lat == 1:
+            assert np.all(offs.sum(1) % 2 == 0)
+        else:
+            assert np.all((offs % 2 == offs[:, :1] % 2).all(1))
+
+
+def test_pair_energy_difference_equals_total_energy_change(oracle):
+    v = workloads.random_v(4, [30.0, 15.0, 10.0, 7.5], 3)
+    L = oracle.OracleLattice(2, (4, 4, 4), v)
+    gen = oracle.Generator(11)
+    cfg = L.initial_setup(gen, species_numbers=workloads.equal_counts(128, 4))
+    for _ in range(200):
+        a = L.random_site(gen)
+        b = L.random_site(gen)
+        e0 = L.total_energy(cfg)
+        p0 = L.pair_energy(cfg, a, b)
+        i, j = [x - 1 for x in a], [x - 1 for x in b]
+        cfg[i[0], i[1], i[2]], cfg[j[0], j[1], j[2]] = cfg[j[0], j[1], j[2]], cfg[i[0], i[1], i[2]]
+        p1 = L.pair_energy(cfg, a, b)
+        e1 = L.total_energy(cfg)
+        assert abs((p1 - p0) - (e1 - e0)) < 1e-12
+
+
+def test_translation_invariance(oracle):
+    v = workloads.random_v(3, [20.0, 10.0, 5.0], 4)
+    for lat, shift in ((1, (1, 1, 0)), (1, (2, 0, 0)), (2, (1, 1, 1)), (2, (0, 2, 0))):
+        L = oracle.OracleLattice(lat, (4, 4, 4), v)
+        cfg = L.initial_setup(oracle.Generator(8), species_numbers=workloads.equal_counts(L.n_atoms, 3))
+        moved = np.asfortranarray(np.roll(cfg, shift, axis=(0, 1, 2)))
+        assert np.array_equal(L.histogram(cfg), L.histogram(moved))
+
+
+def test_initial_setup_counts(oracle):
+    v = workloads.random_v(5, [10.0], 2)
+    for lat, n in ((1, (3, 4, 5)), (2, (5, 3, 4))):
+        L = oracle.OracleLattice(lat, n, v)
+        counts = workloads.dirichlet_counts(L.n_atoms, 5, np.random.default_rng(2))
+        cfg = L.initial_setup(oracle.Generator(9), species_numbers=counts)
+        got = np.bincount(cfg.ravel(), minlength=6)
+        assert np.array_equal(got[1:], counts)
+        occ = cfg > 0
+        pts, _ = _sites(lat, n)
+        assert occ.sum() == len(pts)
+        assert all(occ[x, y, z] for x, y, z in pts)
+
+
+def test_concentration_rounding(oracle):
+    """nint + round-robin correction (initialise.F90:480-504)."""
+    v = workloads.random_v(3, [10.0], 2)
+    L = oracle.OracleLattice(1, (3, 3, 3), v)   # 108 sites
+    cfg = L.initial_setup(oracle.Generator(1), concentrations=[1 / 3, 1 / 3, 1 / 3])
+    assert np.array_equal(np.bincount(cfg.ravel(), minlength=4)[1:], [36, 36, 36])
+    cfg = L.initial_setup(oracle.Generator(1), concentrations=[0.335, 0.335, 0.33])
+    # nint(108*0.335)=36, 36, nint(35.64)=36 -> 108
+    assert np.bincount(cfg.ravel(), minlength=4)[1:].sum() == 108
+
+
+def test_infinite_temperature_accepts_everything(oracle):
+    v = workloads.random_v(2, [20.0, 5.0], 1)
+    L = oracle.OracleLattice(1, (4, 4, 4), v)
+    gen = oracle.Generator(3)
+    cfg = L.initial_setup(gen, species_numbers=[128, 128])
+    assert sum(L.mc_step(cfg, 0.0, gen) for _ in range(500)) == 500
+
+
+def test_radial_density_sum_rules(oracle):
+    v = workloads.random_v(3, [20.0, 10.0], 1)
+    L = oracle.OracleLattice(2, (5, 5, 5), v)
+    cfg = L.initial_setup(oracle.Generator(2), species_numbers=workloads.equal_counts(250, 3))
+    shells = L.lattice_shells(cfg, 4)
+    np.testing.assert_allclose(shells, np.float32(np.sqrt([0, 3, 4, 8])).astype(np.float64))
+    rho = L.radial_densities(cfg, 4, shells)
+    np.testing.assert_allclose(rho[:, :, 0], np.eye(3))
+    np.testing.assert_allclose(rho.sum(axis=1)[:, 1:], np.tile([8, 6, 12], (3, 1)), rtol=1e-12)
+
+
+def test_wl_bin_index_truncates_toward_zero(oracle):
+    edges = np.linspace(-1.0, 0.0, 11)
+    bi = lambda e: oracle.lib().bwo_wl_bin_index(e, edges.ctypes.data_as(
+        oracle.C.POINTER(oracle.C.c_double)), 10)
+    assert bi(-1.0) == 1
+    assert bi(-1.0 - 0.05) == 1   # int() truncates -0.5 to 0
+    assert bi(-1.0 - 0.15) == 0
+    assert bi(-0.05) == 10
+    assert bi(0.0) == 11
+
+
+def test_wl_window_setup_matches_reference_formulae(oracle):
+    v = workloads.random_v(4, [10.0, 5.0], 1)
+    L = oracle.OracleLattice(2, (4, 4, 4), v)
+    w, edges = L.wl_setup(512, 4, 0.25, -96.0, 0.0)
+    # divide_range: factor = (b-1)/n; end_i = floor(factor*i + 1)
+    f = 511 / 4
+    ends = [int(np.floor(f * i + 1)) for i in range(1, 4)] + [512]
+    assert w[:, 1].tolist() == ends
+    assert w[0, 0] == 1
+    # bin edges span energy_min..energy_max in Ry (n_atoms/(Ry_to_eV*1000))
+    to_ry = 128 / (13.605693122 * 1000)
+    assert abs(edges[0] - (-96.0 * to_ry)) < 1e-15
+    assert abs(edges[-1]) < 1e-12
