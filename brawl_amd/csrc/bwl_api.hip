@@ -360,7 +360,9 @@ int bwl_metropolis_anneal(bwl_ctx *c, const bwl_anneal_params *p, double *energi
   ParPlan plan;
   rc = plan_parallel(g, plan, g_err);
   if (rc) return rc;
-  int threads = ((plan.npairs + 63) / 64) * 64;
+  int max_pairs = 0;
+  for (const RoundVar &v : plan.vars) max_pairs = v.n2 > max_pairs ? v.n2 : max_pairs;
+  int threads = ((max_pairs + 63) / 64) * 64;
   if (threads > 1024) threads = 1024;
   if (threads < 64) threads = 64;
   size_t lds = par_lds_bytes(g, threads);
@@ -381,15 +383,13 @@ int bwl_metropolis_anneal(bwl_ctx *c, const bwl_anneal_params *p, double *energi
   a.compact = c->d_compact;
   a.off = c->d_off;
   a.vex = c->d_vex;
-  for (int k = 0; k < 3; ++k) {
-    a.K[k] = plan.K[k];
-    a.stride[k] = plan.stride[k];
-  }
-  a.A = plan.A;
-  a.npairs = plan.npairs;
-  a.phases_per_sweep = plan.phases_per_sweep;
-  a.n_mul = (int32_t)plan.muls.size();
+  a.stride = plan.stride;
+  a.L = plan.L;
+  for (int k = 0; k < 3; ++k) a.K[k] = plan.K[k];
+  a.n_var = (int32_t)plan.vars.size();
+  for (size_t k = 0; k < plan.vars.size(); ++k) a.var[k] = plan.vars[k];
   a.muls = (const uint32_t *)(base + off_muls);
+  a.min_trials_per_sweep = g.n_atoms;
   a.T0 = p->T;
   a.dT = p->delta_T;
   a.T_steps = p->T_steps;
@@ -404,7 +404,7 @@ int bwl_metropolis_anneal(bwl_ctx *c, const bwl_anneal_params *p, double *energi
   a.acc_of_T = a.C_of_T + T;
   a.e_trace = nullptr;
   a.counters = (long long *)(base + off_cnt);
-  long long cnt0[2] = {0, 0};
+  long long cnt0[3] = {0, 0, 0};
   HIP_OK(hipMemcpyAsync(base + off_args, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipMemcpyAsync(base + off_muls, plan.muls.data(), plan.muls.size() * 4,
                         hipMemcpyHostToDevice, c->stream));
@@ -415,7 +415,7 @@ int bwl_metropolis_anneal(bwl_ctx *c, const bwl_anneal_params *p, double *energi
   std::vector<double> out(3 * T);
   HIP_OK(hipMemcpyAsync(out.data(), a.energies_of_T, 3 * T * sizeof(double), hipMemcpyDeviceToHost,
                         c->stream));
-  long long cnt[2];
+  long long cnt[3];
   HIP_OK(hipMemcpyAsync(cnt, a.counters, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   memcpy(energies_of_T, out.data(), T * sizeof(double));
@@ -424,8 +424,7 @@ int bwl_metropolis_anneal(bwl_ctx *c, const bwl_anneal_params *p, double *energi
   if (stats) {
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    stats->trials = (long long)p->T_steps * (p->n_sweeps + p->burn_in_sweeps) *
-                    (long long)plan.phases_per_sweep * plan.npairs;
+    stats->trials = cnt[2];
     stats->accepted = cnt[1];
     stats->kernel_ms = ms;
   }

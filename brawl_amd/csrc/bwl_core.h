@@ -58,6 +58,21 @@ __host__ __device__ inline int sub_of(int lattice, int bx, int by) {
   return lattice == 1 ? 2 * bx + by : bx;
 }
 
+// One kind of round of the parallel sampler.  Active sites form two
+// interleaved grids: along axis `axis` the layers sit at
+//   o + (j/2)*L + (j odd ? u : 0),  j = 0..nL1+nL2-1   (even j: grid 1,
+// odd j: grid 2), and along the other two axes both grids are stride-s
+// grids with independent random origins.  Every two active sites are at
+// least rcut+1 apart along some axis (periodically), and an odd u moves
+// grid 2 to a different sublattice, so swaps cross sublattices.
+struct RoundVar {
+  int32_t axis, u;
+  int32_t nL1, nL2;      // layers of grid 1 / grid 2
+  int32_t n1, n2;        // sites of grid 1 / grid 2 (n2 <= n1), pairs = n2
+  int32_t mul_off, n_mul;  // multipliers coprime with n1
+};
+constexpr int kMaxRoundVars = 6;
+
 // Padded LDS address of the interior cell (s, cx, cy, cz).
 __host__ __device__ inline int padded_addr(const Geo &g, int s, int cx, int cy, int cz) {
   return s * g.sub_stride + ((cz + g.h) * g.P[1] + (cy + g.h)) * g.P[0] + (cx + g.h);

@@ -210,40 +210,87 @@ int initial_setup_host(const Geo &g, const int64_t *species_numbers, const doubl
 
 // ---------------------------------------------------------------------------
 int plan_parallel(const Geo &g, ParPlan &p, std::string &err) {
-  const int need = g.rcut + 1;
-  const int s = (need % 2) ? need + 1 : need;  // smallest even >= rcut+1
-  int A = 1;
+  const int r = g.rcut;
+  const int need = r + 1;
+  p.stride = (need % 2) ? need + 1 : need;  // smallest even >= rcut+1
+  p.L = 2 * r + 4;                           // even; u and L-u both >= rcut+1
   for (int a = 0; a < 3; ++a) {
     const int m = g.m[a];
     if (m < need) {
       err = "lattice too small for simultaneous swaps (extent below interaction range)";
       return BWL_ERR_UNSUPPORTED;
     }
-    int K = (m % s == 0) ? m / s : (m - need) / s + 1;
-    if (K < 1) K = 1;
-    p.K[a] = K;
-    p.stride[a] = s;
-    A *= K;
+    p.K[a] = (m % p.stride == 0) ? m / p.stride : (m - need) / p.stride + 1;
   }
-  if (A < 2) {
-    err = "lattice too small for simultaneous swaps (fewer than two active sites)";
+  p.vars.clear();
+  p.muls.clear();
+  for (int axis = 0; axis < 3; ++axis)
+    for (int par = 0; par < 2; ++par) {
+      RoundVar v;
+      v.axis = axis;
+      v.u = ((r + 1) % 2 == par) ? r + 1 : r + 2;
+      // layers at (j/2)*L + (j&1)*u while the wrap gap stays >= rcut+1
+      int nL = 0;
+      for (int j = 0;; ++j) {
+        int pos = (j >> 1) * p.L + (j & 1) * v.u;
+        if (pos > g.m[axis] - need) break;
+        ++nL;
+      }
+      v.nL1 = (nL + 1) / 2;
+      v.nL2 = nL / 2;
+      if (v.nL2 < 1) continue;
+      const int kk = p.K[(axis + 1) % 3] * p.K[(axis + 2) % 3];
+      v.n1 = v.nL1 * kk;
+      v.n2 = v.nL2 * kk;
+      v.mul_off = (int32_t)p.muls.size();
+      uint64_t x = 0x9E3779B97F4A7C15ull ^ (uint64_t)(axis * 2 + par);
+      int got = 0;
+      for (int tries = 0; tries < 4096 && got < 16; ++tries) {
+        x = x * 6364136223846793005ull + 1442695040888963407ull;
+        uint32_t c = (uint32_t)((x >> 33) % (uint64_t)v.n1);
+        if (c == 0 || std::gcd(c, (uint32_t)v.n1) != 1) continue;
+        if (std::find(p.muls.begin() + v.mul_off, p.muls.end(), c) != p.muls.end()) continue;
+        p.muls.push_back(c);
+        ++got;
+      }
+      if (got == 0) {
+        p.muls.push_back(1);
+        got = 1;
+      }
+      v.n_mul = got;
+      p.vars.push_back(v);
+    }
+  if (p.vars.empty()) {
+    err = "lattice too small for simultaneous swaps (fewer than two active layers)";
     return BWL_ERR_UNSUPPORTED;
   }
-  p.A = A;
-  p.npairs = A / 2;
-  p.phases_per_sweep = (g.n_atoms + p.npairs - 1) / p.npairs;
-  p.muls.clear();
-  // multipliers coprime with A, spread over [1, A)
-  uint64_t x = 0x9E3779B97F4A7C15ull;
-  for (int tries = 0; tries < 4096 && p.muls.size() < 64; ++tries) {
-    x = x * 6364136223846793005ull + 1442695040888963407ull;
-    uint32_t c = (uint32_t)((x >> 33) % (uint64_t)A);
-    if (c == 0) continue;
-    if (std::gcd(c, (uint32_t)A) != 1) continue;
-    if (std::find(p.muls.begin(), p.muls.end(), c) != p.muls.end()) continue;
-    p.muls.push_back(c);
-  }
-  if (p.muls.empty()) p.muls.push_back(1);
+  // The rounds must be able to exchange atoms between every pair of
+  // sublattices, or the sublattice compositions would be frozen: union the
+  // sublattices each round kind can pair and require one component.
+  int parent[4] = {0, 1, 2, 3};
+  auto find = [&](int x) {
+    while (parent[x] != x) x = parent[x] = parent[parent[x]];
+    return x;
+  };
+  for (const RoundVar &v : p.vars)
+    for (int s1 = 0; s1 < g.nsub; ++s1) {
+      int b1[3];
+      sub_bits(g.lattice, s1, b1[0], b1[1], b1[2]);
+      const int A = v.axis, B = (A + 1) % 3, Cx = (A + 2) % 3;
+      for (int fb = 0; fb <= 1; ++fb) {
+        int b2[3];
+        b2[A] = b1[A] ^ (v.u & 1);
+        b2[B] = g.lattice == BWL_FCC ? fb : b2[A];
+        b2[Cx] = g.lattice == BWL_FCC ? (b2[A] ^ b2[B]) : b2[A];
+        int s2 = sub_of(g.lattice, b2[0], b2[1]);
+        parent[find(s1)] = find(s2);
+      }
+    }
+  for (int s = 1; s < g.nsub; ++s)
+    if (find(s) != find(0)) {
+      err = "lattice too small for simultaneous swaps across sublattices";
+      return BWL_ERR_UNSUPPORTED;
+    }
   return BWL_OK;
 }
 

@@ -30,10 +30,11 @@ struct MT19937 {
 int initial_setup_host(const Geo &g, const int64_t *species_numbers, const double *concentrations,
                        MT19937 &mt, int8_t *config, std::string &err);
 
-// Round plan of the parallel sampler.
+// Round plan of the parallel sampler (see RoundVar in bwl_core.h).
 struct ParPlan {
-  int32_t K[3], stride[3];
-  int32_t A, npairs, phases_per_sweep;
+  int32_t stride, L;
+  int32_t K[3];
+  std::vector<RoundVar> vars;
   std::vector<uint32_t> muls;
 };
 int plan_parallel(const Geo &g, ParPlan &p, std::string &err);

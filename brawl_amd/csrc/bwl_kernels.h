@@ -45,14 +45,14 @@ struct ParArgs {
   int8_t *compact;       // in/out lattice
   const int32_t *off;
   const double *vex;
-  // active-site grid of a round
-  int32_t K[3];          // active positions per axis
-  int32_t stride[3];     // even strides, doubled-grid units
-  int32_t A;             // K0*K1*K2
-  int32_t npairs;        // A/2 pair trials per round
-  int32_t phases_per_sweep;
-  int32_t n_mul;
-  const uint32_t *muls;  // multipliers coprime with A
+  // rounds: two interleaved active grids (see RoundVar)
+  int32_t stride;        // even stride of both grids on the two free axes
+  int32_t L;             // layer period on the round's axis
+  int32_t K[3];          // stride-`stride` grid positions per axis
+  int32_t n_var;         // usable round kinds
+  RoundVar var[kMaxRoundVars];
+  const uint32_t *muls;
+  long long min_trials_per_sweep;  // trials per sweep (>= n_atoms)
   // schedule
   double T0, dT;
   int32_t T_steps, n_sweeps, burn_sweeps;
@@ -61,7 +61,7 @@ struct ParArgs {
   // outputs
   double *energies_of_T, *C_of_T, *acc_of_T;
   double *e_trace;       // optional [T_steps][n_sweeps]
-  long long *counters;   // [0] = next round counter, [1] += accepted
+  long long *counters;   // [0] = next round counter, [1] += accepted, [2] += trials
 };
 
 hipError_t launch_pack(const int8_t *ref, int8_t *compact, const Geo &g, hipStream_t s);

@@ -318,22 +318,92 @@ __global__ __launch_bounds__(64) void k_metro_seq(SeqArgs A) {
 // ------------------------------------------------------------------
 // Parallel Metropolis sampler.
 //
-// A round ("phase") selects the active sites
-//     (o + k*stride) mod m  per axis,  k = 0..K-1,
-// with a random occupied origin o and even strides; every pair of active
-// sites is at least rcut+1 apart along some axis (periodically), so no
-// active site lies in another's neighbour shells and the pair trials of a
-// round are independent Metropolis moves on disjoint, non-interacting site
-// pairs.  Active sites are paired by a random affine permutation of their
-// indices.  Site choice and pairing never depend on the state, so each
-// pair swap is a symmetric proposal and detailed balance holds.
+// A round picks a kind (RoundVar: axis A and layer offset u) and two random
+// origins; its active sites are two interleaved grids (grid 1 on layers
+// o_A + k*L, grid 2 on layers o_A + u + k*L; stride-s grids with their own
+// origins on the other axes).  Any two active sites are at least rcut+1
+// apart along some axis (periodically), so no active site lies in another's
+// neighbour shells and the pair trials of a round are independent
+// Metropolis moves on disjoint, non-interacting pairs.  Grid-2 site q is
+// paired with grid-1 site (mul*q + add) mod n1.  Round kind, origins and
+// pairing never depend on the state, each pair swap is a symmetric
+// proposal, and with u odd or even the two grids lie on different or equal
+// sublattices, so every pair of sites is connected by some sequence of
+// rounds (the chain is irreducible over the fixed-composition ensemble).
 //
 // LDS holds the lattice as bit positions FIELD*(species-1); a site's
 // shell-n neighbour species counts are accumulated as packed FIELD-bit
 // fields with one shift-add per neighbour.  dE is contracted in FP64:
 //   dE = sum_n sum_s (c_a[n][s] - c_b[n][s]) * (V(beta,s,n) - V(alpha,s,n))
-// (valid because a and b are not neighbours of each other).
+// (exact because a and b are not neighbours of each other).
 // ------------------------------------------------------------------
+struct RoundGeo {
+  int ax, bx, cx;        // permutation: round axis, then the two free axes
+  int o1[3], o2[3];      // origins of grid 1 / grid 2 (doubled coords)
+  int sub1, sub2;
+  uint32_t mul, add;
+  RoundVar v;
+};
+
+__device__ inline int occupied_coord(int lattice, int pa, int pb, uint32_t r, int n) {
+  // third coordinate of an occupied site given two others
+  return 2 * (int)below(r, (uint32_t)n) + (lattice == 1 ? ((pa + pb) & 1) : (pa & 1));
+}
+
+__device__ inline RoundGeo make_round(const ParArgs &C, unsigned long long phase) {
+  const Geo &g = C.g;
+  U4 rp = philox4x32(U4{(uint32_t)phase, (uint32_t)(phase >> 32), 0xffffffffu, C.chain_id},
+                     C.seed_lo, C.seed_hi);
+  U4 rq = philox4x32(U4{(uint32_t)phase, (uint32_t)(phase >> 32), 0xfffffffeu, C.chain_id},
+                     C.seed_lo, C.seed_hi);
+  RoundGeo R;
+  R.v = C.var[below(rp.w, (uint32_t)C.n_var)];
+  R.ax = R.v.axis;
+  R.bx = (R.ax + 1) % 3;
+  R.cx = (R.ax + 2) % 3;
+  // grid 1 origin: random occupied site
+  R.o1[R.ax] = (int)below(rp.x, (uint32_t)g.m[R.ax]);
+  if (g.lattice == 1)
+    R.o1[R.bx] = (int)below(rp.y, (uint32_t)g.m[R.bx]);
+  else
+    R.o1[R.bx] = 2 * (int)below(rp.y, (uint32_t)g.n[R.bx]) + (R.o1[R.ax] & 1);
+  R.o1[R.cx] = occupied_coord(g.lattice, R.o1[R.ax], R.o1[R.bx], rp.z, g.n[R.cx]);
+  // grid 2 origin: shifted by u along the round axis, free elsewhere
+  R.o2[R.ax] = R.o1[R.ax] + R.v.u;
+  if (g.lattice == 1)
+    R.o2[R.bx] = (int)below(rq.x, (uint32_t)g.m[R.bx]);
+  else
+    R.o2[R.bx] = 2 * (int)below(rq.x, (uint32_t)g.n[R.bx]) + (R.o2[R.ax] & 1);
+  R.o2[R.cx] = occupied_coord(g.lattice, R.o2[R.ax], R.o2[R.bx], rq.y, g.n[R.cx]);
+  R.o2[R.ax] %= g.m[R.ax];
+  R.sub1 = sub_of(g.lattice, R.o1[0] & 1, R.o1[1] & 1);
+  R.sub2 = sub_of(g.lattice, R.o2[0] & 1, R.o2[1] & 1);
+  R.mul = C.muls[R.v.mul_off + below(rq.z, (uint32_t)R.v.n_mul)];
+  R.add = below(rq.w, (uint32_t)R.v.n1);
+  return R;
+}
+
+// Site of grid-`which` with linear index q: (layer, kb, kc), layer slowest.
+__device__ inline void round_site(const ParArgs &C, const RoundGeo &R, int which, uint32_t q,
+                                  int &addr, int cell[3]) {
+  const Geo &g = C.g;
+  const int KB = C.K[R.bx], KC = C.K[R.cx];
+  const int kb = q % KB;
+  const int kc = (q / KB) % KC;
+  const int jl = q / (KB * KC);
+  const int *o = which ? R.o2 : R.o1;
+  int p[3];
+  p[R.ax] = o[R.ax] + jl * C.L;
+  p[R.bx] = o[R.bx] + kb * C.stride;
+  p[R.cx] = o[R.cx] + kc * C.stride;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (p[a] >= g.m[a]) p[a] -= g.m[a];
+    cell[a] = p[a] >> 1;
+  }
+  addr = padded_addr(g, which ? R.sub2 : R.sub1, cell[0], cell[1], cell[2]);
+}
+
 template <typename CW, int FIELD>
 __global__ __launch_bounds__(1024) void k_metro_par(const ParArgs *__restrict__ chains) {
   extern __shared__ __align__(16) uint8_t smem[];
@@ -355,11 +425,8 @@ __global__ __launch_bounds__(1024) void k_metro_par(const ParArgs *__restrict__ 
   __syncthreads();
 
   const int S = g.S, R = g.R;
-  const uint32_t A = (uint32_t)C.A;
   unsigned long long phase = C.phase0;
-  long long acc_total = 0;
-  const double trials_per_T =
-      (double)C.n_sweeps * (double)C.phases_per_sweep * (double)C.npairs;
+  long long acc_total = 0, trials_total = 0;
 
   for (int t = 0; t < C.T_steps; ++t) {
     const double temp = C.T0 + (double)t * C.dT;
@@ -367,46 +434,22 @@ __global__ __launch_bounds__(1024) void k_metro_par(const ParArgs *__restrict__ 
     const double beta = 1.0 / sim_temp;
     double step_E = 0.0, step_Esq = 0.0;
     unsigned long long acc_local = 0;
+    long long trials_sampled = 0;
     const int nsw = C.burn_sweeps + C.n_sweeps;
     for (int sw = 0; sw < nsw; ++sw) {
       const bool sampled = sw >= C.burn_sweeps;
-      for (int ph = 0; ph < C.phases_per_sweep; ++ph, ++phase) {
-        // round parameters (uniform across the block)
-        U4 rp = philox4x32(U4{(uint32_t)phase, (uint32_t)(phase >> 32), 0xffffffffu, C.chain_id},
-                           C.seed_lo, C.seed_hi);
-        int ox, oy, oz;
-        ox = (int)below(rp.x, (uint32_t)g.m[0]);
-        if (g.lattice == 1) {
-          oy = (int)below(rp.y, (uint32_t)g.m[1]);
-          oz = 2 * (int)below(rp.z, (uint32_t)g.n[2]) + ((ox + oy) & 1);
-        } else {
-          oy = 2 * (int)below(rp.y, (uint32_t)g.n[1]) + (ox & 1);
-          oz = 2 * (int)below(rp.z, (uint32_t)g.n[2]) + (ox & 1);
-        }
-        U4 rq = philox4x32(U4{(uint32_t)phase, (uint32_t)(phase >> 32), 0xfffffffeu, C.chain_id},
-                           C.seed_lo, C.seed_hi);
-        const uint32_t mul = C.muls[below(rp.w, (uint32_t)C.n_mul)];
-        const uint32_t add = below(rq.x, A);
-        const int sub = sub_of(g.lattice, ox & 1, oy & 1);
-        const int32_t *off = C.off + sub * g.z;
-
-        for (int q = tid; q < C.npairs; q += blockDim.x) {
-          uint32_t ia = (uint32_t)(((uint64_t)mul * (uint32_t)(2 * q) + add) % A);
-          uint32_t ib = (uint32_t)(((uint64_t)mul * (uint32_t)(2 * q + 1) + add) % A);
-          int addr[2], cell[2][3];
-#pragma unroll
-          for (int w = 0; w < 2; ++w) {
-            uint32_t ii = w ? ib : ia;
-            int kx = ii % C.K[0];
-            int ky = (ii / C.K[0]) % C.K[1];
-            int kz = ii / (C.K[0] * C.K[1]);
-            int x = ox + kx * C.stride[0]; if (x >= g.m[0]) x -= g.m[0];
-            int y = oy + ky * C.stride[1]; if (y >= g.m[1]) y -= g.m[1];
-            int z = oz + kz * C.stride[2]; if (z >= g.m[2]) z -= g.m[2];
-            cell[w][0] = x >> 1; cell[w][1] = y >> 1; cell[w][2] = z >> 1;
-            addr[w] = padded_addr(g, sub, x >> 1, y >> 1, z >> 1);
-          }
-          const int codeA = lat[addr[0]], codeB = lat[addr[1]];
+      long long done = 0;
+      while (done < C.min_trials_per_sweep) {
+        const RoundGeo RG = make_round(C, phase);
+        const int32_t *off1 = C.off + RG.sub1 * g.z;
+        const int32_t *off2 = C.off + RG.sub2 * g.z;
+        const int npairs = RG.v.n2;
+        for (int q = tid; q < npairs; q += blockDim.x) {
+          int addrA, addrB, cellA[3], cellB[3];
+          round_site(C, RG, 1, (uint32_t)q, addrB, cellB);
+          uint32_t ia = (uint32_t)(((uint64_t)RG.mul * (uint32_t)q + RG.add) % (uint32_t)RG.v.n1);
+          round_site(C, RG, 0, ia, addrA, cellA);
+          const int codeA = lat[addrA], codeB = lat[addrB];
           if (codeA == codeB) {
             if (sampled) acc_local++;
             continue;
@@ -417,9 +460,8 @@ __global__ __launch_bounds__(1024) void k_metro_par(const ParArgs *__restrict__ 
             CW sa = 0, sb = 0;
             if (n < R) {
               for (int tt = g.shell_start[n]; tt < g.shell_start[n + 1]; ++tt) {
-                const int d = off[tt];
-                sa += (CW)1 << lat[addr[0] + d];
-                sb += (CW)1 << lat[addr[1] + d];
+                sa += (CW)1 << lat[addrA + off1[tt]];
+                sb += (CW)1 << lat[addrB + off2[tt]];
               }
             }
             ca[n] = sa;
@@ -446,14 +488,18 @@ __global__ __launch_bounds__(1024) void k_metro_par(const ParArgs *__restrict__ 
             acc = u01_53(r.x, r.y) < exp(-(beta * dE));
           }
           if (acc) {
-            write_site(lat, g, sub, cell[0][0], cell[0][1], cell[0][2], (int8_t)codeB, 0, 1);
-            write_site(lat, g, sub, cell[1][0], cell[1][1], cell[1][2], (int8_t)codeA, 0, 1);
+            write_site(lat, g, RG.sub1, cellA[0], cellA[1], cellA[2], (int8_t)codeB, 0, 1);
+            write_site(lat, g, RG.sub2, cellB[0], cellB[1], cellB[2], (int8_t)codeA, 0, 1);
             if (sampled) acc_local++;
           }
         }
+        done += npairs;
+        ++phase;
         __syncthreads();
       }
+      trials_total += done;
       if (sampled) {
+        trials_sampled += done;
         // energy recount after the sweep
         const int cells = g.n[0] * g.n[1] * g.n[2];
         const int nsites = g.nsub * cells;
@@ -497,7 +543,7 @@ __global__ __launch_bounds__(1024) void k_metro_par(const ParArgs *__restrict__ 
       if (Cv < 0.0) Cv = 0.0;
       if (temp <= 0.0) Cv = 0.0;
       C.C_of_T[t] = Cv;
-      C.acc_of_T[t] = (double)acc_sum / trials_per_T;
+      C.acc_of_T[t] = (double)acc_sum / (double)trials_sampled;
       acc_total += (long long)acc_sum;
     }
   }
@@ -506,6 +552,7 @@ __global__ __launch_bounds__(1024) void k_metro_par(const ParArgs *__restrict__ 
   if (tid == 0) {
     C.counters[0] = (long long)phase;
     C.counters[1] += acc_total;
+    C.counters[2] += trials_total;
   }
 }
 

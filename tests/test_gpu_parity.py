@@ -226,3 +226,28 @@ def test_parallel_sampler_statistics_match_reference_sampler(hip, oracle):
     sigma = np.sqrt(e_gpu.var(ddof=1) / len(e_gpu) + e_ref.var(ddof=1) / len(e_ref))
     _say("<E>/N gpu", e_gpu.mean(), "ref", e_ref.mean(), "diff/sigma", diff / sigma)
     assert abs(diff) <= 2.0 * sigma + 1e-9
+
+
+@pytest.mark.parametrize("lat,n,R", [(1, (8, 8, 8), 2), (2, (8, 8, 8), 4), (2, (7, 5, 6), 3)])
+def test_parallel_sampler_moves_atoms_between_sublattices(hip, oracle, lat, n, R):
+    """Rounds pair sites of different sublattices: starting from a fully
+    sublattice-ordered configuration at high temperature, each sublattice's
+    composition relaxes toward the global one."""
+    from brawl_amd import workloads
+    S = 2
+    v = workloads.random_v(S, [10.0 / k for k in range(1, R + 1)], 3)
+    ref = oracle.OracleLattice(lat, n, v)
+    cfg = ref.empty_config()
+    m = [2 * x for x in n]
+    for x in range(m[0]):
+        for y in range(m[1]):
+            for z in range(m[2]):
+                occ = ((x + y + z) % 2 == 0) if lat == 1 else (x % 2 == y % 2 == z % 2)
+                if occ:
+                    cfg[x, y, z] = 1 + (x % 2)   # species set by the x parity
+    with hip.Lattice(lat, n, v) as L:
+        L.set_config(cfg)
+        L.metropolis_simulated_annealing(T=20000.0, n_sweeps=20, seed=3)
+        out = L.get_config()
+    frac = (out[0::2][out[0::2] > 0] == 1).mean()   # species 1 on even-x sites
+    assert abs(frac - 0.5) < 0.1, frac
