@@ -362,10 +362,18 @@ int bwl_metropolis_anneal(bwl_ctx *c, const bwl_anneal_params *p, double *energi
   if (rc) return rc;
   int max_pairs = 0;
   for (const RoundVar &v : plan.vars) max_pairs = v.n2 > max_pairs ? v.n2 : max_pairs;
-  int threads = ((max_pairs + 63) / 64) * 64;
-  if (threads > 1024) threads = 1024;
+  // fewest passes over the pairs, then the fewest threads for that
+  const int passes = (max_pairs + 511) / 512;
+  int threads = (((max_pairs + passes - 1) / passes + 63) / 64) * 64;
+  if (threads > 512) threads = 512;
   if (threads < 64) threads = 64;
-  size_t lds = par_lds_bytes(g, threads);
+  NatGeo nat;
+  bool use_nat = make_nat(g, nat);
+  size_t lds = par_lds_bytes(g, use_nat ? &nat : nullptr, threads);
+  if (use_nat && lds > (size_t)lds_limit()) {
+    use_nat = false;
+    lds = par_lds_bytes(g, nullptr, threads);
+  }
   if (lds > (size_t)lds_limit()) return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
   const bool wide = g.S > 5;
   const size_t T = (size_t)p->T_steps;
@@ -380,6 +388,7 @@ int bwl_metropolis_anneal(bwl_ctx *c, const bwl_anneal_params *p, double *energi
   ParArgs a;
   memset(&a, 0, sizeof(a));
   a.g = g;
+  a.nat = nat;
   a.compact = c->d_compact;
   a.off = c->d_off;
   a.vex = c->d_vex;
@@ -389,6 +398,11 @@ int bwl_metropolis_anneal(bwl_ctx *c, const bwl_anneal_params *p, double *energi
   a.n_var = (int32_t)plan.vars.size();
   for (size_t k = 0; k < plan.vars.size(); ++k) a.var[k] = plan.vars[k];
   a.muls = (const uint32_t *)(base + off_muls);
+  for (int k = 0; k < 3; ++k) {
+    a.n_units[k] = plan.n_units[k];
+    for (int u = 0; u < 16; ++u) a.units[k][u] = plan.units[k][u];
+  }
+  a.fast_halo = plan.fast_halo ? 1 : 0;
   a.min_trials_per_sweep = g.n_atoms;
   a.T0 = p->T;
   a.dT = p->delta_T;
@@ -410,7 +424,8 @@ int bwl_metropolis_anneal(bwl_ctx *c, const bwl_anneal_params *p, double *energi
                         hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipMemcpyAsync(a.counters, cnt0, sizeof(cnt0), hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipEventRecord(c->ev0, c->stream));
-  HIP_OK(launch_metro_par((const ParArgs *)(base + off_args), 1, lds, threads, wide, c->stream));
+  HIP_OK(launch_metro_par((const ParArgs *)(base + off_args), 1, g.lattice, g.R, use_nat, lds,
+                          threads, wide, c->stream));
   HIP_OK(hipEventRecord(c->ev1, c->stream));
   std::vector<double> out(3 * T);
   HIP_OK(hipMemcpyAsync(out.data(), a.energies_of_T, 3 * T * sizeof(double), hipMemcpyDeviceToHost,

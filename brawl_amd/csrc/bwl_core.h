@@ -69,9 +69,21 @@ struct RoundVar {
   int32_t axis, u;
   int32_t nL1, nL2;      // layers of grid 1 / grid 2
   int32_t n1, n2;        // sites of grid 1 / grid 2 (n2 <= n1), pairs = n2
-  int32_t mul_off, n_mul;  // multipliers coprime with n1
+  int32_t mul_off, n_mul;  // multipliers coprime with nL1
 };
 constexpr int kMaxRoundVars = 6;
+
+// "Natural" LDS layout of a cubic supercell (n1 = n2 = n3 = n): lattice
+// coordinates in which every neighbour shell is a fixed set of integer
+// vectors, so neighbour offsets do not depend on the site.
+//   fcc: (a,b,c) = (x, y, (z-x-y)/2 mod n), box (2n, 2n, n)
+//   bcc: (a,b,c) = (z, (x-z)/2 mod n, (y-z)/2 mod n), box (2n, n, n)
+// Stored [c+h2][b+h1][a+h0] with periodic halos h.
+struct NatGeo {
+  int32_t D[3], h[3], P[3];
+  int32_t lds_bytes;
+  int32_t off[kMaxNbr];  // linear neighbour offsets in shell/list order
+};
 
 // Padded LDS address of the interior cell (s, cx, cy, cz).
 __host__ __device__ inline int padded_addr(const Geo &g, int s, int cx, int cy, int cz) {

@@ -224,6 +224,15 @@ int plan_parallel(const Geo &g, ParPlan &p, std::string &err) {
   }
   p.vars.clear();
   p.muls.clear();
+  p.fast_halo = true;
+  for (int a = 0; a < 3; ++a) {
+    if (g.n[a] < 2 * g.h) p.fast_halo = false;
+    int got = 0;
+    for (uint32_t c = 1; c <= (uint32_t)p.K[a] && got < 16; ++c)
+      if ((c < (uint32_t)p.K[a] || p.K[a] == 1) && std::gcd(c, (uint32_t)p.K[a]) == 1)
+        p.units[a][got++] = c;
+    p.n_units[a] = got;
+  }
   for (int axis = 0; axis < 3; ++axis)
     for (int par = 0; par < 2; ++par) {
       RoundVar v;
@@ -243,20 +252,13 @@ int plan_parallel(const Geo &g, ParPlan &p, std::string &err) {
       v.n1 = v.nL1 * kk;
       v.n2 = v.nL2 * kk;
       v.mul_off = (int32_t)p.muls.size();
-      uint64_t x = 0x9E3779B97F4A7C15ull ^ (uint64_t)(axis * 2 + par);
       int got = 0;
-      for (int tries = 0; tries < 4096 && got < 16; ++tries) {
-        x = x * 6364136223846793005ull + 1442695040888963407ull;
-        uint32_t c = (uint32_t)((x >> 33) % (uint64_t)v.n1);
-        if (c == 0 || std::gcd(c, (uint32_t)v.n1) != 1) continue;
-        if (std::find(p.muls.begin() + v.mul_off, p.muls.end(), c) != p.muls.end()) continue;
-        p.muls.push_back(c);
-        ++got;
-      }
-      if (got == 0) {
-        p.muls.push_back(1);
-        got = 1;
-      }
+      for (uint32_t c = 1; c <= (uint32_t)v.nL1 && got < 16; ++c)
+        if (c < (uint32_t)v.nL1 || v.nL1 == 1)
+          if (std::gcd(c, (uint32_t)v.nL1) == 1) {
+            p.muls.push_back(c);
+            ++got;
+          }
       v.n_mul = got;
       p.vars.push_back(v);
     }
@@ -292,6 +294,38 @@ int plan_parallel(const Geo &g, ParPlan &p, std::string &err) {
       return BWL_ERR_UNSUPPORTED;
     }
   return BWL_OK;
+}
+
+bool make_nat(const Geo &g, NatGeo &nat) {
+  memset(&nat, 0, sizeof(nat));
+  if (g.n[0] != g.n[1] || g.n[1] != g.n[2]) return false;
+  const int n = g.n[0];
+  const int8_t(*tab)[3] = g.lattice == BWL_FCC ? kFccOffsets : kBccOffsets;
+  int d[kMaxNbr][3];
+  for (int t = 0; t < g.z; ++t) {
+    const int dx = tab[t][0], dy = tab[t][1], dz = tab[t][2];
+    if (g.lattice == BWL_FCC) {
+      d[t][0] = dx; d[t][1] = dy; d[t][2] = (dz - dx - dy) / 2;
+    } else {
+      d[t][0] = dz; d[t][1] = (dx - dz) / 2; d[t][2] = (dy - dz) / 2;
+    }
+  }
+  if (g.lattice == BWL_FCC) {
+    nat.D[0] = 2 * n; nat.D[1] = 2 * n; nat.D[2] = n;
+  } else {
+    nat.D[0] = 2 * n; nat.D[1] = n; nat.D[2] = n;
+  }
+  for (int a = 0; a < 3; ++a) {
+    int h = 0;
+    for (int t = 0; t < g.z; ++t) h = std::max(h, std::abs(d[t][a]));
+    nat.h[a] = h;
+    if (nat.D[a] < 2 * h) return false;
+    nat.P[a] = nat.D[a] + 2 * h;
+  }
+  nat.lds_bytes = nat.P[0] * nat.P[1] * nat.P[2];
+  for (int t = 0; t < g.z; ++t)
+    nat.off[t] = (d[t][2] * nat.P[1] + d[t][1]) * nat.P[0] + d[t][0];
+  return true;
 }
 
 int check_config(const Geo &g, const int8_t *config, std::string &err) {

@@ -35,9 +35,16 @@ struct ParPlan {
   int32_t stride, L;
   int32_t K[3];
   std::vector<RoundVar> vars;
-  std::vector<uint32_t> muls;
+  std::vector<uint32_t> muls;      // per round kind, coprime with its nL1
+  uint32_t units[3][16];           // per axis, coprime with K[axis]
+  int32_t n_units[3];
+  bool fast_halo;
 };
 int plan_parallel(const Geo &g, ParPlan &p, std::string &err);
+
+// Natural layout of a cubic supercell; false if the lattice is not cubic or
+// an axis is shorter than twice its halo.
+bool make_nat(const Geo &g, NatGeo &nat);
 
 // Validate a reference-layout config: species 1..S on sites, 0 elsewhere.
 int check_config(const Geo &g, const int8_t *config, std::string &err);

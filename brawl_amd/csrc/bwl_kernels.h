@@ -42,6 +42,7 @@ struct SeqArgs {
 
 struct ParArgs {
   Geo g;
+  NatGeo nat;            // used by the natural-layout kernel
   int8_t *compact;       // in/out lattice
   const int32_t *off;
   const double *vex;
@@ -51,7 +52,10 @@ struct ParArgs {
   int32_t K[3];          // stride-`stride` grid positions per axis
   int32_t n_var;         // usable round kinds
   RoundVar var[kMaxRoundVars];
-  const uint32_t *muls;
+  const uint32_t *muls;  // per round kind: multipliers coprime with its nL1
+  uint32_t units[3][16]; // per axis: multipliers coprime with K[axis]
+  int32_t n_units[3];
+  int32_t fast_halo;     // n >= 2h on every axis
   long long min_trials_per_sweep;  // trials per sweep (>= n_atoms)
   // schedule
   double T0, dT;
@@ -73,8 +77,8 @@ size_t site_lds_bytes(const Geo &g);
 hipError_t launch_site_energy(const SiteArgs &a, hipStream_t s);
 size_t seq_lds_bytes(const Geo &g);
 hipError_t launch_metro_seq(const SeqArgs &a, hipStream_t s);
-size_t par_lds_bytes(const Geo &g, int threads);
-hipError_t launch_metro_par(const ParArgs *args_dev, int n_chains, size_t lds, int threads,
-                            bool wide, hipStream_t s);
+size_t par_lds_bytes(const Geo &g, const NatGeo *nat, int threads);
+hipError_t launch_metro_par(const ParArgs *args_dev, int n_chains, int lattice, int R, bool nat,
+                            size_t lds, int threads, bool wide, hipStream_t s);
 
 }  // namespace bwl

@@ -7,9 +7,7 @@
 //   k_site_energy          one site's energy in the reference's FP64 order
 //   k_metro_seq            bit-exact sequential Metropolis driven by the
 //                          reference's MT19937 stream (one wavefront)
-//   k_metro_par            parallel Metropolis: rounds of simultaneous,
-//                          non-interacting pair swaps on an LDS-resident
-//                          lattice, one workgroup per chain
+// (the parallel Metropolis sampler is in bwl_metro_par.hip)
 #include <hip/hip_runtime.h>
 
 #include "bwl_device.h"
@@ -316,247 +314,6 @@ __global__ __launch_bounds__(64) void k_metro_seq(SeqArgs A) {
 }
 
 // ------------------------------------------------------------------
-// Parallel Metropolis sampler.
-//
-// A round picks a kind (RoundVar: axis A and layer offset u) and two random
-// origins; its active sites are two interleaved grids (grid 1 on layers
-// o_A + k*L, grid 2 on layers o_A + u + k*L; stride-s grids with their own
-// origins on the other axes).  Any two active sites are at least rcut+1
-// apart along some axis (periodically), so no active site lies in another's
-// neighbour shells and the pair trials of a round are independent
-// Metropolis moves on disjoint, non-interacting pairs.  Grid-2 site q is
-// paired with grid-1 site (mul*q + add) mod n1.  Round kind, origins and
-// pairing never depend on the state, each pair swap is a symmetric
-// proposal, and with u odd or even the two grids lie on different or equal
-// sublattices, so every pair of sites is connected by some sequence of
-// rounds (the chain is irreducible over the fixed-composition ensemble).
-//
-// LDS holds the lattice as bit positions FIELD*(species-1); a site's
-// shell-n neighbour species counts are accumulated as packed FIELD-bit
-// fields with one shift-add per neighbour.  dE is contracted in FP64:
-//   dE = sum_n sum_s (c_a[n][s] - c_b[n][s]) * (V(beta,s,n) - V(alpha,s,n))
-// (exact because a and b are not neighbours of each other).
-// ------------------------------------------------------------------
-struct RoundGeo {
-  int ax, bx, cx;        // permutation: round axis, then the two free axes
-  int o1[3], o2[3];      // origins of grid 1 / grid 2 (doubled coords)
-  int sub1, sub2;
-  uint32_t mul, add;
-  RoundVar v;
-};
-
-__device__ inline int occupied_coord(int lattice, int pa, int pb, uint32_t r, int n) {
-  // third coordinate of an occupied site given two others
-  return 2 * (int)below(r, (uint32_t)n) + (lattice == 1 ? ((pa + pb) & 1) : (pa & 1));
-}
-
-__device__ inline RoundGeo make_round(const ParArgs &C, unsigned long long phase) {
-  const Geo &g = C.g;
-  U4 rp = philox4x32(U4{(uint32_t)phase, (uint32_t)(phase >> 32), 0xffffffffu, C.chain_id},
-                     C.seed_lo, C.seed_hi);
-  U4 rq = philox4x32(U4{(uint32_t)phase, (uint32_t)(phase >> 32), 0xfffffffeu, C.chain_id},
-                     C.seed_lo, C.seed_hi);
-  RoundGeo R;
-  R.v = C.var[below(rp.w, (uint32_t)C.n_var)];
-  R.ax = R.v.axis;
-  R.bx = (R.ax + 1) % 3;
-  R.cx = (R.ax + 2) % 3;
-  // grid 1 origin: random occupied site
-  R.o1[R.ax] = (int)below(rp.x, (uint32_t)g.m[R.ax]);
-  if (g.lattice == 1)
-    R.o1[R.bx] = (int)below(rp.y, (uint32_t)g.m[R.bx]);
-  else
-    R.o1[R.bx] = 2 * (int)below(rp.y, (uint32_t)g.n[R.bx]) + (R.o1[R.ax] & 1);
-  R.o1[R.cx] = occupied_coord(g.lattice, R.o1[R.ax], R.o1[R.bx], rp.z, g.n[R.cx]);
-  // grid 2 origin: shifted by u along the round axis, free elsewhere
-  R.o2[R.ax] = R.o1[R.ax] + R.v.u;
-  if (g.lattice == 1)
-    R.o2[R.bx] = (int)below(rq.x, (uint32_t)g.m[R.bx]);
-  else
-    R.o2[R.bx] = 2 * (int)below(rq.x, (uint32_t)g.n[R.bx]) + (R.o2[R.ax] & 1);
-  R.o2[R.cx] = occupied_coord(g.lattice, R.o2[R.ax], R.o2[R.bx], rq.y, g.n[R.cx]);
-  R.o2[R.ax] %= g.m[R.ax];
-  R.sub1 = sub_of(g.lattice, R.o1[0] & 1, R.o1[1] & 1);
-  R.sub2 = sub_of(g.lattice, R.o2[0] & 1, R.o2[1] & 1);
-  R.mul = C.muls[R.v.mul_off + below(rq.z, (uint32_t)R.v.n_mul)];
-  R.add = below(rq.w, (uint32_t)R.v.n1);
-  return R;
-}
-
-// Site of grid-`which` with linear index q: (layer, kb, kc), layer slowest.
-__device__ inline void round_site(const ParArgs &C, const RoundGeo &R, int which, uint32_t q,
-                                  int &addr, int cell[3]) {
-  const Geo &g = C.g;
-  const int KB = C.K[R.bx], KC = C.K[R.cx];
-  const int kb = q % KB;
-  const int kc = (q / KB) % KC;
-  const int jl = q / (KB * KC);
-  const int *o = which ? R.o2 : R.o1;
-  int p[3];
-  p[R.ax] = o[R.ax] + jl * C.L;
-  p[R.bx] = o[R.bx] + kb * C.stride;
-  p[R.cx] = o[R.cx] + kc * C.stride;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    if (p[a] >= g.m[a]) p[a] -= g.m[a];
-    cell[a] = p[a] >> 1;
-  }
-  addr = padded_addr(g, which ? R.sub2 : R.sub1, cell[0], cell[1], cell[2]);
-}
-
-template <typename CW, int FIELD>
-__global__ __launch_bounds__(1024) void k_metro_par(const ParArgs *__restrict__ chains) {
-  extern __shared__ __align__(16) uint8_t smem[];
-  const ParArgs &C = chains[blockIdx.x];
-  const Geo &g = C.g;
-  const int tid = threadIdx.x;
-  int8_t *lat = (int8_t *)smem;
-  size_t o = (g.lds_bytes + 15) & ~15;
-  double *vt = (double *)(smem + o);  // vt[(n*S + x)*S + s] = V_ex(x+1, s+1, n+1)
-  o += sizeof(double) * g.S * g.S * g.R;
-  double *red = (double *)(smem + o);
-  unsigned long long *redu = (unsigned long long *)(smem + o);
-
-  for (int i = tid; i < g.S * g.S * g.R; i += blockDim.x) {
-    int n = i / (g.S * g.S), x = (i / g.S) % g.S, s = i % g.S;
-    vt[i] = C.vex[x + g.S * (s + g.S * n)];
-  }
-  stage_lattice(lat, C.compact, g, FIELD);
-  __syncthreads();
-
-  const int S = g.S, R = g.R;
-  unsigned long long phase = C.phase0;
-  long long acc_total = 0, trials_total = 0;
-
-  for (int t = 0; t < C.T_steps; ++t) {
-    const double temp = C.T0 + (double)t * C.dT;
-    const double sim_temp = temp * kKbInRy;
-    const double beta = 1.0 / sim_temp;
-    double step_E = 0.0, step_Esq = 0.0;
-    unsigned long long acc_local = 0;
-    long long trials_sampled = 0;
-    const int nsw = C.burn_sweeps + C.n_sweeps;
-    for (int sw = 0; sw < nsw; ++sw) {
-      const bool sampled = sw >= C.burn_sweeps;
-      long long done = 0;
-      while (done < C.min_trials_per_sweep) {
-        const RoundGeo RG = make_round(C, phase);
-        const int32_t *off1 = C.off + RG.sub1 * g.z;
-        const int32_t *off2 = C.off + RG.sub2 * g.z;
-        const int npairs = RG.v.n2;
-        for (int q = tid; q < npairs; q += blockDim.x) {
-          int addrA, addrB, cellA[3], cellB[3];
-          round_site(C, RG, 1, (uint32_t)q, addrB, cellB);
-          uint32_t ia = (uint32_t)(((uint64_t)RG.mul * (uint32_t)q + RG.add) % (uint32_t)RG.v.n1);
-          round_site(C, RG, 0, ia, addrA, cellA);
-          const int codeA = lat[addrA], codeB = lat[addrB];
-          if (codeA == codeB) {
-            if (sampled) acc_local++;
-            continue;
-          }
-          CW ca[kMaxShells], cb[kMaxShells];
-#pragma unroll
-          for (int n = 0; n < kMaxShells; ++n) {
-            CW sa = 0, sb = 0;
-            if (n < R) {
-              for (int tt = g.shell_start[n]; tt < g.shell_start[n + 1]; ++tt) {
-                sa += (CW)1 << lat[addrA + off1[tt]];
-                sb += (CW)1 << lat[addrB + off2[tt]];
-              }
-            }
-            ca[n] = sa;
-            cb[n] = sb;
-          }
-          const int alpha = codeA / FIELD, betasp = codeB / FIELD;
-          double dE = 0.0;
-#pragma unroll
-          for (int n = 0; n < kMaxShells; ++n) {
-            if (n < R) {
-              const double *va = vt + (n * S + alpha) * S;
-              const double *vb = vt + (n * S + betasp) * S;
-              for (int s = 0; s < S; ++s) {
-                int d = (int)((ca[n] >> (FIELD * s)) & ((1 << FIELD) - 1)) -
-                        (int)((cb[n] >> (FIELD * s)) & ((1 << FIELD) - 1));
-                if (d) dE += (double)d * (vb[s] - va[s]);
-              }
-            }
-          }
-          bool acc = dE < 0.0;
-          if (!acc) {
-            U4 r = philox4x32(U4{(uint32_t)phase, (uint32_t)(phase >> 32), (uint32_t)q, C.chain_id},
-                              C.seed_lo, C.seed_hi);
-            acc = u01_53(r.x, r.y) < exp(-(beta * dE));
-          }
-          if (acc) {
-            write_site(lat, g, RG.sub1, cellA[0], cellA[1], cellA[2], (int8_t)codeB, 0, 1);
-            write_site(lat, g, RG.sub2, cellB[0], cellB[1], cellB[2], (int8_t)codeA, 0, 1);
-            if (sampled) acc_local++;
-          }
-        }
-        done += npairs;
-        ++phase;
-        __syncthreads();
-      }
-      trials_total += done;
-      if (sampled) {
-        trials_sampled += done;
-        // energy recount after the sweep
-        const int cells = g.n[0] * g.n[1] * g.n[2];
-        const int nsites = g.nsub * cells;
-        double part = 0.0;
-        for (int i = tid; i < nsites; i += blockDim.x) {
-          int s = i / cells;
-          int r = i - s * cells;
-          int cz = r / (g.n[0] * g.n[1]);
-          r -= cz * g.n[0] * g.n[1];
-          int cy = r / g.n[0];
-          int cx = r - cy * g.n[0];
-          int ad = padded_addr(g, s, cx, cy, cz);
-          int a = lat[ad] / FIELD;
-          const int32_t *oo = C.off + s * g.z;
-#pragma unroll
-          for (int n = 0; n < kMaxShells; ++n) {
-            if (n < R) {
-              CW c = 0;
-              for (int tt = g.shell_start[n]; tt < g.shell_start[n + 1]; ++tt)
-                c += (CW)1 << lat[ad + oo[tt]];
-              const double *va = vt + (n * S + a) * S;
-              for (int s2 = 0; s2 < S; ++s2)
-                part += (double)((c >> (FIELD * s2)) & ((1 << FIELD) - 1)) * va[s2];
-            }
-          }
-        }
-        double E = 0.5 * block_sum(part, red);
-        if (tid == 0) {
-          step_E = step_E + E;
-          step_Esq = step_Esq + E * E;
-          if (C.e_trace) C.e_trace[(long long)t * C.n_sweeps + (sw - C.burn_sweeps)] = E;
-        }
-      }
-    }
-    unsigned long long acc_sum = block_sum_u64(acc_local, redu);
-    if (tid == 0) {
-      const double ns = (double)C.n_sweeps;
-      const double na = (double)g.n_atoms;
-      C.energies_of_T[t] = step_E / ns / na;
-      double Cv = (step_Esq / ns - (step_E / ns) * (step_E / ns)) / (sim_temp * temp) / na;
-      if (Cv < 0.0) Cv = 0.0;
-      if (temp <= 0.0) Cv = 0.0;
-      C.C_of_T[t] = Cv;
-      C.acc_of_T[t] = (double)acc_sum / (double)trials_sampled;
-      acc_total += (long long)acc_sum;
-    }
-  }
-  __syncthreads();
-  unstage_lattice(lat, C.compact, g, FIELD);
-  if (tid == 0) {
-    C.counters[0] = (long long)phase;
-    C.counters[1] += acc_total;
-    C.counters[2] += trials_total;
-  }
-}
-
-// ------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------
 hipError_t launch_pack(const int8_t *ref, int8_t *compact, const Geo &g, hipStream_t s) {
@@ -612,24 +369,6 @@ hipError_t launch_metro_seq(const SeqArgs &a, hipStream_t s) {
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_metro_seq, dim3(1), dim3(64), lds, s, a);
-  return hipGetLastError();
-}
-
-size_t par_lds_bytes(const Geo &g, int threads) {
-  return (size_t)((g.lds_bytes + 15) & ~15) + sizeof(double) * g.S * g.S * g.R +
-         sizeof(double) * (threads / 64);
-}
-
-hipError_t launch_metro_par(const ParArgs *args_dev, int n_chains, size_t lds, int threads,
-                            bool wide, hipStream_t s) {
-  const void *fn = wide ? (const void *)k_metro_par<uint64_t, 8>
-                        : (const void *)k_metro_par<uint32_t, 6>;
-  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
-  if (wide)
-    hipLaunchKernelGGL((k_metro_par<uint64_t, 8>), dim3(n_chains), dim3(threads), lds, s, args_dev);
-  else
-    hipLaunchKernelGGL((k_metro_par<uint32_t, 6>), dim3(n_chains), dim3(threads), lds, s, args_dev);
   return hipGetLastError();
 }
 

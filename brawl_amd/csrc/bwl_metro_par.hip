@@ -1,0 +1,663 @@
+// bwl_metro_par.hip -- parallel Metropolis sampler on an LDS-resident lattice
+// (one workgroup per chain; many chains per launch).
+//
+// Rounds.  A round picks a kind (RoundVar: axis A and layer offset u) and
+// two random origins; its active sites are two interleaved grids (grid 1 on
+// layers o_A + k*L, grid 2 on layers o_A + u + k*L; stride-s grids with
+// their own origins on the other axes).  Any two active sites are at least
+// rcut+1 apart along some axis (periodically), so no active site lies in
+// another's neighbour shells and the pair trials of a round are independent
+// Metropolis moves on disjoint, non-interacting pairs.  Grid-2 site
+// (layer j, b, c) is paired with grid-1 site (ml*j+al mod nL1, mb*b+ab mod
+// KB, mc*c+ac mod KC), a bijection drawn per round.  Round kind, origins and
+// pairing never depend on the state, so each pair swap is a symmetric
+// proposal; the round kinds connect every pair of sublattices (checked on
+// the host), so the chain is irreducible over the fixed-composition
+// ensemble.
+//
+// LDS layout ("natural" coordinates, cubic supercells).  fcc and bcc sites
+// are the points of a lattice, so in lattice coordinates every neighbour
+// shell is a fixed set of integer vectors:
+//   fcc: (a,b,c) = (x, y, (z-x-y)/2 mod n)     box 2n x 2n x n
+//   bcc: (a,b,c) = (z, (x-z)/2 mod n, (y-z)/2 mod n)   box 2n x n x n
+// For n1 = n2 = n3 these boxes are plainly periodic, so with a halo every
+// neighbour of every site is at one fixed linear offset: the z offsets are
+// kernel constants held in registers and the hot loop is one LDS byte load
+// and one shift-add per neighbour.  Other shapes (or lattices too large for
+// LDS with this halo) use the per-sublattice layout with per-sublattice
+// offset tables (k_metro_cmp).
+//
+// Energy change.  LDS holds bit positions FIELD*(species-1); a site's
+// shell-n neighbour counts are packed FIELD-bit fields.  dE is contracted
+// in FP64 from the count differences:
+//   dE = sum_n sum_s (c_a[n][s] - c_b[n][s]) * (V(beta,s,n) - V(alpha,s,n))
+// (exact because a and b are not neighbours).  The energy is recounted from
+// the lattice at the start of every temperature step and carried through
+// the sweeps by summing the accepted dE.
+#include <hip/hip_runtime.h>
+
+#include "bwl_device.h"
+#include "bwl_kernels.h"
+#include "bwl_tables.h"
+
+namespace bwl {
+
+namespace {
+
+__device__ const double kKbInRyPar = 8.167333262e-5 / 13.605693122990;
+
+template <int LAT>
+constexpr int shell_size(int n) {
+  return LAT == 1 ? kFccShellSize[n] : kBccShellSize[n];
+}
+
+template <int LAT>
+constexpr int shell_begin(int n) {
+  int s = 0;
+  for (int k = 0; k < n; ++k) s += shell_size<LAT>(k);
+  return s;
+}
+
+// x / m and x % m for x < 2^22, m <= 2^12: float reciprocal, one correction.
+__device__ __forceinline__ void divmod_small(uint32_t x, uint32_t m, float inv_m, uint32_t &q,
+                                             uint32_t &r) {
+  uint32_t qq = (uint32_t)((float)x * inv_m);
+  int rr = (int)x - (int)(qq * m);
+  if (rr < 0) {
+    qq -= 1;
+    rr += (int)m;
+  } else if (rr >= (int)m) {
+    qq += 1;
+    rr -= (int)m;
+  }
+  q = qq;
+  r = (uint32_t)rr;
+}
+
+__device__ __forceinline__ uint32_t mod_small(uint32_t x, uint32_t m, float inv_m) {
+  uint32_t q, r;
+  divmod_small(x, m, inv_m, q, r);
+  return r;
+}
+
+// Map (value on round axis, first free axis, second free axis) to (x,y,z);
+// the free axes of round axis A are (A+1)%3 and (A+2)%3.
+__device__ __forceinline__ void place(int ax, int A, int B, int Cc, int &x, int &y, int &z) {
+  x = ax == 0 ? A : (ax == 1 ? Cc : B);
+  y = ax == 0 ? B : (ax == 1 ? A : Cc);
+  z = ax == 0 ? Cc : (ax == 1 ? B : A);
+}
+
+__device__ __forceinline__ int pick3(int ax, const int32_t *v) {
+  return ax == 0 ? v[0] : (ax == 1 ? v[1] : v[2]);
+}
+
+struct Round {
+  int ax;
+  int mA, mB, mC;        // extents along round axis / free axes
+  int oA1, oB1, oC1;     // grid 1 origin
+  int oA2, oB2, oC2;     // grid 2 origin
+  int sub1, sub2;
+  int KB, KC, nL1, n2;
+  float invKB, invKC, invL1, invKBC;
+  uint32_t mb, ab, mc, ac, ml, al;
+};
+
+__device__ inline Round make_round(const ParArgs &C, unsigned long long phase) {
+  const Geo &g = C.g;
+  const uint32_t plo = (uint32_t)phase, phi = (uint32_t)(phase >> 32);
+  U4 rp = philox4x32(U4{plo, phi, 0xffffffffu, C.chain_id}, C.seed_lo, C.seed_hi);
+  U4 rq = philox4x32(U4{plo, phi, 0xfffffffeu, C.chain_id}, C.seed_lo, C.seed_hi);
+  U4 rs = philox4x32(U4{plo, phi, 0xfffffffdu, C.chain_id}, C.seed_lo, C.seed_hi);
+  Round R;
+  const RoundVar &v = C.var[below(rp.w, (uint32_t)C.n_var)];
+  R.ax = v.axis;
+  const int bx = R.ax == 2 ? 0 : R.ax + 1, cx = R.ax == 0 ? 2 : R.ax - 1;
+  R.mA = pick3(R.ax, g.m);
+  R.mB = pick3(bx, g.m);
+  R.mC = pick3(cx, g.m);
+  const int nB = pick3(bx, g.n), nC = pick3(cx, g.n);
+  const bool fcc = g.lattice == 1;
+  // grid 1 origin: a random occupied site
+  R.oA1 = (int)below(rp.x, (uint32_t)R.mA);
+  R.oB1 = fcc ? (int)below(rp.y, (uint32_t)R.mB) : 2 * (int)below(rp.y, (uint32_t)nB) + (R.oA1 & 1);
+  R.oC1 = 2 * (int)below(rp.z, (uint32_t)nC) + (fcc ? ((R.oA1 + R.oB1) & 1) : (R.oA1 & 1));
+  // grid 2 origin: u further along the round axis, free elsewhere
+  const int a2 = R.oA1 + v.u;
+  R.oB2 = fcc ? (int)below(rq.x, (uint32_t)R.mB) : 2 * (int)below(rq.x, (uint32_t)nB) + (a2 & 1);
+  R.oC2 = 2 * (int)below(rq.y, (uint32_t)nC) + (fcc ? ((a2 + R.oB2) & 1) : (a2 & 1));
+  R.oA2 = a2 >= R.mA ? a2 - R.mA : a2;
+  int x, y, z;
+  place(R.ax, R.oA1, R.oB1, R.oC1, x, y, z);
+  R.sub1 = sub_of(g.lattice, x & 1, y & 1);
+  place(R.ax, R.oA2, R.oB2, R.oC2, x, y, z);
+  R.sub2 = sub_of(g.lattice, x & 1, y & 1);
+  R.KB = pick3(bx, C.K);
+  R.KC = pick3(cx, C.K);
+  R.nL1 = v.nL1;
+  R.n2 = v.n2;
+  R.invKB = 1.0f / (float)R.KB;
+  R.invKC = 1.0f / (float)R.KC;
+  R.invL1 = 1.0f / (float)R.nL1;
+  R.invKBC = 1.0f / (float)(R.KB * R.KC);
+  const uint32_t nub = (uint32_t)pick3(bx, C.n_units), nuc = (uint32_t)pick3(cx, C.n_units);
+  R.mb = C.units[bx][below(rq.z, nub)];
+  R.mc = C.units[cx][below(rq.w, nuc)];
+  R.ml = C.muls[v.mul_off + below(rs.x, (uint32_t)v.n_mul)];
+  R.ab = below(rs.y, (uint32_t)R.KB);
+  R.ac = below(rs.z, (uint32_t)R.KC);
+  R.al = below(rs.w, (uint32_t)R.nL1);
+  return R;
+}
+
+// Doubled coordinates of a grid site (layer jl, free-axis indices kb, kc).
+__device__ __forceinline__ void grid_xyz(const ParArgs &C, const Round &R, int oA, int oB, int oC,
+                                         int jl, int kb, int kc, int &x, int &y, int &z) {
+  int pa = oA + jl * C.L, pb = oB + kb * C.stride, pc = oC + kc * C.stride;
+  if (pa >= R.mA) pa -= R.mA;
+  if (pb >= R.mB) pb -= R.mB;
+  if (pc >= R.mC) pc -= R.mC;
+  place(R.ax, pa, pb, pc, x, y, z);
+}
+
+// ---- natural layout --------------------------------------------------
+template <int LAT>
+__device__ __forceinline__ void to_natural(int x, int y, int z, int n, int &a, int &b, int &c) {
+  const int m = 2 * n;
+  if (LAT == 1) {
+    a = x;
+    b = y;
+    int t = (z - x - y + 2 * m) >> 1;  // in (0, 3n)
+    if (t >= n) t -= n;
+    if (t >= n) t -= n;
+    c = t;
+  } else {
+    a = z;
+    int tb = (x - z + m) >> 1, tc = (y - z + m) >> 1;  // in (0, 2n)
+    b = tb >= n ? tb - n : tb;
+    c = tc >= n ? tc - n : tc;
+  }
+}
+
+__device__ __forceinline__ int nat_addr(const NatGeo &N, int a, int b, int c) {
+  return ((c + N.h[2]) * N.P[1] + (b + N.h[1])) * N.P[0] + (a + N.h[0]);
+}
+
+// Store v at the interior cell and its periodic images (D >= 2h per axis).
+__device__ __forceinline__ void nat_put(int8_t *lat, const NatGeo &N, int a, int b, int c, int8_t v) {
+  int pa[2], pb[2], pc[2], ka = 1, kb = 1, kc = 1;
+  pa[0] = a + N.h[0];
+  if (a < N.h[0]) pa[ka++] = a + N.h[0] + N.D[0];
+  else if (a >= N.D[0] - N.h[0]) pa[ka++] = a + N.h[0] - N.D[0];
+  pb[0] = b + N.h[1];
+  if (b < N.h[1]) pb[kb++] = b + N.h[1] + N.D[1];
+  else if (b >= N.D[1] - N.h[1]) pb[kb++] = b + N.h[1] - N.D[1];
+  pc[0] = c + N.h[2];
+  if (c < N.h[2]) pc[kc++] = c + N.h[2] + N.D[2];
+  else if (c >= N.D[2] - N.h[2]) pc[kc++] = c + N.h[2] - N.D[2];
+  for (int i = 0; i < kc; ++i)
+    for (int j = 0; j < kb; ++j)
+      for (int k = 0; k < ka; ++k) lat[(pc[i] * N.P[1] + pb[j]) * N.P[0] + pa[k]] = v;
+}
+
+template <int LAT>
+__device__ void nat_stage(int8_t *lat, const int8_t *__restrict__ compact, const Geo &g,
+                          const NatGeo &N, int shift) {
+  const int n = g.n[0], m = 2 * n;
+  const int total = N.P[0] * N.P[1] * N.P[2];
+  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    int pc = idx / (N.P[0] * N.P[1]);
+    int r = idx - pc * N.P[0] * N.P[1];
+    int pb = r / N.P[0];
+    int pa = r - pb * N.P[0];
+    int a = pmod(pa - N.h[0], N.D[0]), b = pmod(pb - N.h[1], N.D[1]), c = pmod(pc - N.h[2], N.D[2]);
+    int x, y, z;
+    if (LAT == 1) {
+      x = a; y = b; z = (a + b + 2 * c) % m;
+    } else {
+      z = a; x = (a + 2 * b) % m; y = (a + 2 * c) % m;
+    }
+    int v = compact[compact_addr(g, sub_of(g.lattice, x & 1, y & 1), x >> 1, y >> 1, z >> 1)];
+    lat[idx] = (int8_t)(shift * (v - 1));
+  }
+}
+
+template <int LAT>
+__device__ void nat_unstage(const int8_t *lat, int8_t *__restrict__ compact, const Geo &g,
+                            const NatGeo &N, int shift) {
+  const int n = g.n[0], m = 2 * n;
+  const int total = N.D[0] * N.D[1] * N.D[2];
+  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    int c = idx / (N.D[0] * N.D[1]);
+    int r = idx - c * N.D[0] * N.D[1];
+    int b = r / N.D[0];
+    int a = r - b * N.D[0];
+    int x, y, z;
+    if (LAT == 1) {
+      x = a; y = b; z = (a + b + 2 * c) % m;
+    } else {
+      z = a; x = (a + 2 * b) % m; y = (a + 2 * c) % m;
+    }
+    compact[compact_addr(g, sub_of(g.lattice, x & 1, y & 1), x >> 1, y >> 1, z >> 1)] =
+        (int8_t)(lat[nat_addr(N, a, b, c)] / shift + 1);
+  }
+}
+
+// Packed shell counts with kernel-constant neighbour offsets.
+template <int LAT, int R, typename CW>
+__device__ __forceinline__ void count_nat(const int8_t *lat, int addr, const int (&off)[shell_begin<LAT>(R)],
+                                          CW (&c)[R]) {
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    CW acc = 0;
+#pragma unroll
+    for (int t = shell_begin<LAT>(n); t < shell_begin<LAT>(n + 1); ++t)
+      acc += (CW)1 << lat[addr + off[t]];
+    c[n] = acc;
+  }
+}
+
+// accept with probability min(1, exp(-beta*dE)) for dE >= 0: a float
+// exponential decides unless u falls within 1e-5 (relative) of it, then FP64.
+__device__ __forceinline__ bool metropolis_accept(double u, double x) {
+  const float ef = __expf((float)x);
+  const double e = (double)ef;
+  if (u < e * (1.0 - 1e-5)) return true;
+  if (u > e * (1.0 + 1e-5) + 1e-300) return false;
+  return u < exp(x);
+}
+
+template <int LAT, int R, typename CW, int FIELD>
+__global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ chains) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const ParArgs &C = chains[blockIdx.x];
+  const Geo &g = C.g;
+  const NatGeo &N = C.nat;
+  constexpr int Z = shell_begin<LAT>(R);
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  const int S = g.S;
+  const int Sp = (S + 1) & ~1;
+  int8_t *lat = (int8_t *)smem;
+  size_t o = (N.lds_bytes + 15) & ~15;
+  double *dtab = (double *)(smem + o);  // [alpha][beta][n][s]: V(beta,s,n)-V(alpha,s,n)
+  o += sizeof(double) * S * S * R * Sp;
+  double *vrow = (double *)(smem + o);  // [x][n][s]: V(x,s,n)
+  o += sizeof(double) * S * R * Sp;
+  double *red = (double *)(smem + o);
+  unsigned long long *redu = (unsigned long long *)(smem + o);
+
+  for (int i = tid; i < S * S * R * Sp; i += nthr) {
+    int s = i % Sp, n = (i / Sp) % R, be = (i / (Sp * R)) % S, al = i / (Sp * R * S);
+    dtab[i] = s < S ? C.vex[be + S * (s + S * n)] - C.vex[al + S * (s + S * n)] : 0.0;
+  }
+  for (int i = tid; i < S * R * Sp; i += nthr) {
+    int s = i % Sp, n = (i / Sp) % R, x = i / (Sp * R);
+    vrow[i] = s < S ? C.vex[x + S * (s + S * n)] : 0.0;
+  }
+  int off[Z];
+#pragma unroll
+  for (int t = 0; t < Z; ++t) off[t] = N.off[t];
+  nat_stage<LAT>(lat, C.compact, g, N, FIELD);
+  __syncthreads();
+
+  const int n_cells = g.n[0];
+  unsigned long long phase = C.phase0;
+  long long acc_total = 0, trials_total = 0;
+
+  for (int t = 0; t < C.T_steps; ++t) {
+    const double temp = C.T0 + (double)t * C.dT;
+    const double sim_temp = temp * kKbInRyPar;
+    const double beta = 1.0 / sim_temp;
+    // recount
+    double part = 0.0;
+    const int nsites = N.D[0] * N.D[1] * N.D[2];
+    for (int i = tid; i < nsites; i += nthr) {
+      int c = i / (N.D[0] * N.D[1]);
+      int r = i - c * N.D[0] * N.D[1];
+      int b = r / N.D[0];
+      int a = r - b * N.D[0];
+      int ad = nat_addr(N, a, b, c);
+      int sp = lat[ad] / FIELD;
+      CW cnt[R];
+      count_nat<LAT, R, CW>(lat, ad, off, cnt);
+#pragma unroll
+      for (int n = 0; n < R; ++n) {
+        const double *vr = vrow + (sp * R + n) * Sp;
+        for (int s = 0; s < S; ++s)
+          part += (double)((cnt[n] >> (FIELD * s)) & ((1 << FIELD) - 1)) * vr[s];
+      }
+    }
+    double E = 0.5 * block_sum(part, red);
+    double step_E = 0.0, step_Esq = 0.0;
+    unsigned long long acc_local = 0;
+    long long trials_sampled = 0;
+    const int nsw = C.burn_sweeps + C.n_sweeps;
+    for (int sw = 0; sw < nsw; ++sw) {
+      const bool sampled = sw >= C.burn_sweeps;
+      double dE_sum = 0.0;
+      long long done = 0;
+      while (done < C.min_trials_per_sweep) {
+        const Round RG = make_round(C, phase);
+        uint32_t jl, rem, kb, kc, djl, drem, dkb, dkc;
+        divmod_small((uint32_t)tid, (uint32_t)(RG.KB * RG.KC), RG.invKBC, jl, rem);
+        divmod_small(rem, (uint32_t)RG.KB, RG.invKB, kc, kb);
+        divmod_small((uint32_t)nthr, (uint32_t)(RG.KB * RG.KC), RG.invKBC, djl, drem);
+        divmod_small(drem, (uint32_t)RG.KB, RG.invKB, dkc, dkb);
+        for (int q = tid; q < RG.n2; q += nthr) {
+          int x, y, z, aA, bA, cA, aB, bB, cB;
+          grid_xyz(C, RG, RG.oA2, RG.oB2, RG.oC2, (int)jl, (int)kb, (int)kc, x, y, z);
+          to_natural<LAT>(x, y, z, n_cells, aB, bB, cB);
+          const uint32_t jl1 = mod_small(RG.ml * jl + RG.al, (uint32_t)RG.nL1, RG.invL1);
+          const uint32_t kb1 = mod_small(RG.mb * kb + RG.ab, (uint32_t)RG.KB, RG.invKB);
+          const uint32_t kc1 = mod_small(RG.mc * kc + RG.ac, (uint32_t)RG.KC, RG.invKC);
+          grid_xyz(C, RG, RG.oA1, RG.oB1, RG.oC1, (int)jl1, (int)kb1, (int)kc1, x, y, z);
+          to_natural<LAT>(x, y, z, n_cells, aA, bA, cA);
+          kb += dkb;
+          uint32_t cy = kb >= (uint32_t)RG.KB;
+          kb -= cy * RG.KB;
+          kc += dkc + cy;
+          cy = kc >= (uint32_t)RG.KC;
+          kc -= cy * RG.KC;
+          jl += djl + cy;
+
+          const int addrA = nat_addr(N, aA, bA, cA), addrB = nat_addr(N, aB, bB, cB);
+          const int codeA = lat[addrA], codeB = lat[addrB];
+          if (codeA == codeB) {
+            if (sampled) acc_local++;
+            continue;
+          }
+          CW ca[R], cb[R];
+          count_nat<LAT, R, CW>(lat, addrA, off, ca);
+          count_nat<LAT, R, CW>(lat, addrB, off, cb);
+          const int alpha = codeA / FIELD, betasp = codeB / FIELD;
+          const double *dt = dtab + ((alpha * S + betasp) * R) * Sp;
+          double dE = 0.0;
+#pragma unroll
+          for (int n = 0; n < R; ++n) {
+            for (int s = 0; s < S; ++s) {
+              int d = (int)((ca[n] >> (FIELD * s)) & ((1 << FIELD) - 1)) -
+                      (int)((cb[n] >> (FIELD * s)) & ((1 << FIELD) - 1));
+              dE += (double)d * dt[n * Sp + s];
+            }
+          }
+          bool acc = dE < 0.0;
+          if (!acc) {
+            U4 r = philox4x32(U4{(uint32_t)phase, (uint32_t)(phase >> 32), (uint32_t)q, C.chain_id},
+                              C.seed_lo, C.seed_hi);
+            acc = metropolis_accept(u01_53(r.x, r.y), -(beta * dE));
+          }
+          if (acc) {
+            nat_put(lat, N, aA, bA, cA, (int8_t)codeB);
+            nat_put(lat, N, aB, bB, cB, (int8_t)codeA);
+            dE_sum += dE;
+            if (sampled) acc_local++;
+          }
+        }
+        done += RG.n2;
+        ++phase;
+        __syncthreads();
+      }
+      trials_total += done;
+      E += block_sum(dE_sum, red);
+      if (sampled) {
+        trials_sampled += done;
+        if (tid == 0) {
+          step_E = step_E + E;
+          step_Esq = step_Esq + E * E;
+          if (C.e_trace) C.e_trace[(long long)t * C.n_sweeps + (sw - C.burn_sweeps)] = E;
+        }
+      }
+    }
+    unsigned long long acc_sum = block_sum_u64(acc_local, redu);
+    if (tid == 0) {
+      const double ns = (double)C.n_sweeps;
+      const double na = (double)g.n_atoms;
+      C.energies_of_T[t] = step_E / ns / na;
+      double Cv = (step_Esq / ns - (step_E / ns) * (step_E / ns)) / (sim_temp * temp) / na;
+      if (Cv < 0.0) Cv = 0.0;
+      if (temp <= 0.0) Cv = 0.0;
+      C.C_of_T[t] = Cv;
+      C.acc_of_T[t] = (double)acc_sum / (double)trials_sampled;
+      acc_total += (long long)acc_sum;
+    }
+  }
+  __syncthreads();
+  nat_unstage<LAT>(lat, C.compact, g, N, FIELD);
+  if (tid == 0) {
+    C.counters[0] = (long long)phase;
+    C.counters[1] += acc_total;
+    C.counters[2] += trials_total;
+  }
+}
+
+// ---- per-sublattice (compact) layout fallback ------------------------------
+template <int LAT, int R, typename CW>
+__device__ __forceinline__ void count_cmp(const int8_t *lat, int addr, const int32_t *__restrict__ off,
+                                          CW (&c)[R]) {
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    CW acc = 0;
+    for (int t = shell_begin<LAT>(n); t < shell_begin<LAT>(n + 1); ++t)
+      acc += (CW)1 << lat[addr + off[t]];
+    c[n] = acc;
+  }
+}
+
+template <int LAT, int R, typename CW, int FIELD>
+__global__ __launch_bounds__(512) void k_metro_cmp(const ParArgs *__restrict__ chains) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const ParArgs &C = chains[blockIdx.x];
+  const Geo &g = C.g;
+  constexpr int Z = shell_begin<LAT>(R);
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  const int S = g.S;
+  int8_t *lat = (int8_t *)smem;
+  size_t o = (g.lds_bytes + 15) & ~15;
+  double *vt = (double *)(smem + o);  // vt[(n*S + x)*S + s] = V_ex(x+1, s+1, n+1)
+  o += sizeof(double) * S * S * R;
+  double *red = (double *)(smem + o);
+  unsigned long long *redu = (unsigned long long *)(smem + o);
+  int32_t *offs = (int32_t *)(red + (nthr / 64 + 1));  // [nsub][Z]
+
+  for (int i = tid; i < S * S * R; i += nthr) {
+    int n = i / (S * S), x = (i / S) % S, s = i % S;
+    vt[i] = C.vex[x + S * (s + S * n)];
+  }
+  for (int i = tid; i < g.nsub * Z; i += nthr) offs[i] = C.off[i];
+  stage_lattice(lat, C.compact, g, FIELD);
+  __syncthreads();
+
+  unsigned long long phase = C.phase0;
+  long long acc_total = 0, trials_total = 0;
+  const int cells = g.n[0] * g.n[1] * g.n[2];
+
+  for (int t = 0; t < C.T_steps; ++t) {
+    const double temp = C.T0 + (double)t * C.dT;
+    const double sim_temp = temp * kKbInRyPar;
+    const double beta = 1.0 / sim_temp;
+    double part = 0.0;
+    for (int i = tid; i < g.nsub * cells; i += nthr) {
+      int s = i / cells;
+      int r = i - s * cells;
+      int cz = r / (g.n[0] * g.n[1]);
+      r -= cz * g.n[0] * g.n[1];
+      int cy = r / g.n[0];
+      int cx = r - cy * g.n[0];
+      int ad = padded_addr(g, s, cx, cy, cz);
+      int a = lat[ad] / FIELD;
+      CW cnt[R];
+      count_cmp<LAT, R, CW>(lat, ad, offs + s * Z, cnt);
+#pragma unroll
+      for (int n = 0; n < R; ++n) {
+        const double *va = vt + (n * S + a) * S;
+        for (int b = 0; b < S; ++b)
+          part += (double)((cnt[n] >> (FIELD * b)) & ((1 << FIELD) - 1)) * va[b];
+      }
+    }
+    double E = 0.5 * block_sum(part, red);
+    double step_E = 0.0, step_Esq = 0.0;
+    unsigned long long acc_local = 0;
+    long long trials_sampled = 0;
+    const int nsw = C.burn_sweeps + C.n_sweeps;
+    for (int sw = 0; sw < nsw; ++sw) {
+      const bool sampled = sw >= C.burn_sweeps;
+      double dE_sum = 0.0;
+      long long done = 0;
+      while (done < C.min_trials_per_sweep) {
+        const Round RG = make_round(C, phase);
+        const int32_t *off1 = offs + RG.sub1 * Z;
+        const int32_t *off2 = offs + RG.sub2 * Z;
+        uint32_t jl, rem, kb, kc, djl, drem, dkb, dkc;
+        divmod_small((uint32_t)tid, (uint32_t)(RG.KB * RG.KC), RG.invKBC, jl, rem);
+        divmod_small(rem, (uint32_t)RG.KB, RG.invKB, kc, kb);
+        divmod_small((uint32_t)nthr, (uint32_t)(RG.KB * RG.KC), RG.invKBC, djl, drem);
+        divmod_small(drem, (uint32_t)RG.KB, RG.invKB, dkc, dkb);
+        for (int q = tid; q < RG.n2; q += nthr) {
+          int x, y, z, cellA[3], cellB[3];
+          grid_xyz(C, RG, RG.oA2, RG.oB2, RG.oC2, (int)jl, (int)kb, (int)kc, x, y, z);
+          cellB[0] = x >> 1; cellB[1] = y >> 1; cellB[2] = z >> 1;
+          const uint32_t jl1 = mod_small(RG.ml * jl + RG.al, (uint32_t)RG.nL1, RG.invL1);
+          const uint32_t kb1 = mod_small(RG.mb * kb + RG.ab, (uint32_t)RG.KB, RG.invKB);
+          const uint32_t kc1 = mod_small(RG.mc * kc + RG.ac, (uint32_t)RG.KC, RG.invKC);
+          grid_xyz(C, RG, RG.oA1, RG.oB1, RG.oC1, (int)jl1, (int)kb1, (int)kc1, x, y, z);
+          cellA[0] = x >> 1; cellA[1] = y >> 1; cellA[2] = z >> 1;
+          kb += dkb;
+          uint32_t cy = kb >= (uint32_t)RG.KB;
+          kb -= cy * RG.KB;
+          kc += dkc + cy;
+          cy = kc >= (uint32_t)RG.KC;
+          kc -= cy * RG.KC;
+          jl += djl + cy;
+          const int addrA = padded_addr(g, RG.sub1, cellA[0], cellA[1], cellA[2]);
+          const int addrB = padded_addr(g, RG.sub2, cellB[0], cellB[1], cellB[2]);
+          const int codeA = lat[addrA], codeB = lat[addrB];
+          if (codeA == codeB) {
+            if (sampled) acc_local++;
+            continue;
+          }
+          CW ca[R], cb[R];
+          count_cmp<LAT, R, CW>(lat, addrA, off1, ca);
+          count_cmp<LAT, R, CW>(lat, addrB, off2, cb);
+          const int alpha = codeA / FIELD, betasp = codeB / FIELD;
+          double dE = 0.0;
+#pragma unroll
+          for (int n = 0; n < R; ++n) {
+            const double *va = vt + (n * S + alpha) * S;
+            const double *vb = vt + (n * S + betasp) * S;
+            for (int s = 0; s < S; ++s) {
+              int d = (int)((ca[n] >> (FIELD * s)) & ((1 << FIELD) - 1)) -
+                      (int)((cb[n] >> (FIELD * s)) & ((1 << FIELD) - 1));
+              if (d) dE += (double)d * (vb[s] - va[s]);
+            }
+          }
+          bool acc = dE < 0.0;
+          if (!acc) {
+            U4 r = philox4x32(U4{(uint32_t)phase, (uint32_t)(phase >> 32), (uint32_t)q, C.chain_id},
+                              C.seed_lo, C.seed_hi);
+            acc = metropolis_accept(u01_53(r.x, r.y), -(beta * dE));
+          }
+          if (acc) {
+            write_site(lat, g, RG.sub1, cellA[0], cellA[1], cellA[2], (int8_t)codeB, 0, 1);
+            write_site(lat, g, RG.sub2, cellB[0], cellB[1], cellB[2], (int8_t)codeA, 0, 1);
+            dE_sum += dE;
+            if (sampled) acc_local++;
+          }
+        }
+        done += RG.n2;
+        ++phase;
+        __syncthreads();
+      }
+      trials_total += done;
+      E += block_sum(dE_sum, red);
+      if (sampled) {
+        trials_sampled += done;
+        if (tid == 0) {
+          step_E = step_E + E;
+          step_Esq = step_Esq + E * E;
+          if (C.e_trace) C.e_trace[(long long)t * C.n_sweeps + (sw - C.burn_sweeps)] = E;
+        }
+      }
+    }
+    unsigned long long acc_sum = block_sum_u64(acc_local, redu);
+    if (tid == 0) {
+      const double ns = (double)C.n_sweeps;
+      const double na = (double)g.n_atoms;
+      C.energies_of_T[t] = step_E / ns / na;
+      double Cv = (step_Esq / ns - (step_E / ns) * (step_E / ns)) / (sim_temp * temp) / na;
+      if (Cv < 0.0) Cv = 0.0;
+      if (temp <= 0.0) Cv = 0.0;
+      C.C_of_T[t] = Cv;
+      C.acc_of_T[t] = (double)acc_sum / (double)trials_sampled;
+      acc_total += (long long)acc_sum;
+    }
+  }
+  __syncthreads();
+  unstage_lattice(lat, C.compact, g, FIELD);
+  if (tid == 0) {
+    C.counters[0] = (long long)phase;
+    C.counters[1] += acc_total;
+    C.counters[2] += trials_total;
+  }
+}
+
+template <int LAT, int R>
+const void *pick(bool nat, bool wide) {
+  if (nat)
+    return wide ? (const void *)k_metro_nat<LAT, R, uint64_t, 8>
+                : (const void *)k_metro_nat<LAT, R, uint32_t, 6>;
+  return wide ? (const void *)k_metro_cmp<LAT, R, uint64_t, 8>
+              : (const void *)k_metro_cmp<LAT, R, uint32_t, 6>;
+}
+
+const void *par_kernel(int lattice, int R, bool nat, bool wide) {
+  if (lattice == 1) {
+    switch (R) {
+      case 1: return pick<1, 1>(nat, wide);
+      case 2: return pick<1, 2>(nat, wide);
+      case 3: return pick<1, 3>(nat, wide);
+      case 4: return pick<1, 4>(nat, wide);
+      case 5: return pick<1, 5>(nat, wide);
+      case 6: return pick<1, 6>(nat, wide);
+    }
+  } else {
+    switch (R) {
+      case 1: return pick<2, 1>(nat, wide);
+      case 2: return pick<2, 2>(nat, wide);
+      case 3: return pick<2, 3>(nat, wide);
+      case 4: return pick<2, 4>(nat, wide);
+      case 5: return pick<2, 5>(nat, wide);
+      case 6: return pick<2, 6>(nat, wide);
+      case 7: return pick<2, 7>(nat, wide);
+      case 8: return pick<2, 8>(nat, wide);
+      case 9: return pick<2, 9>(nat, wide);
+      case 10: return pick<2, 10>(nat, wide);
+    }
+  }
+  return nullptr;
+}
+
+}  // namespace
+
+size_t par_lds_bytes(const Geo &g, const NatGeo *nat, int threads) {
+  const size_t Sp = (size_t)((g.S + 1) & ~1);
+  const size_t red = sizeof(double) * (threads / 64 + 1);
+  if (nat)
+    return (size_t)((nat->lds_bytes + 15) & ~15) + sizeof(double) * g.S * g.S * g.R * Sp +
+           sizeof(double) * g.S * g.R * Sp + red;
+  return (size_t)((g.lds_bytes + 15) & ~15) + sizeof(double) * g.S * g.S * g.R + red +
+         sizeof(int32_t) * g.nsub * g.z;
+}
+
+hipError_t launch_metro_par(const ParArgs *args_dev, int n_chains, int lattice, int R, bool nat,
+                            size_t lds, int threads, bool wide, hipStream_t s) {
+  const void *fn = par_kernel(lattice, R, nat, wide);
+  if (!fn) return hipErrorInvalidValue;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  void *args[] = {(void *)&args_dev};
+  return hipLaunchKernel(fn, dim3(n_chains), dim3(threads), args, lds, s);
+}
+
+}  // namespace bwl
