@@ -375,7 +375,9 @@ int bwl_metropolis_anneal(bwl_ctx *c, const bwl_anneal_params *p, double *energi
     lds = par_lds_bytes(g, nullptr, threads);
   }
   if (lds > (size_t)lds_limit()) return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
-  const bool wide = g.S > 5;
+  // packed counts: natural kernel uses 8-bit fields (u32 holds 4 species),
+  // the per-sublattice kernel 6-bit fields (u32 holds 5)
+  const bool wide = use_nat ? g.S > 4 : g.S > 5;
   const size_t T = (size_t)p->T_steps;
   size_t off_args = 0;
   size_t off_muls = 256 * ((sizeof(ParArgs) + 255) / 256);
@@ -424,8 +426,8 @@ int bwl_metropolis_anneal(bwl_ctx *c, const bwl_anneal_params *p, double *energi
                         hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipMemcpyAsync(a.counters, cnt0, sizeof(cnt0), hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipEventRecord(c->ev0, c->stream));
-  HIP_OK(launch_metro_par((const ParArgs *)(base + off_args), 1, g.lattice, g.R, use_nat, lds,
-                          threads, wide, c->stream));
+  HIP_OK(launch_metro_par((const ParArgs *)(base + off_args), 1, g.lattice, g.R, g.n[0], use_nat,
+                          lds, threads, wide, c->stream));
   HIP_OK(hipEventRecord(c->ev1, c->stream));
   std::vector<double> out(3 * T);
   HIP_OK(hipMemcpyAsync(out.data(), a.energies_of_T, 3 * T * sizeof(double), hipMemcpyDeviceToHost,

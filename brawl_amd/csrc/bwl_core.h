@@ -17,6 +17,8 @@
 
 #include <stdint.h>
 
+#include "bwl_tables.h"
+
 namespace bwl {
 
 constexpr int kMaxShells = 10;
@@ -84,6 +86,65 @@ struct NatGeo {
   int32_t lds_bytes;
   int32_t off[kMaxNbr];  // linear neighbour offsets in shell/list order
 };
+
+// Lattice-coordinate delta of the reference neighbour offset t.
+__host__ __device__ constexpr void nat_delta(int lattice, int t, int &da, int &db, int &dc) {
+  if (lattice == 1) {
+    const int dx = kFccOffsets[t][0], dy = kFccOffsets[t][1], dz = kFccOffsets[t][2];
+    da = dx;
+    db = dy;
+    dc = (dz - dx - dy) / 2;
+  } else {
+    const int dx = kBccOffsets[t][0], dy = kBccOffsets[t][1], dz = kBccOffsets[t][2];
+    da = dz;
+    db = (dx - dz) / 2;
+    dc = (dy - dz) / 2;
+  }
+}
+
+constexpr int kNatBudget = 150 * 1024;  // LDS bytes the natural lattice may take
+
+// Natural layout of a cubic supercell with n cells per axis and R shells.
+// The padded row length P0 is made 2 mod 4 and P1 odd when that fits the
+// budget, so that 2-cell steps along either free axis of a round are odd
+// numbers of LDS dwords (bank-conflict-free byte reads across a wavefront).
+// Returns false when an axis is shorter than twice its halo.
+__host__ __device__ constexpr bool nat_dims(int lattice, int R, int n, NatGeo &G) {
+  int z = 0;
+  for (int s = 0; s < R; ++s) z += lattice == 1 ? kFccShellSize[s] : kBccShellSize[s];
+  G.D[0] = 2 * n;
+  G.D[1] = lattice == 1 ? 2 * n : n;
+  G.D[2] = n;
+  int h[3] = {0, 0, 0};
+  for (int t = 0; t < z; ++t) {
+    int d[3] = {0, 0, 0};
+    nat_delta(lattice, t, d[0], d[1], d[2]);
+    for (int a = 0; a < 3; ++a) {
+      const int v = d[a] < 0 ? -d[a] : d[a];
+      if (v > h[a]) h[a] = v;
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    G.h[a] = h[a];
+    if (G.D[a] < 2 * h[a]) return false;
+    G.P[a] = G.D[a] + 2 * h[a];
+  }
+  int p0 = G.P[0], p1 = G.P[1];
+  if (p0 % 4 == 0) p0 += 2;
+  if (p1 % 2 == 0) p1 += 1;
+  if (p0 * p1 * G.P[2] <= kNatBudget) {
+    G.P[0] = p0;
+    G.P[1] = p1;
+  }
+  G.lds_bytes = G.P[0] * G.P[1] * G.P[2];
+  for (int t = 0; t < kMaxNbr; ++t) G.off[t] = 0;
+  for (int t = 0; t < z; ++t) {
+    int da = 0, db = 0, dc = 0;
+    nat_delta(lattice, t, da, db, dc);
+    G.off[t] = (dc * G.P[1] + db) * G.P[0] + da;
+  }
+  return true;
+}
 
 // Padded LDS address of the interior cell (s, cx, cy, cz).
 __host__ __device__ inline int padded_addr(const Geo &g, int s, int cx, int cy, int cz) {

@@ -299,33 +299,7 @@ int plan_parallel(const Geo &g, ParPlan &p, std::string &err) {
 bool make_nat(const Geo &g, NatGeo &nat) {
   memset(&nat, 0, sizeof(nat));
   if (g.n[0] != g.n[1] || g.n[1] != g.n[2]) return false;
-  const int n = g.n[0];
-  const int8_t(*tab)[3] = g.lattice == BWL_FCC ? kFccOffsets : kBccOffsets;
-  int d[kMaxNbr][3];
-  for (int t = 0; t < g.z; ++t) {
-    const int dx = tab[t][0], dy = tab[t][1], dz = tab[t][2];
-    if (g.lattice == BWL_FCC) {
-      d[t][0] = dx; d[t][1] = dy; d[t][2] = (dz - dx - dy) / 2;
-    } else {
-      d[t][0] = dz; d[t][1] = (dx - dz) / 2; d[t][2] = (dy - dz) / 2;
-    }
-  }
-  if (g.lattice == BWL_FCC) {
-    nat.D[0] = 2 * n; nat.D[1] = 2 * n; nat.D[2] = n;
-  } else {
-    nat.D[0] = 2 * n; nat.D[1] = n; nat.D[2] = n;
-  }
-  for (int a = 0; a < 3; ++a) {
-    int h = 0;
-    for (int t = 0; t < g.z; ++t) h = std::max(h, std::abs(d[t][a]));
-    nat.h[a] = h;
-    if (nat.D[a] < 2 * h) return false;
-    nat.P[a] = nat.D[a] + 2 * h;
-  }
-  nat.lds_bytes = nat.P[0] * nat.P[1] * nat.P[2];
-  for (int t = 0; t < g.z; ++t)
-    nat.off[t] = (d[t][2] * nat.P[1] + d[t][1]) * nat.P[0] + d[t][0];
-  return true;
+  return nat_dims(g.lattice, g.R, g.n[0], nat);
 }
 
 int check_config(const Geo &g, const int8_t *config, std::string &err) {

@@ -78,7 +78,7 @@ hipError_t launch_site_energy(const SiteArgs &a, hipStream_t s);
 size_t seq_lds_bytes(const Geo &g);
 hipError_t launch_metro_seq(const SeqArgs &a, hipStream_t s);
 size_t par_lds_bytes(const Geo &g, const NatGeo *nat, int threads);
-hipError_t launch_metro_par(const ParArgs *args_dev, int n_chains, int lattice, int R, bool nat,
-                            size_t lds, int threads, bool wide, hipStream_t s);
+hipError_t launch_metro_par(const ParArgs *args_dev, int n_chains, int lattice, int R, int n,
+                            bool nat, size_t lds, int threads, bool wide, hipStream_t s);
 
 }  // namespace bwl

@@ -179,38 +179,55 @@ __device__ __forceinline__ void to_natural(int x, int y, int z, int n, int &a, i
   }
 }
 
-__device__ __forceinline__ int nat_addr(const NatGeo &N, int a, int b, int c) {
-  return ((c + N.h[2]) * N.P[1] + (b + N.h[1])) * N.P[0] + (a + N.h[0]);
+// Extents of the natural box: compile-time constants for the specialised
+// shapes, kernel arguments otherwise.
+struct ND {
+  int D0, D1, D2, h0, h1, h2, P0, P1, P2;
+};
+
+__host__ __device__ constexpr ND nd_of(const NatGeo &N) {
+  return ND{N.D[0], N.D[1], N.D[2], N.h[0], N.h[1], N.h[2], N.P[0], N.P[1], N.P[2]};
+}
+
+template <int LAT, int R, int NC>
+__host__ __device__ constexpr NatGeo nat_geo_c() {
+  NatGeo G{};
+  nat_dims(LAT, R, NC, G);
+  return G;
+}
+
+__device__ __forceinline__ int nat_addr(const ND &N, int a, int b, int c) {
+  return ((c + N.h2) * N.P1 + (b + N.h1)) * N.P0 + (a + N.h0);
 }
 
 // Store v at the interior cell and its periodic images (D >= 2h per axis).
-__device__ __forceinline__ void nat_put(int8_t *lat, const NatGeo &N, int a, int b, int c, int8_t v) {
+__device__ __forceinline__ void nat_put(int8_t *lat, const ND &N, int a, int b, int c, int8_t v) {
   int pa[2], pb[2], pc[2], ka = 1, kb = 1, kc = 1;
-  pa[0] = a + N.h[0];
-  if (a < N.h[0]) pa[ka++] = a + N.h[0] + N.D[0];
-  else if (a >= N.D[0] - N.h[0]) pa[ka++] = a + N.h[0] - N.D[0];
-  pb[0] = b + N.h[1];
-  if (b < N.h[1]) pb[kb++] = b + N.h[1] + N.D[1];
-  else if (b >= N.D[1] - N.h[1]) pb[kb++] = b + N.h[1] - N.D[1];
-  pc[0] = c + N.h[2];
-  if (c < N.h[2]) pc[kc++] = c + N.h[2] + N.D[2];
-  else if (c >= N.D[2] - N.h[2]) pc[kc++] = c + N.h[2] - N.D[2];
+  pa[0] = a + N.h0;
+  if (a < N.h0) pa[ka++] = a + N.h0 + N.D0;
+  else if (a >= N.D0 - N.h0) pa[ka++] = a + N.h0 - N.D0;
+  pb[0] = b + N.h1;
+  if (b < N.h1) pb[kb++] = b + N.h1 + N.D1;
+  else if (b >= N.D1 - N.h1) pb[kb++] = b + N.h1 - N.D1;
+  pc[0] = c + N.h2;
+  if (c < N.h2) pc[kc++] = c + N.h2 + N.D2;
+  else if (c >= N.D2 - N.h2) pc[kc++] = c + N.h2 - N.D2;
   for (int i = 0; i < kc; ++i)
     for (int j = 0; j < kb; ++j)
-      for (int k = 0; k < ka; ++k) lat[(pc[i] * N.P[1] + pb[j]) * N.P[0] + pa[k]] = v;
+      for (int k = 0; k < ka; ++k) lat[(pc[i] * N.P1 + pb[j]) * N.P0 + pa[k]] = v;
 }
 
 template <int LAT>
 __device__ void nat_stage(int8_t *lat, const int8_t *__restrict__ compact, const Geo &g,
-                          const NatGeo &N, int shift) {
+                          const ND &N, int shift) {
   const int n = g.n[0], m = 2 * n;
-  const int total = N.P[0] * N.P[1] * N.P[2];
+  const int total = N.P0 * N.P1 * N.P2;
   for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-    int pc = idx / (N.P[0] * N.P[1]);
-    int r = idx - pc * N.P[0] * N.P[1];
-    int pb = r / N.P[0];
-    int pa = r - pb * N.P[0];
-    int a = pmod(pa - N.h[0], N.D[0]), b = pmod(pb - N.h[1], N.D[1]), c = pmod(pc - N.h[2], N.D[2]);
+    int pc = idx / (N.P0 * N.P1);
+    int r = idx - pc * N.P0 * N.P1;
+    int pb = r / N.P0;
+    int pa = r - pb * N.P0;
+    int a = pmod(pa - N.h0, N.D0), b = pmod(pb - N.h1, N.D1), c = pmod(pc - N.h2, N.D2);
     int x, y, z;
     if (LAT == 1) {
       x = a; y = b; z = (a + b + 2 * c) % m;
@@ -224,14 +241,14 @@ __device__ void nat_stage(int8_t *lat, const int8_t *__restrict__ compact, const
 
 template <int LAT>
 __device__ void nat_unstage(const int8_t *lat, int8_t *__restrict__ compact, const Geo &g,
-                            const NatGeo &N, int shift) {
+                            const ND &N, int shift) {
   const int n = g.n[0], m = 2 * n;
-  const int total = N.D[0] * N.D[1] * N.D[2];
+  const int total = N.D0 * N.D1 * N.D2;
   for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-    int c = idx / (N.D[0] * N.D[1]);
-    int r = idx - c * N.D[0] * N.D[1];
-    int b = r / N.D[0];
-    int a = r - b * N.D[0];
+    int c = idx / (N.D0 * N.D1);
+    int r = idx - c * N.D0 * N.D1;
+    int b = r / N.D0;
+    int a = r - b * N.D0;
     int x, y, z;
     if (LAT == 1) {
       x = a; y = b; z = (a + b + 2 * c) % m;
@@ -243,9 +260,31 @@ __device__ void nat_unstage(const int8_t *lat, int8_t *__restrict__ compact, con
   }
 }
 
+// Packed shell counts with offsets fixed at compile time: every neighbour
+// load is one ds_read_u8 with an immediate offset from a per-site base.
+template <int LAT, int R, typename CW, int NC>
+__device__ __forceinline__ void count_c(const int8_t *lat, int addr, CW (&c)[R]) {
+  constexpr NatGeo K = nat_geo_c<LAT, R, NC>();
+  constexpr int Z = shell_begin<LAT>(R);
+  constexpr int MIN = [] {
+    int mn = 0;
+    for (int t = 0; t < Z; ++t) mn = K.off[t] < mn ? K.off[t] : mn;
+    return mn;
+  }();
+  const int8_t *base = lat + (addr + MIN);
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    CW acc = 0;
+#pragma unroll
+    for (int t = shell_begin<LAT>(n); t < shell_begin<LAT>(n + 1); ++t)
+      acc += (CW)1 << base[K.off[t] - MIN];
+    c[n] = acc;
+  }
+}
+
 // Packed shell counts with kernel-constant neighbour offsets.
-template <int LAT, int R, typename CW>
-__device__ __forceinline__ void count_nat(const int8_t *lat, int addr, const int (&off)[shell_begin<LAT>(R)],
+template <int LAT, int R, typename CW, int ZN>
+__device__ __forceinline__ void count_nat(const int8_t *lat, int addr, const int (&off)[ZN],
                                           CW (&c)[R]) {
 #pragma unroll
   for (int n = 0; n < R; ++n) {
@@ -257,7 +296,7 @@ __device__ __forceinline__ void count_nat(const int8_t *lat, int addr, const int
   }
 }
 
-// accept with probability min(1, exp(-beta*dE)) for dE >= 0: a float
+// accept with probability min(1, exp(x)), x = -beta*dE <= 0: a float
 // exponential decides unless u falls within 1e-5 (relative) of it, then FP64.
 __device__ __forceinline__ bool metropolis_accept(double u, double x) {
   const float ef = __expf((float)x);
@@ -267,41 +306,91 @@ __device__ __forceinline__ bool metropolis_accept(double u, double x) {
   return u < exp(x);
 }
 
-template <int LAT, int R, typename CW, int FIELD>
+// Parameters of one round, computed ahead (state-independent) into LDS.
+struct RoundP {
+  int32_t ax, mA, mB, mC;
+  int32_t oA1, oB1, oC1, oA2, oB2, oC2;
+  int32_t KB, KC, nL1, n2;
+  int32_t al, ab, ac, pad;
+};
+constexpr int kRoundBatch = 64;
+
+__device__ inline RoundP round_params(const ParArgs &C, unsigned long long phase) {
+  const Round R = make_round(C, phase);
+  RoundP P;
+  P.ax = R.ax;
+  P.mA = R.mA; P.mB = R.mB; P.mC = R.mC;
+  P.oA1 = R.oA1; P.oB1 = R.oB1; P.oC1 = R.oC1;
+  P.oA2 = R.oA2; P.oB2 = R.oB2; P.oC2 = R.oC2;
+  P.KB = R.KB; P.KC = R.KC; P.nL1 = R.nL1; P.n2 = R.n2;
+  P.al = (int32_t)R.al; P.ab = (int32_t)R.ab; P.ac = (int32_t)R.ac;
+  P.pad = 0;
+  return P;
+}
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Natural-layout address of the grid site (pa, pb, pc) along (round axis,
+// free axes); coordinates already reduced into the box.
+template <int LAT>
+__device__ __forceinline__ void site_of(int ax, int pa, int pb, int pc, int n,
+                                        int &a, int &b, int &c) {
+  int x, y, z;
+  place(ax, pa, pb, pc, x, y, z);
+  to_natural<LAT>(x, y, z, n, a, b, c);
+}
+
+// (128 + c_a - c_b) per byte lane of a packed count word
+__device__ __forceinline__ uint32_t biased_diff(uint32_t a, uint32_t b) { return a + 0x80808080u - b; }
+
+template <int LAT, int R, typename CW, int NC>
 __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ chains) {
   extern __shared__ __align__(16) uint8_t smem[];
+  constexpr int FIELD = 8;
   const ParArgs &C = chains[blockIdx.x];
   const Geo &g = C.g;
-  const NatGeo &N = C.nat;
+  constexpr NatGeo KG = nat_geo_c<LAT, R, NC ? NC : 8>();
+  const ND N = NC ? nd_of(KG) : nd_of(C.nat);
   constexpr int Z = shell_begin<LAT>(R);
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
   const int S = g.S;
-  const int Sp = (S + 1) & ~1;
+  const int S1 = S - 1 > 0 ? S - 1 : 1;
   int8_t *lat = (int8_t *)smem;
-  size_t o = (N.lds_bytes + 15) & ~15;
-  double *dtab = (double *)(smem + o);  // [alpha][beta][n][s]: V(beta,s,n)-V(alpha,s,n)
-  o += sizeof(double) * S * S * R * Sp;
+  size_t o = ((N.P0 * N.P1 * N.P2) + 15) & ~15;
+  // dtab[alpha][beta][n][s<S-1]: (V(beta,s,n)-V(alpha,s,n)) - (same for s=S-1);
+  // the S-1 independent count differences suffice since each shell is full
+  double *dtab = (double *)(smem + o);
+  o += sizeof(double) * S * S * R * S1;
   double *vrow = (double *)(smem + o);  // [x][n][s]: V(x,s,n)
-  o += sizeof(double) * S * R * Sp;
+  o += sizeof(double) * S * R * S;
+  RoundP *rps = (RoundP *)(smem + o);
+  o += sizeof(RoundP) * kRoundBatch;
+  int32_t *offl = (int32_t *)(smem + o);  // neighbour offsets, read into VGPRs
+  o += sizeof(int32_t) * ((Z + 3) & ~3);
   double *red = (double *)(smem + o);
   unsigned long long *redu = (unsigned long long *)(smem + o);
 
-  for (int i = tid; i < S * S * R * Sp; i += nthr) {
-    int s = i % Sp, n = (i / Sp) % R, be = (i / (Sp * R)) % S, al = i / (Sp * R * S);
-    dtab[i] = s < S ? C.vex[be + S * (s + S * n)] - C.vex[al + S * (s + S * n)] : 0.0;
+  for (int i = tid; i < Z; i += nthr) offl[i] = C.nat.off[i];
+  for (int i = tid; i < S * S * R * S1; i += nthr) {
+    int s = i % S1, n = (i / S1) % R, be = (i / (S1 * R)) % S, al = i / (S1 * R * S);
+    const double *V = C.vex;
+    double d_s = V[be + S * (s + S * n)] - V[al + S * (s + S * n)];
+    double d_l = V[be + S * ((S - 1) + S * n)] - V[al + S * ((S - 1) + S * n)];
+    dtab[i] = S > 1 ? d_s - d_l : 0.0;
   }
-  for (int i = tid; i < S * R * Sp; i += nthr) {
-    int s = i % Sp, n = (i / Sp) % R, x = i / (Sp * R);
-    vrow[i] = s < S ? C.vex[x + S * (s + S * n)] : 0.0;
+  for (int i = tid; i < S * R * S; i += nthr) {
+    int s = i % S, n = (i / S) % R, x = i / (S * R);
+    vrow[i] = C.vex[x + S * (s + S * n)];
   }
-  int off[Z];
-#pragma unroll
-  for (int t = 0; t < Z; ++t) off[t] = N.off[t];
   nat_stage<LAT>(lat, C.compact, g, N, FIELD);
   __syncthreads();
+  int off[NC ? 1 : Z];
+#pragma unroll
+  for (int t = 0; t < (NC ? 1 : Z); ++t) off[t] = offl[t];
 
   const int n_cells = g.n[0];
+  const int L = C.L, stride = C.stride;
   unsigned long long phase = C.phase0;
   long long acc_total = 0, trials_total = 0;
 
@@ -309,23 +398,25 @@ __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ c
     const double temp = C.T0 + (double)t * C.dT;
     const double sim_temp = temp * kKbInRyPar;
     const double beta = 1.0 / sim_temp;
-    // recount
+    // energy recount at the start of the temperature step
     double part = 0.0;
-    const int nsites = N.D[0] * N.D[1] * N.D[2];
+    const int nsites = N.D0 * N.D1 * N.D2;
     for (int i = tid; i < nsites; i += nthr) {
-      int c = i / (N.D[0] * N.D[1]);
-      int r = i - c * N.D[0] * N.D[1];
-      int b = r / N.D[0];
-      int a = r - b * N.D[0];
+      int c = i / (N.D0 * N.D1);
+      int r = i - c * N.D0 * N.D1;
+      int b = r / N.D0;
+      int a = r - b * N.D0;
       int ad = nat_addr(N, a, b, c);
       int sp = lat[ad] / FIELD;
       CW cnt[R];
-      count_nat<LAT, R, CW>(lat, ad, off, cnt);
+      if constexpr (NC > 0)
+        count_c<LAT, R, CW, NC>(lat, ad, cnt);
+      else
+        count_nat<LAT, R, CW>(lat, ad, off, cnt);
 #pragma unroll
       for (int n = 0; n < R; ++n) {
-        const double *vr = vrow + (sp * R + n) * Sp;
-        for (int s = 0; s < S; ++s)
-          part += (double)((cnt[n] >> (FIELD * s)) & ((1 << FIELD) - 1)) * vr[s];
+        const double *vr = vrow + (sp * R + n) * S;
+        for (int s = 0; s < S; ++s) part += (double)((cnt[n] >> (FIELD * s)) & 0xff) * vr[s];
       }
     }
     double E = 0.5 * block_sum(part, red);
@@ -338,65 +429,104 @@ __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ c
       double dE_sum = 0.0;
       long long done = 0;
       while (done < C.min_trials_per_sweep) {
-        const Round RG = make_round(C, phase);
-        uint32_t jl, rem, kb, kc, djl, drem, dkb, dkc;
-        divmod_small((uint32_t)tid, (uint32_t)(RG.KB * RG.KC), RG.invKBC, jl, rem);
-        divmod_small(rem, (uint32_t)RG.KB, RG.invKB, kc, kb);
-        divmod_small((uint32_t)nthr, (uint32_t)(RG.KB * RG.KC), RG.invKBC, djl, drem);
-        divmod_small(drem, (uint32_t)RG.KB, RG.invKB, dkc, dkb);
-        for (int q = tid; q < RG.n2; q += nthr) {
-          int x, y, z, aA, bA, cA, aB, bB, cB;
-          grid_xyz(C, RG, RG.oA2, RG.oB2, RG.oC2, (int)jl, (int)kb, (int)kc, x, y, z);
-          to_natural<LAT>(x, y, z, n_cells, aB, bB, cB);
-          const uint32_t jl1 = mod_small(RG.ml * jl + RG.al, (uint32_t)RG.nL1, RG.invL1);
-          const uint32_t kb1 = mod_small(RG.mb * kb + RG.ab, (uint32_t)RG.KB, RG.invKB);
-          const uint32_t kc1 = mod_small(RG.mc * kc + RG.ac, (uint32_t)RG.KC, RG.invKC);
-          grid_xyz(C, RG, RG.oA1, RG.oB1, RG.oC1, (int)jl1, (int)kb1, (int)kc1, x, y, z);
-          to_natural<LAT>(x, y, z, n_cells, aA, bA, cA);
-          kb += dkb;
-          uint32_t cy = kb >= (uint32_t)RG.KB;
-          kb -= cy * RG.KB;
-          kc += dkc + cy;
-          cy = kc >= (uint32_t)RG.KC;
-          kc -= cy * RG.KC;
-          jl += djl + cy;
+        // parameters of the next kRoundBatch rounds, one thread each
+        if (tid < kRoundBatch) rps[tid] = round_params(C, phase + (unsigned long long)tid);
+        __syncthreads();
+        int r = 0;
+        for (; r < kRoundBatch && done < C.min_trials_per_sweep; ++r) {
+          const RoundP &P = rps[r];
+          const int ax = uni(P.ax), mA = uni(P.mA), mB = uni(P.mB), mC = uni(P.mC);
+          const int oA1 = uni(P.oA1), oB1 = uni(P.oB1), oC1 = uni(P.oC1);
+          const int oA2 = uni(P.oA2), oB2 = uni(P.oB2), oC2 = uni(P.oC2);
+          const int KB = uni(P.KB), KC = uni(P.KC), nL1 = uni(P.nL1), n2 = uni(P.n2);
+          const int al = uni(P.al), ab = uni(P.ab), ac = uni(P.ac);
+          const unsigned long long ph = phase + (unsigned long long)r;
+          // grid-2 coordinates of this thread's first pair, and the stride
+          uint32_t jl, rem, kb, kc, djl, drem, dkb, dkc;
+          const float invKBC = 1.0f / (float)(KB * KC), invKB = 1.0f / (float)KB;
+          divmod_small((uint32_t)tid, (uint32_t)(KB * KC), invKBC, jl, rem);
+          divmod_small(rem, (uint32_t)KB, invKB, kc, kb);
+          divmod_small((uint32_t)nthr, (uint32_t)(KB * KC), invKBC, djl, drem);
+          divmod_small(drem, (uint32_t)KB, invKB, dkc, dkb);
+          U4 rnd = {0, 0, 0, 0};
+          int k = 0;
+          for (int q = tid; q < n2; q += nthr, ++k) {
+            if ((k & 3) == 0)
+              rnd = philox4x32(U4{(uint32_t)ph, (uint32_t)(ph >> 32),
+                                  (uint32_t)tid | ((uint32_t)(k >> 2) << 12), C.chain_id},
+                               C.seed_lo, C.seed_hi);
+            const uint32_t ru = (k & 3) == 0 ? rnd.x : (k & 3) == 1 ? rnd.y : (k & 3) == 2 ? rnd.z : rnd.w;
+            // grid-2 site
+            int pa = oA2 + (int)jl * L, pb = oB2 + (int)kb * stride, pc = oC2 + (int)kc * stride;
+            if (pa >= mA) pa -= mA;
+            if (pb >= mB) pb -= mB;
+            if (pc >= mC) pc -= mC;
+            int aB, bB, cB;
+            site_of<LAT>(ax, pa, pb, pc, n_cells, aB, bB, cB);
+            // its grid-1 partner (translation by (al, ab, ac) in grid indices)
+            int j1 = (int)jl + al, k1 = (int)kb + ab, c1 = (int)kc + ac;
+            if (j1 >= nL1) j1 -= nL1;
+            if (k1 >= KB) k1 -= KB;
+            if (c1 >= KC) c1 -= KC;
+            pa = oA1 + j1 * L;
+            pb = oB1 + k1 * stride;
+            pc = oC1 + c1 * stride;
+            if (pa >= mA) pa -= mA;
+            if (pb >= mB) pb -= mB;
+            if (pc >= mC) pc -= mC;
+            int aA, bA, cA;
+            site_of<LAT>(ax, pa, pb, pc, n_cells, aA, bA, cA);
+            // advance to this thread's next pair
+            kb += dkb;
+            uint32_t cy = kb >= (uint32_t)KB;
+            kb -= cy * KB;
+            kc += dkc + cy;
+            cy = kc >= (uint32_t)KC;
+            kc -= cy * KC;
+            jl += djl + cy;
 
-          const int addrA = nat_addr(N, aA, bA, cA), addrB = nat_addr(N, aB, bB, cB);
-          const int codeA = lat[addrA], codeB = lat[addrB];
-          if (codeA == codeB) {
-            if (sampled) acc_local++;
-            continue;
-          }
-          CW ca[R], cb[R];
-          count_nat<LAT, R, CW>(lat, addrA, off, ca);
-          count_nat<LAT, R, CW>(lat, addrB, off, cb);
-          const int alpha = codeA / FIELD, betasp = codeB / FIELD;
-          const double *dt = dtab + ((alpha * S + betasp) * R) * Sp;
-          double dE = 0.0;
+            const int addrA = nat_addr(N, aA, bA, cA), addrB = nat_addr(N, aB, bB, cB);
+            const int codeA = lat[addrA], codeB = lat[addrB];
+            if (codeA == codeB) {
+              if (sampled) acc_local++;
+              continue;
+            }
+            CW ca[R], cb[R];
+            if constexpr (NC > 0) {
+              count_c<LAT, R, CW, NC>(lat, addrA, ca);
+              count_c<LAT, R, CW, NC>(lat, addrB, cb);
+            } else {
+              count_nat<LAT, R, CW>(lat, addrA, off, ca);
+              count_nat<LAT, R, CW>(lat, addrB, off, cb);
+            }
+            const int alpha = codeA >> 3, betasp = codeB >> 3;
+            const double *dt = dtab + ((alpha * S + betasp) * R) * S1;
+            double dE = 0.0;
 #pragma unroll
-          for (int n = 0; n < R; ++n) {
-            for (int s = 0; s < S; ++s) {
-              int d = (int)((ca[n] >> (FIELD * s)) & ((1 << FIELD) - 1)) -
-                      (int)((cb[n] >> (FIELD * s)) & ((1 << FIELD) - 1));
-              dE += (double)d * dt[n * Sp + s];
+            for (int n = 0; n < R; ++n) {
+              const uint32_t d0 = biased_diff((uint32_t)ca[n], (uint32_t)cb[n]);
+              for (int s = 0; s < S - 1 && s < 4; ++s)
+                dE += (double)((float)((d0 >> (8 * s)) & 0xffu) - 128.0f) * dt[n * S1 + s];
+              if (sizeof(CW) > 4) {
+                const uint32_t d1 =
+                    biased_diff((uint32_t)((uint64_t)ca[n] >> 32), (uint32_t)((uint64_t)cb[n] >> 32));
+                for (int s = 4; s < S - 1; ++s)
+                  dE += (double)((float)((d1 >> (8 * (s - 4))) & 0xffu) - 128.0f) * dt[n * S1 + s];
+              }
+            }
+            bool acc = dE < 0.0;
+            if (!acc) acc = metropolis_accept((double)ru * (1.0 / 4294967296.0), -(beta * dE));
+            if (acc) {
+              nat_put(lat, N, aA, bA, cA, (int8_t)codeB);
+              nat_put(lat, N, aB, bB, cB, (int8_t)codeA);
+              dE_sum += dE;
+              if (sampled) acc_local++;
             }
           }
-          bool acc = dE < 0.0;
-          if (!acc) {
-            U4 r = philox4x32(U4{(uint32_t)phase, (uint32_t)(phase >> 32), (uint32_t)q, C.chain_id},
-                              C.seed_lo, C.seed_hi);
-            acc = metropolis_accept(u01_53(r.x, r.y), -(beta * dE));
-          }
-          if (acc) {
-            nat_put(lat, N, aA, bA, cA, (int8_t)codeB);
-            nat_put(lat, N, aB, bB, cB, (int8_t)codeA);
-            dE_sum += dE;
-            if (sampled) acc_local++;
-          }
+          done += n2;
+          __syncthreads();
         }
-        done += RG.n2;
-        ++phase;
-        __syncthreads();
+        phase += (unsigned long long)r;
       }
       trials_total += done;
       E += block_sum(dE_sum, red);
@@ -605,13 +735,26 @@ __global__ __launch_bounds__(512) void k_metro_cmp(const ParArgs *__restrict__ c
 template <int LAT, int R>
 const void *pick(bool nat, bool wide) {
   if (nat)
-    return wide ? (const void *)k_metro_nat<LAT, R, uint64_t, 8>
-                : (const void *)k_metro_nat<LAT, R, uint32_t, 6>;
+    return wide ? (const void *)k_metro_nat<LAT, R, uint64_t, 0>
+                : (const void *)k_metro_nat<LAT, R, uint32_t, 0>;
   return wide ? (const void *)k_metro_cmp<LAT, R, uint64_t, 8>
               : (const void *)k_metro_cmp<LAT, R, uint32_t, 6>;
 }
 
-const void *par_kernel(int lattice, int R, bool nat, bool wide) {
+// Shapes with compile-time neighbour offsets: the quaternary bcc anneal
+// (bcc, 4 shells, 32^3 cells) and the binary fcc run (fcc, 2 shells, 8^3).
+const void *specialised(int lattice, int R, int n, bool wide) {
+  if (wide) return nullptr;
+  if (lattice == 2 && R == 4 && n == 32) return (const void *)k_metro_nat<2, 4, uint32_t, 32>;
+  if (lattice == 1 && R == 2 && n == 8) return (const void *)k_metro_nat<1, 2, uint32_t, 8>;
+  return nullptr;
+}
+
+const void *par_kernel(int lattice, int R, int n, bool nat, bool wide) {
+  if (nat) {
+    const void *f = specialised(lattice, R, n, wide);
+    if (f) return f;
+  }
   if (lattice == 1) {
     switch (R) {
       case 1: return pick<1, 1>(nat, wide);
@@ -641,18 +784,19 @@ const void *par_kernel(int lattice, int R, bool nat, bool wide) {
 }  // namespace
 
 size_t par_lds_bytes(const Geo &g, const NatGeo *nat, int threads) {
-  const size_t Sp = (size_t)((g.S + 1) & ~1);
   const size_t red = sizeof(double) * (threads / 64 + 1);
+  const size_t S1 = g.S > 1 ? g.S - 1 : 1;
   if (nat)
-    return (size_t)((nat->lds_bytes + 15) & ~15) + sizeof(double) * g.S * g.S * g.R * Sp +
-           sizeof(double) * g.S * g.R * Sp + red;
+    return (size_t)((nat->lds_bytes + 15) & ~15) + sizeof(double) * g.S * g.S * g.R * S1 +
+           sizeof(double) * g.S * g.R * g.S + sizeof(RoundP) * kRoundBatch +
+           sizeof(int32_t) * ((g.z + 3) & ~3) + red;
   return (size_t)((g.lds_bytes + 15) & ~15) + sizeof(double) * g.S * g.S * g.R + red +
          sizeof(int32_t) * g.nsub * g.z;
 }
 
-hipError_t launch_metro_par(const ParArgs *args_dev, int n_chains, int lattice, int R, bool nat,
-                            size_t lds, int threads, bool wide, hipStream_t s) {
-  const void *fn = par_kernel(lattice, R, nat, wide);
+hipError_t launch_metro_par(const ParArgs *args_dev, int n_chains, int lattice, int R, int n,
+                            bool nat, size_t lds, int threads, bool wide, hipStream_t s) {
+  const void *fn = par_kernel(lattice, R, n, nat, wide);
   if (!fn) return hipErrorInvalidValue;
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
