@@ -57,6 +57,42 @@ def cpu_baseline(chain, budget_s: float):
                       f"trials with the reference's per-sweep energy recount, {dt:.1f} s"}
 
 
+def batched(steps: int, warmup: int):
+    """The batched form: 256 uneven Metropolis chains (fcc/bcc, 8^3..24^3
+    cells, 2-5 species with Dirichlet(1) counts, 3 shells, T in [200,2000] K,
+    500 sweeps each) in one batched call."""
+    from brawl_amd import Lattice, anneal_batch, workloads
+    chains = workloads.config3()
+    lats, params = [], []
+    for ch in chains:
+        L = Lattice(ch.lattice, ch.n, ch.v_ex)
+        L.seed(ch.seed & 0xFFFFFFFF)
+        L.initial_setup(species_numbers=ch.counts)
+        lats.append(L)
+        params.append(dict(T=ch.T, n_sweeps=ch.n_sweeps, seed=ch.seed))
+    starts = [L.get_config() for L in lats]
+    rates, mks, trials = [], [], 0
+    for k in range(warmup + steps):
+        for L, s0 in zip(lats, starts):
+            L.set_config(s0)
+        t0 = time.perf_counter()
+        res, mk = anneal_batch(lats, params)
+        dt = time.perf_counter() - t0
+        tr = sum(r.trials for r in res)
+        log(f"[bench] batched {'warmup' if k < warmup else 'step'}: {tr} trials, "
+            f"makespan {mk:.1f} ms, wall {dt * 1e3:.1f} ms")
+        if k >= warmup:
+            rates.append(tr / dt)
+            mks.append(mk)
+            trials = tr
+    for L in lats:
+        L.close()
+    return {"workload": "256 chains, L~U{8..24} fcc/bcc, S~U{2..5}, 3 shells, T~U[200,2000] K, "
+                        "500 sweeps", "chains": len(chains), "trials": trials,
+            "value": float(np.mean(rates)), "unit": "swap trials/s",
+            "device_rate": trials / (np.mean(mks) / 1e3), "makespan_ms": float(np.mean(mks))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +100,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -136,6 +173,9 @@ def main():
                                               float(last.acceptance_of_T[-1])]},
     }
     lat.close()
+    if not args.no_batched:
+        out["batched"] = batched(max(1, args.steps), min(1, args.warmup))
+        out["batched"]["vs_plain"] = out["batched"]["value"] / value
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(chain, args.cpu_budget)

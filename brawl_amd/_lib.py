@@ -26,6 +26,7 @@ EXPORTS = (
     "bwl_set_config", "bwl_get_config", "bwl_pair_histogram", "bwl_total_energy",
     "bwl_site_energy", "bwl_pair_energy", "bwl_genrand_seed", "bwl_genrand_get_state",
     "bwl_genrand_set_state", "bwl_initial_setup", "bwl_mc_steps", "bwl_metropolis_anneal",
+    "bwl_metropolis_anneal_batch",
 )
 
 
@@ -83,6 +84,8 @@ def load():
         "bwl_initial_setup": (C.c_int, [vp, i64p, dp]),
         "bwl_mc_steps": (C.c_int, [vp, C.c_double, C.c_int32, C.c_int64, C.c_int64, dp, i64p]),
         "bwl_metropolis_anneal": (C.c_int, [vp, P(AnnealParams), dp, dp, dp, P(RunStats)]),
+        "bwl_metropolis_anneal_batch": (C.c_int, [C.c_int32, P(vp), P(AnnealParams), P(dp), P(dp),
+                                                  P(dp), P(RunStats), dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -350,100 +351,225 @@ int bwl_mc_steps(bwl_ctx *c, double beta, int32_t nbr_swap, int64_t n_trials, in
 
 int bwl_metropolis_anneal(bwl_ctx *c, const bwl_anneal_params *p, double *energies_of_T,
                           double *C_of_T, double *acceptance_of_T, bwl_run_stats *stats) {
-  int rc = check_ctx(c, true);
-  if (rc) return rc;
-  if (!p || !energies_of_T || !C_of_T || !acceptance_of_T)
+  bwl_ctx *cs[1] = {c};
+  double *e[1] = {energies_of_T}, *cv[1] = {C_of_T}, *ac[1] = {acceptance_of_T};
+  return bwl_metropolis_anneal_batch(1, cs, p, e, cv, ac, stats, nullptr);
+}
+
+int bwl_metropolis_anneal_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_anneal_params *params,
+                                double *const *energies_of_T, double *const *C_of_T,
+                                double *const *acceptance_of_T, bwl_run_stats *stats,
+                                double *makespan_ms) {
+  if (n < 1 || !ctxs || !params || !energies_of_T || !C_of_T || !acceptance_of_T)
     return fail(BWL_ERR_ARG, "null argument");
-  if (p->T_steps < 1 || p->n_sweeps < 1 || p->burn_in_sweeps < 0)
-    return fail(BWL_ERR_ARG, "T_steps and n_sweeps must be positive");
-  const Geo &g = c->g;
-  ParPlan plan;
-  rc = plan_parallel(g, plan, g_err);
-  if (rc) return rc;
-  int max_pairs = 0;
-  for (const RoundVar &v : plan.vars) max_pairs = v.n2 > max_pairs ? v.n2 : max_pairs;
-  // fewest passes over the pairs, then the fewest threads for that
-  const int passes = (max_pairs + 511) / 512;
-  int threads = (((max_pairs + passes - 1) / passes + 63) / 64) * 64;
-  if (threads > 512) threads = 512;
-  if (threads < 64) threads = 64;
-  NatGeo nat;
-  bool use_nat = make_nat(g, nat);
-  size_t lds = par_lds_bytes(g, use_nat ? &nat : nullptr, threads);
-  if (use_nat && lds > (size_t)lds_limit()) {
-    use_nat = false;
-    lds = par_lds_bytes(g, nullptr, threads);
+  struct Item {
+    int idx;
+    const void *fn;
+    int threads;
+    size_t lds;
+    ParArgs a;
+    ParPlan plan;
+    size_t off_muls, off_out, off_cnt;
+    double work;
+  };
+  std::vector<Item> items((size_t)n);
+  size_t total = 0;
+  const int lim = lds_limit();
+  for (int i = 0; i < n; ++i) {
+    bwl_ctx *c = ctxs[i];
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if (i > 0 && c->device != ctxs[0]->device)
+      return fail(BWL_ERR_ARG, "all chains of a batch must be on one device");
+    const bwl_anneal_params *p = &params[i];
+    if (!energies_of_T[i] || !C_of_T[i] || !acceptance_of_T[i]) return fail(BWL_ERR_ARG, "null output");
+    if (p->T_steps < 1 || p->n_sweeps < 1 || p->burn_in_sweeps < 0)
+      return fail(BWL_ERR_ARG, "T_steps and n_sweeps must be positive");
+    Item &it = items[(size_t)i];
+    it.idx = i;
+    const Geo &g = c->g;
+    rc = plan_parallel(g, it.plan, g_err);
+    if (rc) return rc;
+    int max_pairs = 0;
+    for (const RoundVar &v : it.plan.vars) max_pairs = v.n2 > max_pairs ? v.n2 : max_pairs;
+    // fewest passes over a round's pairs, then the fewest threads for that
+    const int passes = (max_pairs + 511) / 512;
+    int threads = (((max_pairs + passes - 1) / passes + 63) / 64) * 64;
+    if (threads > 512) threads = 512;
+    if (threads < 64) threads = 64;
+    NatGeo nat;
+    bool use_nat = make_nat(g, nat);
+    size_t lds = par_lds_bytes(g, use_nat ? &nat : nullptr, threads);
+    if (use_nat && lds > (size_t)lim) {
+      use_nat = false;
+      lds = par_lds_bytes(g, nullptr, threads);
+    }
+    if (lds > (size_t)lim) return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
+    // packed counts: the natural kernel uses 8-bit fields (u32 holds 4
+    // species), the per-sublattice kernel 6-bit fields (u32 holds 5)
+    const bool wide = use_nat ? g.S > 4 : g.S > 5;
+    it.fn = par_kernel_for(g.lattice, g.R, g.n[0], use_nat, wide);
+    if (!it.fn) return fail(BWL_ERR_UNSUPPORTED, "no kernel for this lattice");
+    it.threads = threads;
+    it.lds = lds;
+    ParArgs &a = it.a;
+    memset(&a, 0, sizeof(a));
+    a.g = g;
+    a.nat = nat;
+    a.compact = c->d_compact;
+    a.off = c->d_off;
+    a.vex = c->d_vex;
+    a.stride = it.plan.stride;
+    a.L = it.plan.L;
+    for (int k = 0; k < 3; ++k) {
+      a.K[k] = it.plan.K[k];
+      a.n_units[k] = it.plan.n_units[k];
+      for (int u = 0; u < 16; ++u) a.units[k][u] = it.plan.units[k][u];
+    }
+    a.n_var = (int32_t)it.plan.vars.size();
+    for (size_t k = 0; k < it.plan.vars.size(); ++k) a.var[k] = it.plan.vars[k];
+    a.fast_halo = it.plan.fast_halo ? 1 : 0;
+    a.min_trials_per_sweep = g.n_atoms;
+    a.T0 = p->T;
+    a.dT = p->delta_T;
+    a.T_steps = p->T_steps;
+    a.n_sweeps = p->n_sweeps;
+    a.burn_sweeps = p->burn_in_sweeps;
+    a.seed_lo = (uint32_t)p->seed;
+    a.seed_hi = (uint32_t)(p->seed >> 32);
+    a.chain_id = 0;  // the stream depends on the seed and position only, not on the batch
+    a.phase0 = c->phase;
+    a.e_trace = nullptr;
+    const size_t T = (size_t)p->T_steps;
+    it.off_muls = total;
+    total += 256 * ((it.plan.muls.size() * 4 + 255) / 256);
+    it.off_out = total;
+    total += 256 * ((3 * T * sizeof(double) + 255) / 256);
+    it.off_cnt = total;
+    total += 256;
+    it.work = (double)g.n_atoms * (double)p->T_steps * (double)(p->n_sweeps + p->burn_in_sweeps);
   }
-  if (lds > (size_t)lds_limit()) return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
-  // packed counts: natural kernel uses 8-bit fields (u32 holds 4 species),
-  // the per-sublattice kernel 6-bit fields (u32 holds 5)
-  const bool wide = use_nat ? g.S > 4 : g.S > 5;
-  const size_t T = (size_t)p->T_steps;
-  size_t off_args = 0;
-  size_t off_muls = 256 * ((sizeof(ParArgs) + 255) / 256);
-  size_t off_out = off_muls + 256 * ((plan.muls.size() * 4 + 255) / 256);
-  size_t off_cnt = off_out + 3 * T * sizeof(double);
-  size_t need = off_cnt + 64;
-  rc = ensure_scratch(c, need);
-  if (rc) return rc;
-  char *base = (char *)c->d_scratch;
-  ParArgs a;
-  memset(&a, 0, sizeof(a));
-  a.g = g;
-  a.nat = nat;
-  a.compact = c->d_compact;
-  a.off = c->d_off;
-  a.vex = c->d_vex;
-  a.stride = plan.stride;
-  a.L = plan.L;
-  for (int k = 0; k < 3; ++k) a.K[k] = plan.K[k];
-  a.n_var = (int32_t)plan.vars.size();
-  for (size_t k = 0; k < plan.vars.size(); ++k) a.var[k] = plan.vars[k];
-  a.muls = (const uint32_t *)(base + off_muls);
-  for (int k = 0; k < 3; ++k) {
-    a.n_units[k] = plan.n_units[k];
-    for (int u = 0; u < 16; ++u) a.units[k][u] = plan.units[k][u];
+  // group chains that run the same kernel with the same block size;
+  // largest work first inside a group
+  std::vector<int> order((size_t)n);
+  for (int i = 0; i < n; ++i) order[(size_t)i] = i;
+  std::sort(order.begin(), order.end(), [&](int x, int y) {
+    const Item &A = items[(size_t)x], &B = items[(size_t)y];
+    if (A.fn != B.fn) return A.fn < B.fn;
+    if (A.threads != B.threads) return A.threads < B.threads;
+    return A.work > B.work;
+  });
+  struct Group {
+    const void *fn;
+    int threads;
+    size_t lds;
+    std::vector<int> members;
+    size_t off_args;
+  };
+  std::vector<Group> groups;
+  for (int i : order) {
+    const Item &it = items[(size_t)i];
+    if (groups.empty() || groups.back().fn != it.fn || groups.back().threads != it.threads)
+      groups.push_back(Group{it.fn, it.threads, 0, {}, 0});
+    groups.back().members.push_back(i);
+    if (it.lds > groups.back().lds) groups.back().lds = it.lds;
   }
-  a.fast_halo = plan.fast_halo ? 1 : 0;
-  a.min_trials_per_sweep = g.n_atoms;
-  a.T0 = p->T;
-  a.dT = p->delta_T;
-  a.T_steps = p->T_steps;
-  a.n_sweeps = p->n_sweeps;
-  a.burn_sweeps = p->burn_in_sweeps;
-  a.seed_lo = (uint32_t)p->seed;
-  a.seed_hi = (uint32_t)(p->seed >> 32);
-  a.chain_id = 0;
-  a.phase0 = 0;
-  a.energies_of_T = (double *)(base + off_out);
-  a.C_of_T = a.energies_of_T + T;
-  a.acc_of_T = a.C_of_T + T;
-  a.e_trace = nullptr;
-  a.counters = (long long *)(base + off_cnt);
-  long long cnt0[3] = {0, 0, 0};
-  HIP_OK(hipMemcpyAsync(base + off_args, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
-  HIP_OK(hipMemcpyAsync(base + off_muls, plan.muls.data(), plan.muls.size() * 4,
-                        hipMemcpyHostToDevice, c->stream));
-  HIP_OK(hipMemcpyAsync(a.counters, cnt0, sizeof(cnt0), hipMemcpyHostToDevice, c->stream));
-  HIP_OK(hipEventRecord(c->ev0, c->stream));
-  HIP_OK(launch_metro_par((const ParArgs *)(base + off_args), 1, g.lattice, g.R, g.n[0], use_nat,
-                          lds, threads, wide, c->stream));
-  HIP_OK(hipEventRecord(c->ev1, c->stream));
-  std::vector<double> out(3 * T);
-  HIP_OK(hipMemcpyAsync(out.data(), a.energies_of_T, 3 * T * sizeof(double), hipMemcpyDeviceToHost,
-                        c->stream));
-  long long cnt[3];
-  HIP_OK(hipMemcpyAsync(cnt, a.counters, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
-  HIP_OK(hipStreamSynchronize(c->stream));
-  memcpy(energies_of_T, out.data(), T * sizeof(double));
-  memcpy(C_of_T, out.data() + T, T * sizeof(double));
-  memcpy(acceptance_of_T, out.data() + 2 * T, T * sizeof(double));
-  if (stats) {
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    stats->trials = cnt[2];
-    stats->accepted = cnt[1];
-    stats->kernel_ms = ms;
+  for (Group &gr : groups) {
+    gr.off_args = total;
+    total += 256 * ((sizeof(ParArgs) * gr.members.size() + 255) / 256);
+  }
+  bwl_ctx *c0 = ctxs[0];
+  void *scratch = nullptr;
+  if (hipMalloc(&scratch, total) != hipSuccess) return fail(BWL_ERR_NOMEM, "device allocation failed");
+  char *base = (char *)scratch;
+  struct Freer {
+    void *p;
+    ~Freer() { (void)hipFree(p); }
+  } freer{scratch};
+  std::vector<ParArgs> host_args;
+  for (int i = 0; i < n; ++i) {
+    Item &it = items[(size_t)i];
+    const size_t T = (size_t)params[i].T_steps;
+    it.a.muls = (const uint32_t *)(base + it.off_muls);
+    it.a.energies_of_T = (double *)(base + it.off_out);
+    it.a.C_of_T = it.a.energies_of_T + T;
+    it.a.acc_of_T = it.a.C_of_T + T;
+    it.a.counters = (long long *)(base + it.off_cnt);
+    HIP_OK(hipMemcpyAsync(base + it.off_muls, it.plan.muls.data(), it.plan.muls.size() * 4,
+                          hipMemcpyHostToDevice, c0->stream));
+  }
+  for (Group &gr : groups) {
+    host_args.clear();
+    for (int i : gr.members) host_args.push_back(items[(size_t)i].a);
+    HIP_OK(hipMemcpyAsync(base + gr.off_args, host_args.data(), sizeof(ParArgs) * host_args.size(),
+                          hipMemcpyHostToDevice, c0->stream));
+  }
+  for (int i = 0; i < n; ++i) {
+    long long zero[3] = {0, 0, 0};
+    HIP_OK(hipMemcpyAsync(items[(size_t)i].a.counters, zero, sizeof(zero), hipMemcpyHostToDevice,
+                          c0->stream));
+  }
+  // launch every group on its own stream so groups overlap on the device
+  std::vector<hipStream_t> streams;
+  for (size_t k = 0; k < groups.size(); ++k) {
+    hipStream_t s = nullptr;
+    if (k == 0) {
+      s = c0->stream;
+    } else if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+      for (hipStream_t t : streams) (void)hipStreamDestroy(t);
+      return fail(BWL_ERR_DEVICE, "stream creation failed");
+    }
+    if (k > 0) streams.push_back(s);
+  }
+  struct StreamFreer {
+    std::vector<hipStream_t> *v;
+    ~StreamFreer() {
+      for (hipStream_t t : *v) (void)hipStreamDestroy(t);
+    }
+  } sfree{&streams};
+  hipEvent_t ready = nullptr;
+  HIP_OK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+  struct EventFreer {
+    hipEvent_t e;
+    ~EventFreer() { (void)hipEventDestroy(e); }
+  } efree{ready};
+  HIP_OK(hipEventRecord(c0->ev0, c0->stream));
+  HIP_OK(hipEventRecord(ready, c0->stream));
+  for (size_t k = 0; k < groups.size(); ++k) {
+    hipStream_t s = k == 0 ? c0->stream : streams[k - 1];
+    if (k > 0) HIP_OK(hipStreamWaitEvent(s, ready, 0));
+    const Group &gr = groups[k];
+    HIP_OK(launch_metro_par_fn(gr.fn, (const ParArgs *)(base + gr.off_args), (int)gr.members.size(),
+                               gr.lds, gr.threads, s));
+  }
+  for (size_t k = 1; k < groups.size(); ++k) {
+    hipEvent_t done = nullptr;
+    HIP_OK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    HIP_OK(hipEventRecord(done, streams[k - 1]));
+    HIP_OK(hipStreamWaitEvent(c0->stream, done, 0));
+    (void)hipEventDestroy(done);
+  }
+  HIP_OK(hipEventRecord(c0->ev1, c0->stream));
+  HIP_OK(hipStreamSynchronize(c0->stream));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, c0->ev0, c0->ev1);
+  if (makespan_ms) *makespan_ms = ms;
+  for (int i = 0; i < n; ++i) {
+    const Item &it = items[(size_t)i];
+    const size_t T = (size_t)params[i].T_steps;
+    std::vector<double> out(3 * T);
+    long long cnt[3];
+    HIP_OK(hipMemcpy(out.data(), it.a.energies_of_T, 3 * T * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(cnt, it.a.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    memcpy(energies_of_T[i], out.data(), T * sizeof(double));
+    memcpy(C_of_T[i], out.data() + T, T * sizeof(double));
+    memcpy(acceptance_of_T[i], out.data() + 2 * T, T * sizeof(double));
+    if (stats) {
+      stats[i].trials = cnt[2];
+      stats[i].accepted = cnt[1];
+      stats[i].kernel_ms = ms;
+    }
+    ctxs[i]->phase = (unsigned long long)cnt[0];
   }
   return BWL_OK;
 }

@@ -78,6 +78,9 @@ hipError_t launch_site_energy(const SiteArgs &a, hipStream_t s);
 size_t seq_lds_bytes(const Geo &g);
 hipError_t launch_metro_seq(const SeqArgs &a, hipStream_t s);
 size_t par_lds_bytes(const Geo &g, const NatGeo *nat, int threads);
+const void *par_kernel_for(int lattice, int R, int n, bool nat, bool wide);
+hipError_t launch_metro_par_fn(const void *fn, const ParArgs *args_dev, int n_chains, size_t lds,
+                               int threads, hipStream_t s);
 hipError_t launch_metro_par(const ParArgs *args_dev, int n_chains, int lattice, int R, int n,
                             bool nat, size_t lds, int threads, bool wide, hipStream_t s);
 

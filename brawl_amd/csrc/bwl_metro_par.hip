@@ -794,6 +794,19 @@ size_t par_lds_bytes(const Geo &g, const NatGeo *nat, int threads) {
          sizeof(int32_t) * g.nsub * g.z;
 }
 
+const void *par_kernel_for(int lattice, int R, int n, bool nat, bool wide) {
+  return par_kernel(lattice, R, n, nat, wide);
+}
+
+hipError_t launch_metro_par_fn(const void *fn, const ParArgs *args_dev, int n_chains, size_t lds,
+                               int threads, hipStream_t s) {
+  if (!fn) return hipErrorInvalidValue;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  void *args[] = {(void *)&args_dev};
+  return hipLaunchKernel(fn, dim3(n_chains), dim3(threads), args, lds, s);
+}
+
 hipError_t launch_metro_par(const ParArgs *args_dev, int n_chains, int lattice, int R, int n,
                             bool nat, size_t lds, int threads, bool wide, hipStream_t s) {
   const void *fn = par_kernel(lattice, R, n, nat, wide);

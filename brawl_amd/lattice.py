@@ -181,5 +181,35 @@ class Lattice:
         return AnnealResult(e, c, a, int(st.trials), int(st.accepted), float(st.kernel_ms))
 
 
-__all__ = ["Lattice", "BrawlError", "AnnealResult", "beta_of", "FCC", "BCC", "K_B_IN_RY",
-           "RY_TO_EV"]
+def anneal_batch(lattices, params):
+    """Batched form of metropolis_simulated_annealing: one workgroup per
+    chain, all chains in as few launches as possible.  `params` is a list of
+    dicts with the keyword arguments of metropolis_simulated_annealing.
+    Returns (list of AnnealResult, makespan_ms)."""
+    n = len(lattices)
+    if n != len(params) or n == 0:
+        raise ValueError("one parameter set per lattice")
+    L = _lib.load()
+    P = (AnnealParams * n)()
+    outs = []
+    for i, p in enumerate(params):
+        T_steps = int(p.get("T_steps", 1))
+        P[i] = AnnealParams(float(p["T"]), float(p.get("delta_T", 0.0)), T_steps,
+                            int(p["n_sweeps"]), int(p.get("burn_in_sweeps", 0)), 0,
+                            int(p.get("seed", 20261202)) & (2**64 - 1))
+        outs.append((np.zeros(T_steps), np.zeros(T_steps), np.zeros(T_steps)))
+    H = (C.c_void_p * n)(*[lat._h for lat in lattices])
+    DP = C.POINTER(C.c_double)
+    E = (DP * n)(*[_p(o[0], C.c_double) for o in outs])
+    Cv = (DP * n)(*[_p(o[1], C.c_double) for o in outs])
+    A = (DP * n)(*[_p(o[2], C.c_double) for o in outs])
+    st = (RunStats * n)()
+    mk = C.c_double()
+    check(L.bwl_metropolis_anneal_batch(n, H, P, E, Cv, A, st, C.byref(mk)))
+    res = [AnnealResult(o[0], o[1], o[2], int(s.trials), int(s.accepted), float(s.kernel_ms))
+           for o, s in zip(outs, st)]
+    return res, float(mk.value)
+
+
+__all__ = ["Lattice", "BrawlError", "AnnealResult", "anneal_batch", "beta_of", "FCC", "BCC",
+           "K_B_IN_RY", "RY_TO_EV"]

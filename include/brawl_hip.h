@@ -121,9 +121,12 @@ int bwl_mc_steps(bwl_ctx *ctx, double beta, int32_t nbr_swap, int64_t n_trials,
  * (metropolis.F90:193-437) for lattice-wide swaps.  Each sweep makes at
  * least n_atoms swap trials as rounds of simultaneous, non-interacting
  * pair trials (sites of a round are further apart than the interaction
- * range); proposals and acceptances use Philox4x32-10 keyed by `seed`.
- * The energy is recounted after every sweep; outputs per T step follow
- * metropolis.F90:414-437 (<E>/N, C, acceptance rate). */
+ * range); proposals and acceptances use Philox4x32-10 keyed by `seed`,
+ * continuing the context's stream position across calls.  The energy is
+ * recounted from the lattice at the start of each T step and carried
+ * through the sweeps as the sum of the accepted dE, sampled after every
+ * sweep; outputs per T step follow metropolis.F90:414-437 (<E>/N, C,
+ * acceptance rate with equal-species trials counted as accepted). */
 typedef struct {
   double T;            /* start temperature (K) */
   double delta_T;      /* step (K) */
@@ -143,6 +146,23 @@ typedef struct {
 int bwl_metropolis_anneal(bwl_ctx *ctx, const bwl_anneal_params *p,
                           double *energies_of_T, double *C_of_T,
                           double *acceptance_of_T, bwl_run_stats *stats);
+
+/* Batched form: n independent chains (contexts on one device, uneven
+ * lattice sizes, compositions and schedules) annealed in as few launches
+ * as possible -- one workgroup per chain, chains with the same kernel
+ * variant share a launch, variants run concurrently.  Per-chain outputs
+ * and stats are as for bwl_metropolis_anneal; *makespan_ms (optional) is
+ * the device time of the whole batch.  Each chain's random stream depends
+ * only on its seed and its own position in that stream, so a chain gives
+ * the same result alone and in any batch.  This is the MPI replica
+ * ensemble of the reference (comms.F90:122-160, one chain per rank) held
+ * on one GPU. */
+int bwl_metropolis_anneal_batch(int32_t n, bwl_ctx *const *ctxs,
+                                const bwl_anneal_params *params,
+                                double *const *energies_of_T,
+                                double *const *C_of_T,
+                                double *const *acceptance_of_T,
+                                bwl_run_stats *stats, double *makespan_ms);
 
 #ifdef __cplusplus
 }
