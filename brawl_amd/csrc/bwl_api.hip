@@ -68,6 +68,8 @@ int check_ctx(bwl_ctx *c, bool need_config) {
   return BWL_OK;
 }
 
+int c0dev(bwl_ctx *const *ctxs) { return ctxs[0]->device; }
+
 int lds_limit() {
   int dev = 0, v = 0;
   if (hipGetDevice(&dev) != hipSuccess) return kLdsBytes;
@@ -430,6 +432,7 @@ int bwl_metropolis_anneal_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_annea
     for (size_t k = 0; k < it.plan.vars.size(); ++k) a.var[k] = it.plan.vars[k];
     a.fast_halo = it.plan.fast_halo ? 1 : 0;
     a.min_trials_per_sweep = g.n_atoms;
+    a.threads = threads;
     a.T0 = p->T;
     a.dT = p->delta_T;
     a.T_steps = p->T_steps;
@@ -449,14 +452,15 @@ int bwl_metropolis_anneal_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_annea
     total += 256;
     it.work = (double)g.n_atoms * (double)p->T_steps * (double)(p->n_sweeps + p->burn_in_sweeps);
   }
-  // group chains that run the same kernel with the same block size;
-  // largest work first inside a group
+  // group chains that run the same kernel (one launch each, the block size
+  // of its largest member; each chain loops over its own pairs with the
+  // threads it has), largest work first inside a group.  Few groups keep
+  // every launch on its own hardware queue, so all groups run at once.
   std::vector<int> order((size_t)n);
   for (int i = 0; i < n; ++i) order[(size_t)i] = i;
   std::sort(order.begin(), order.end(), [&](int x, int y) {
     const Item &A = items[(size_t)x], &B = items[(size_t)y];
     if (A.fn != B.fn) return A.fn < B.fn;
-    if (A.threads != B.threads) return A.threads < B.threads;
     return A.work > B.work;
   });
   struct Group {
@@ -469,10 +473,29 @@ int bwl_metropolis_anneal_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_annea
   std::vector<Group> groups;
   for (int i : order) {
     const Item &it = items[(size_t)i];
-    if (groups.empty() || groups.back().fn != it.fn || groups.back().threads != it.threads)
+    if (groups.empty() || groups.back().fn != it.fn)
       groups.push_back(Group{it.fn, it.threads, 0, {}, 0});
-    groups.back().members.push_back(i);
-    if (it.lds > groups.back().lds) groups.back().lds = it.lds;
+    Group &gr = groups.back();
+    gr.members.push_back(i);
+    if (it.threads > gr.threads) gr.threads = it.threads;
+  }
+  // LDS per launch: the largest member's need at the group's block size
+  for (Group &gr : groups)
+    for (int i : gr.members) {
+      const Item &it = items[(size_t)i];
+      const size_t need = it.lds + sizeof(double) * (size_t)(gr.threads - it.threads) / 64;
+      if (need > gr.lds) gr.lds = need;
+    }
+  // With no more chains than compute units, give every chain a CU of its
+  // own: ask each block for more than half the LDS so no two share a CU
+  // (the largest chains set the makespan; sharing a CU would halve them).
+  int n_cu = 0;
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c0dev(ctxs)) != hipSuccess)
+    n_cu = 0;
+  if (n_cu > 0 && n <= n_cu) {
+    const size_t exclusive = (size_t)lim / 2 + 1024;
+    for (Group &gr : groups)
+      if (gr.lds < exclusive && exclusive <= (size_t)lim) gr.lds = exclusive;
   }
   for (Group &gr : groups) {
     gr.off_args = total;

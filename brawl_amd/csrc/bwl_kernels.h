@@ -57,6 +57,7 @@ struct ParArgs {
   int32_t n_units[3];
   int32_t fast_halo;     // n >= 2h on every axis
   long long min_trials_per_sweep;  // trials per sweep (>= n_atoms)
+  int32_t threads;       // threads that do the chain's work (<= block size)
   // schedule
   double T0, dT;
   int32_t T_steps, n_sweeps, burn_sweeps;

@@ -271,7 +271,11 @@ __device__ __forceinline__ void count_c(const int8_t *lat, int addr, CW (&c)[R])
     for (int t = 0; t < Z; ++t) mn = K.off[t] < mn ? K.off[t] : mn;
     return mn;
   }();
-  const int8_t *base = lat + (addr + MIN);
+  // hide base = addr + MIN from the optimiser so it cannot re-split the
+  // constant; every offset below is then a non-negative immediate
+  int b = addr + MIN;
+  asm volatile("" : "+v"(b));
+  const int8_t *base = lat + b;
 #pragma unroll
   for (int n = 0; n < R; ++n) {
     CW acc = 0;
@@ -353,7 +357,11 @@ __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ c
   const ND N = NC ? nd_of(KG) : nd_of(C.nat);
   constexpr int Z = shell_begin<LAT>(R);
   const int tid = threadIdx.x;
-  const int nthr = blockDim.x;
+  // the chain's own thread count; a batched launch may give the block more
+  // threads, which then only join barriers (results do not depend on the
+  // block size a chain is launched with)
+  const int nthr = C.threads;
+  const bool active = tid < nthr;
   const int S = g.S;
   const int S1 = S - 1 > 0 ? S - 1 : 1;
   int8_t *lat = (int8_t *)smem;
@@ -371,15 +379,15 @@ __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ c
   double *red = (double *)(smem + o);
   unsigned long long *redu = (unsigned long long *)(smem + o);
 
-  for (int i = tid; i < Z; i += nthr) offl[i] = C.nat.off[i];
-  for (int i = tid; i < S * S * R * S1; i += nthr) {
+  for (int i = tid; i < Z; i += blockDim.x) offl[i] = C.nat.off[i];
+  for (int i = tid; i < S * S * R * S1; i += blockDim.x) {
     int s = i % S1, n = (i / S1) % R, be = (i / (S1 * R)) % S, al = i / (S1 * R * S);
     const double *V = C.vex;
     double d_s = V[be + S * (s + S * n)] - V[al + S * (s + S * n)];
     double d_l = V[be + S * ((S - 1) + S * n)] - V[al + S * ((S - 1) + S * n)];
     dtab[i] = S > 1 ? d_s - d_l : 0.0;
   }
-  for (int i = tid; i < S * R * S; i += nthr) {
+  for (int i = tid; i < S * R * S; i += blockDim.x) {
     int s = i % S, n = (i / S) % R, x = i / (S * R);
     vrow[i] = C.vex[x + S * (s + S * n)];
   }
@@ -401,7 +409,7 @@ __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ c
     // energy recount at the start of the temperature step
     double part = 0.0;
     const int nsites = N.D0 * N.D1 * N.D2;
-    for (int i = tid; i < nsites; i += nthr) {
+    for (int i = active ? tid : nsites; i < nsites; i += nthr) {
       int c = i / (N.D0 * N.D1);
       int r = i - c * N.D0 * N.D1;
       int b = r / N.D0;
@@ -450,7 +458,7 @@ __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ c
           divmod_small(drem, (uint32_t)KB, invKB, dkc, dkb);
           U4 rnd = {0, 0, 0, 0};
           int k = 0;
-          for (int q = tid; q < n2; q += nthr, ++k) {
+          for (int q = active ? tid : n2; q < n2; q += nthr, ++k) {
             if ((k & 3) == 0)
               rnd = philox4x32(U4{(uint32_t)ph, (uint32_t)(ph >> 32),
                                   (uint32_t)tid | ((uint32_t)(k >> 2) << 12), C.chain_id},
@@ -581,7 +589,8 @@ __global__ __launch_bounds__(512) void k_metro_cmp(const ParArgs *__restrict__ c
   const Geo &g = C.g;
   constexpr int Z = shell_begin<LAT>(R);
   const int tid = threadIdx.x;
-  const int nthr = blockDim.x;
+  const int nthr = C.threads;  // the chain's own thread count (see k_metro_nat)
+  const bool active = tid < nthr;
   const int S = g.S;
   int8_t *lat = (int8_t *)smem;
   size_t o = (g.lds_bytes + 15) & ~15;
@@ -589,13 +598,13 @@ __global__ __launch_bounds__(512) void k_metro_cmp(const ParArgs *__restrict__ c
   o += sizeof(double) * S * S * R;
   double *red = (double *)(smem + o);
   unsigned long long *redu = (unsigned long long *)(smem + o);
-  int32_t *offs = (int32_t *)(red + (nthr / 64 + 1));  // [nsub][Z]
+  int32_t *offs = (int32_t *)(red + (blockDim.x / 64 + 1));  // [nsub][Z]
 
-  for (int i = tid; i < S * S * R; i += nthr) {
+  for (int i = tid; i < S * S * R; i += blockDim.x) {
     int n = i / (S * S), x = (i / S) % S, s = i % S;
     vt[i] = C.vex[x + S * (s + S * n)];
   }
-  for (int i = tid; i < g.nsub * Z; i += nthr) offs[i] = C.off[i];
+  for (int i = tid; i < g.nsub * Z; i += blockDim.x) offs[i] = C.off[i];
   stage_lattice(lat, C.compact, g, FIELD);
   __syncthreads();
 
@@ -608,7 +617,7 @@ __global__ __launch_bounds__(512) void k_metro_cmp(const ParArgs *__restrict__ c
     const double sim_temp = temp * kKbInRyPar;
     const double beta = 1.0 / sim_temp;
     double part = 0.0;
-    for (int i = tid; i < g.nsub * cells; i += nthr) {
+    for (int i = active ? tid : g.nsub * cells; i < g.nsub * cells; i += nthr) {
       int s = i / cells;
       int r = i - s * cells;
       int cz = r / (g.n[0] * g.n[1]);
@@ -644,7 +653,7 @@ __global__ __launch_bounds__(512) void k_metro_cmp(const ParArgs *__restrict__ c
         divmod_small(rem, (uint32_t)RG.KB, RG.invKB, kc, kb);
         divmod_small((uint32_t)nthr, (uint32_t)(RG.KB * RG.KC), RG.invKBC, djl, drem);
         divmod_small(drem, (uint32_t)RG.KB, RG.invKB, dkc, dkb);
-        for (int q = tid; q < RG.n2; q += nthr) {
+        for (int q = active ? tid : RG.n2; q < RG.n2; q += nthr) {
           int x, y, z, cellA[3], cellB[3];
           grid_xyz(C, RG, RG.oA2, RG.oB2, RG.oC2, (int)jl, (int)kb, (int)kc, x, y, z);
           cellB[0] = x >> 1; cellB[1] = y >> 1; cellB[2] = z >> 1;
