@@ -3,9 +3,15 @@
 Workload (one "step"): the quaternary bcc anneal -- bcc 32^3 conventional
 cells (65536 sites), 4 equiatomic species, 4 shells with V^(n) ~ N(0, 30/n
 meV) (zero-mean per shell), T from 3000 K to 100 K in 50 linear steps of 200
-sweeps each, lattice-wide swaps, energy recounted after every sweep.  One
-step = the whole anneal (>= 655.36 M swap trials) from the same synthetic
-start, on an HBM-resident lattice, through the C-ABI.
+sweeps each, lattice-wide swaps; the energy is recounted from the lattice at
+the start of every T step and carried through the sweeps as the sum of the
+accepted dE, sampled after every sweep.  One step = the whole anneal
+(>= 655.36 M swap trials) from the same synthetic start, on an HBM-resident
+lattice, through the C-ABI.
+
+--calibrate-copy runs only a plain 256 MiB device-to-device copy (three
+times), the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters
+into bytes.
 
 Prints ONE JSON line.  With --gpus N > 1 it exits: this build runs on one GPU.
 """
@@ -30,6 +36,24 @@ def log(*a):
 def bytes_per_trial(z: int) -> int:
     # int8 occupations of both sites and their z neighbours, plus two writes
     return 2 * (z + 1) + 2
+
+
+# HBM bytes per launch of the plain anneal kernel, from rocprofv3 --pmc
+# FETCH_SIZE (51.125 KiB, x2.0 copy-kernel scale) and WRITE_SIZE (65.22 KiB,
+# x1.0): the lattice is read once into LDS and written back once.
+TRAFFIC_BYTES = int(round((51.125 * 2.0 + 65.21875) * 1024))
+PLAIN_KERNEL = "k_metro_nat<bcc, R=4, 32-bit counts, 32 cells>"
+
+
+def calibrate_copy():
+    import torch
+    n = 256 << 20
+    a = torch.ones(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "copies": 3}))
 
 
 def cpu_baseline(chain, budget_s: float):
@@ -101,7 +125,11 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(the multi-GPU layout is described in DESIGN.md)"}))
@@ -164,8 +192,9 @@ def main():
                    "form": "plain (one chain per call)", "trials_per_step": trials_per_launch,
                    "bytes_per_trial": bpt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": "k_metro_par", "kernel_ms": float(np.mean(kernel_ms))},
+                     "frac": achieved / peak, "traffic": TRAFFIC_BYTES,
+                     "kernel": PLAIN_KERNEL, "kernel_ms": float(np.mean(kernel_ms)),
+                     "bytes_per_launch": trials_per_launch * bpt},
         "results": {"E_per_atom_first_T": float(last.energies_of_T[0]),
                     "E_per_atom_last_T": float(last.energies_of_T[-1]),
                     "C_max": float(np.max(last.C_of_T)),
