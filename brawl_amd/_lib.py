@@ -26,7 +26,12 @@ EXPORTS = (
     "bwl_set_config", "bwl_get_config", "bwl_pair_histogram", "bwl_total_energy",
     "bwl_site_energy", "bwl_pair_energy", "bwl_genrand_seed", "bwl_genrand_get_state",
     "bwl_genrand_set_state", "bwl_initial_setup", "bwl_mc_steps", "bwl_metropolis_anneal",
-    "bwl_metropolis_anneal_batch",
+    "bwl_metropolis_anneal_batch", "bwl_lattice_shells", "bwl_radial_densities",
+    "bwl_nested_sampling", "bwl_nested_sampling_batch", "bwl_wl_create", "bwl_wl_destroy",
+    "bwl_wl_n_walkers", "bwl_wl_windows", "bwl_wl_hist_stride", "bwl_wl_initial_setup",
+    "bwl_wl_set_walker", "bwl_wl_get_walker", "bwl_wl_genrand_get_state",
+    "bwl_wl_genrand_set_state", "bwl_wl_get_lng", "bwl_wl_set_lng", "bwl_wl_enter_windows",
+    "bwl_wl_sweeps",
 )
 
 
@@ -43,6 +48,18 @@ class AnnealParams(C.Structure):
 
 class RunStats(C.Structure):
     _fields_ = [("trials", C.c_int64), ("accepted", C.c_int64), ("kernel_ms", C.c_double)]
+
+
+class NsParams(C.Structure):
+    _fields_ = [("n_walkers", C.c_int32), ("n_steps", C.c_int32), ("n_iter", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class WlParams(C.Structure):
+    _fields_ = [("bins", C.c_int32), ("num_windows", C.c_int32),
+                ("walkers_per_window", C.c_int32), ("bin_overlap", C.c_float),
+                ("energy_min", C.c_float), ("energy_max", C.c_float), ("nbr_swap", C.c_int32),
+                ("mc_sweeps", C.c_int32), ("rank_offset", C.c_uint32)]
 
 
 class BrawlError(RuntimeError):
@@ -86,6 +103,25 @@ def load():
         "bwl_metropolis_anneal": (C.c_int, [vp, P(AnnealParams), dp, dp, dp, P(RunStats)]),
         "bwl_metropolis_anneal_batch": (C.c_int, [C.c_int32, P(vp), P(AnnealParams), P(dp), P(dp),
                                                   P(dp), P(RunStats), dp]),
+        "bwl_lattice_shells": (C.c_int, [vp, C.c_int32, dp]),
+        "bwl_radial_densities": (C.c_int, [vp, C.c_int32, dp]),
+        "bwl_nested_sampling": (C.c_int, [vp, P(NsParams), i64p, dp, dp, P(RunStats)]),
+        "bwl_nested_sampling_batch": (C.c_int, [C.c_int32, P(vp), P(NsParams), P(i64p), P(dp),
+                                                P(dp), P(RunStats), dp]),
+        "bwl_wl_create": (C.c_int, [P(LatticeDesc), dp, P(WlParams), P(vp)]),
+        "bwl_wl_destroy": (C.c_int, [vp]),
+        "bwl_wl_n_walkers": (C.c_int, [vp, i32p]),
+        "bwl_wl_windows": (C.c_int, [vp, i32p, dp]),
+        "bwl_wl_hist_stride": (C.c_int, [vp, i32p]),
+        "bwl_wl_initial_setup": (C.c_int, [vp, i64p, dp]),
+        "bwl_wl_set_walker": (C.c_int, [vp, C.c_int32, i8p]),
+        "bwl_wl_get_walker": (C.c_int, [vp, C.c_int32, i8p]),
+        "bwl_wl_genrand_get_state": (C.c_int, [vp, C.c_int32, u32p, i32p]),
+        "bwl_wl_genrand_set_state": (C.c_int, [vp, C.c_int32, u32p, C.c_int32]),
+        "bwl_wl_get_lng": (C.c_int, [vp, dp, dp]),
+        "bwl_wl_set_lng": (C.c_int, [vp, dp, dp]),
+        "bwl_wl_enter_windows": (C.c_int, [vp, C.c_int64, i64p, i32p]),
+        "bwl_wl_sweeps": (C.c_int, [vp, C.c_double, C.c_int32, i64p, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -262,6 +262,7 @@ __host__ __device__ inline DD dd_log(double u) {
 
 // True iff log(u) < t with log correctly rounded to FP64 (u > 0).
 __host__ __device__ inline bool log_lt(double u, double t) {
+  if (u <= 0.0) return t > -INFINITY;  // log(0) = -inf
   double l = log(u);
   double tol = fabs(l) * 8.9e-16 + 1e-300;
   if (l < t - tol) return true;

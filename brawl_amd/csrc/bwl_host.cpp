@@ -320,3 +320,113 @@ int check_config(const Geo &g, const int8_t *config, std::string &err) {
 }
 
 }  // namespace bwl
+
+namespace bwl {
+
+void lattice_shells_host(const Geo &g, int wc, double *shells) {
+  // Distances are sqrt of integers, so the distinct values are those of the
+  // distinct squared distances; 0 is always present (site (1,1,1) is
+  // occupied, and the reference pads its sorted list with zeros).
+  const int m0 = g.m[0], m1 = g.m[1], m2 = g.m[2];
+  const long long dmax = (long long)(m0 - 1) * (m0 - 1) + (long long)(m1 - 1) * (m1 - 1) +
+                         (long long)(m2 - 1) * (m2 - 1);
+  std::vector<uint8_t> seen((size_t)dmax + 1, 0);
+  seen[0] = 1;
+  for (int k = 0; k < m2; ++k)
+    for (int j = 0; j < m1; ++j)
+      for (int i = 0; i < m0; ++i) {
+        const bool occ = g.lattice == BWL_FCC ? (((i + j + k) & 1) == 0)
+                                              : ((i & 1) == (j & 1) && (j & 1) == (k & 1));
+        if (occ) seen[(size_t)(i * i + j * j + k * k)] = 1;
+      }
+  std::vector<double> vals;
+  for (long long d = 0; d <= dmax; ++d)
+    if (seen[(size_t)d]) {
+      // REAL(4) sum of the squares, then REAL(4) sqrt (analytics.f90:247-249)
+      vals.push_back((double)sqrtf((float)d));
+    }
+  for (int s = 0; s < wc; ++s) shells[s] = 0.0;
+  // a shell ends where the next distance differs by 1e-3 or more
+  int cnt = 0;
+  for (size_t i = 0; i + 1 < vals.size() && cnt < wc; ++i) {
+    if (fabs(vals[i] - vals[i + 1]) < 1e-3) continue;
+    shells[cnt++] = vals[i];
+  }
+}
+
+void asro_offsets_host(const Geo &g, int wc, const double *shells, std::vector<int32_t> &off3,
+                       int32_t *shell_start) {
+  const int w[3] = {std::min(g.n[0], 5), std::min(g.n[1], 5), std::min(g.n[2], 5)};
+  std::vector<std::vector<int32_t>> per((size_t)wc);
+  for (int dz = -w[2]; dz <= w[2]; ++dz)
+    for (int dy = -w[1]; dy <= w[1]; ++dy)
+      for (int dx = -w[0]; dx <= w[0]; ++dx) {
+        const bool lat = g.lattice == BWL_FCC
+                             ? (((dx + dy + dz) & 1) == 0)
+                             : (((dx ^ dy) & 1) == 0 && ((dy ^ dz) & 1) == 0);
+        if (!lat) continue;
+        // minimum image of the index difference, as the reference computes
+        // it from site (1,1,1): REAL(4) differences, NINT of the ratio
+        const int d[3] = {dx, dy, dz};
+        double r[3];
+        for (int a = 0; a < 3; ++a) {
+          const int m = g.m[a];
+          const int j = ((d[a] % m) + m) % m + 1;
+          double v = (double)(float)(1 - j);
+          v = v - (double)(float)m * (double)lround(v / (double)(float)m);
+          r[a] = v;
+        }
+        const double dist = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        for (int l = 0; l < wc; ++l)
+          if (fabs(dist - shells[l]) < 1e-3) {
+            per[(size_t)l].push_back(dx);
+            per[(size_t)l].push_back(dy);
+            per[(size_t)l].push_back(dz);
+          }
+      }
+  off3.clear();
+  shell_start[0] = 0;
+  for (int l = 0; l < wc; ++l) {
+    off3.insert(off3.end(), per[(size_t)l].begin(), per[(size_t)l].end());
+    shell_start[l + 1] = (int32_t)(off3.size() / 3);
+  }
+}
+
+void wl_setup_host(const Geo &g, int bins, int num_windows, float bin_overlap, float energy_min,
+                   float energy_max, int32_t *windows, double *bin_edges) {
+  // divide_range: window i ends at floor(factor * i + 1) with
+  // factor = (bins - 1) / ((n + 1)^1 - 1); the next starts one bin later
+  std::vector<int32_t> iv((size_t)2 * num_windows);
+  const double factor = ((double)bins - 1.0) / (pow((double)num_windows + 1.0, 1.0) - 1.0);
+  iv[0] = 1;
+  iv[(size_t)2 * (num_windows - 1) + 1] = bins;
+  for (int i = 2; i <= num_windows; ++i) {
+    iv[(size_t)2 * (i - 2) + 1] = (int32_t)floor(factor * pow((double)(i - 1), 1.0) + 1.0);
+    iv[(size_t)2 * (i - 1)] = iv[(size_t)2 * (i - 2) + 1] + 1;
+  }
+  for (int q = 0; q < 2 * num_windows; ++q) windows[q] = iv[(size_t)q];
+  // create_overlap: every window after the first starts ceil(overlap *
+  // width of the previous window) bins earlier (at least 2)
+  if (num_windows > 1) {
+    for (int i = 2; i <= num_windows - 1; ++i) {
+      const int nb = windows[2 * (i - 2) + 1] - windows[2 * (i - 2)] + 1;
+      int ov = (int)ceilf(bin_overlap * (float)nb);
+      ov = std::max(ov, 2);
+      windows[2 * (i - 1)] = iv[(size_t)2 * (i - 1)] - ov;
+      windows[2 * (i - 1) + 1] = iv[(size_t)2 * (i - 1) + 1];
+    }
+    const int nw = num_windows;
+    const int nb = windows[2 * (nw - 2) + 1] - windows[2 * (nw - 2)];
+    int ov = (int)ceilf(bin_overlap * (float)nb);
+    ov = std::max(ov, 2);
+    windows[2 * (nw - 1)] = iv[(size_t)2 * (nw - 1)] - ov;
+    windows[2 * (nw - 1) + 1] = iv[(size_t)2 * (nw - 1) + 1];
+  }
+  // create_energy_bins: meV/atom (REAL(4)) to Ry for the whole lattice
+  const double to_ry = (double)g.n_atoms / (13.605693122 * 1000.0);
+  const double width = (double)((energy_max - energy_min) / (float)bins) * to_ry;
+  for (int i = 1; i <= bins + 1; ++i)
+    bin_edges[i - 1] = (double)energy_min * to_ry + (double)(i - 1) * width;
+}
+
+}  // namespace bwl

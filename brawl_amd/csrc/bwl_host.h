@@ -46,6 +46,25 @@ int plan_parallel(const Geo &g, ParPlan &p, std::string &err);
 // an axis is shorter than twice its halo.
 bool make_nat(const Geo &g, NatGeo &nat);
 
+// lattice_shells (analytics.f90:205-275): the first `wc` distinct REAL(4)
+// distances from site (1,1,1) to the occupied sites (0 first), in doubled-
+// grid units.  Shells the lattice does not have stay 0.
+void lattice_shells_host(const Geo &g, int wc, double *shells);
+
+// Window offsets of radial_densities (analytics.f90:293-404), grouped by
+// shell: offsets (dx,dy,dz) in the +-min(n,5) doubled-cell window (listed
+// with multiplicity when the window wraps a short axis) that join two
+// occupied sites whose minimum-image distance is within 1e-3 of shell l.
+// off3 gets 3 ints per entry; shell_start gets wc+1 entries.
+void asro_offsets_host(const Geo &g, int wc, const double *shells, std::vector<int32_t> &off3,
+                       int32_t *shell_start);
+
+// divide_range + create_overlap + create_energy_bins (wang-landau.F90:
+// 855-992): windows gets (start, end) 1-based bins per window, bin_edges
+// bins+1 energies (Ry, whole lattice).
+void wl_setup_host(const Geo &g, int bins, int num_windows, float bin_overlap, float energy_min,
+                   float energy_max, int32_t *windows, double *bin_edges);
+
 // Validate a reference-layout config: species 1..S on sites, 0 elsewhere.
 int check_config(const Geo &g, const int8_t *config, std::string &err);
 

@@ -69,6 +69,71 @@ struct ParArgs {
   long long *counters;   // [0] = next round counter, [1] += accepted, [2] += trials
 };
 
+// Nested sampling (nested_sampling.f90:45-206), one wavefront per run.
+struct NsArgs {
+  Geo g;
+  const int32_t *off;
+  const double *vex;
+  int8_t *walkers;        // n_walkers compact lattices, g.compact_bytes apart
+  const double *rnde;     // per walker: the genrand() drawn after its set-up
+  double *E;              // out: final walker energies
+  uint32_t *mt;           // 624 words, in/out
+  int32_t *mt_index;      // in/out
+  int32_t n_walkers, n_steps, n_iter, n_at;
+  long long max_redraws;  // bound on second-site redraws of one trial
+  double *e_limits;       // n_iter energy limits
+  long long *counters;    // [0] trials, [1] accepted, [2] status (0 ok, 1 redraw bound hit)
+};
+
+// One `sweeps` call of one Wang-Landau walker (wang-landau.F90:539-632).
+struct WlArgs {
+  Geo g;
+  const int32_t *off;
+  const double *vex;
+  int8_t *compact;
+  uint32_t *mt;
+  int32_t *mt_index;
+  int32_t nbr_swap, mc_sweeps, bins, start_idx, end_idx;  // 1-based window bins
+  double wl_f;
+  const double *bin_edges;  // bins + 1
+  double *lng;              // bins, in/out
+  double *hist;             // end_idx - start_idx + 1, in/out
+  long long *accepted;      // out
+};
+
+// enter_energy_window (wang-landau.F90:643-741) of one walker, resumable:
+// the kernel stops when the walker is inside its window (status 0), when
+// the reference would re-run initial_setup (status 1; the host does that
+// and relaunches with resume = 1), or after `budget` trials (status 2).
+struct WlEnterArgs {
+  Geo g;
+  const int32_t *off;
+  const double *vex;
+  int8_t *compact;
+  uint32_t *mt;
+  int32_t *mt_index;
+  double min_e, max_e, denom;
+  long long period;        // n_atoms * 250 iterations between re-set-ups
+  long long budget;        // trials this launch may make
+  int32_t init_e;          // recount the energy first
+  int32_t *resume;         // in/out: skip the first iteration's count
+  double *e_unswapped;     // in/out
+  long long *i_steps;      // in/out
+  long long *trials;       // in/out, cumulative
+  int32_t *status;         // out
+};
+
+// Windowed pair-count histogram of radial_densities (analytics.f90:293-404).
+struct AsroArgs {
+  Geo g;
+  const int8_t *compact;
+  int32_t wc;                 // shells including the on-site one
+  int32_t shell_start[17];    // offsets of shell l: [shell_start[l], shell_start[l+1])
+  const int32_t *off;         // 3 ints (dx, dy, dz) per entry, doubled-grid units
+  unsigned long long *hist;   // H(S, S, wc), zeroed by the caller
+};
+constexpr int kMaxAsroShells = 16;
+
 hipError_t launch_pack(const int8_t *ref, int8_t *compact, const Geo &g, hipStream_t s);
 hipError_t launch_unpack(const int8_t *compact, int8_t *ref, const Geo &g, hipStream_t s);
 size_t hist_lds_bytes(const Geo &g, int threads);
@@ -84,5 +149,15 @@ hipError_t launch_metro_par_fn(const void *fn, const ParArgs *args_dev, int n_ch
                                int threads, hipStream_t s);
 hipError_t launch_metro_par(const ParArgs *args_dev, int n_chains, int lattice, int R, int n,
                             bool nat, size_t lds, int threads, bool wide, hipStream_t s);
+
+size_t ns_lds_bytes(const Geo &g, int n_walkers);
+hipError_t launch_ns(const NsArgs *args_dev, int n, size_t lds, hipStream_t s);
+size_t wl_lds_bytes(const Geo &g, int bins, int window_bins);
+hipError_t launch_wl_sweeps(const WlArgs *args_dev, int n, size_t lds, hipStream_t s);
+hipError_t launch_wl_average(double *lng, double *hist, int n_windows, int walkers_per_window,
+                             int bins, int hist_stride, hipStream_t s);
+size_t wl_enter_lds_bytes(const Geo &g);
+hipError_t launch_wl_enter(const WlEnterArgs *args_dev, int n, size_t lds, hipStream_t s);
+hipError_t launch_asro(const AsroArgs &a, hipStream_t s);
 
 }  // namespace bwl

@@ -10,13 +10,11 @@
 // (the parallel Metropolis sampler is in bwl_metro_par.hip)
 #include <hip/hip_runtime.h>
 
-#include "bwl_device.h"
 #include "bwl_kernels.h"
+#include "bwl_seq.h"
 
 namespace bwl {
 
-// The reference's Boltzmann constant (constants.f90:41-46, typo kept).
-__device__ const double kKbInRy = 8.167333262e-5 / 13.605693122990;
 
 // ------------------------------------------------------------------
 // layout conversion
@@ -104,44 +102,6 @@ __global__ __launch_bounds__(512) void k_histogram(const HistArgs *__restrict__ 
   }
 }
 
-// ------------------------------------------------------------------
-// One site's energy in the reference's order: lane n sums shell n+1 from
-// 0.0 in list order (bw_hamiltonian.f90 *_shellN_energy), then the shells
-// are added left to right (*_energy_Rshells).
-// ------------------------------------------------------------------
-__device__ inline double shell_sum_lane(const int8_t *lat, const Geo &g, const int32_t *off,
-                                        const double *vex, int addr, int sub, int species, int n) {
-  const int32_t *o = off + sub * g.z;
-  double e = 0.0;
-  for (int t = g.shell_start[n]; t < g.shell_start[n + 1]; ++t) {
-    int nb = lat[addr + o[t]];
-    e = e + vex[(species - 1) + g.S * ((nb - 1) + g.S * n)];
-  }
-  return e;
-}
-
-// Energy of sites A and B (lanes [0,R) and [R,2R)); returns e_A + e_B on
-// every lane, each in reference order.
-__device__ inline double pair_energy_lanes(const int8_t *lat, const Geo &g, const int32_t *off,
-                                           const double *vex, int addrA, int subA,
-                                           int addrB, int subB, double *eA_out) {
-  const int lane = threadIdx.x & 63;
-  double v = 0.0;
-  if (lane < 2 * g.R) {
-    bool isA = lane < g.R;
-    int n = isA ? lane : lane - g.R;
-    int addr = isA ? addrA : addrB;
-    v = shell_sum_lane(lat, g, off, vex, addr, isA ? subA : subB, lat[addr], n);
-  }
-  double eA = __shfl(v, 0), eB = __shfl(v, g.R);
-  for (int n = 1; n < g.R; ++n) {
-    eA = eA + __shfl(v, n);
-    eB = eB + __shfl(v, g.R + n);
-  }
-  if (eA_out) *eA_out = eA;
-  return eA + eB;
-}
-
 __global__ __launch_bounds__(64) void k_site_energy(SiteArgs A) {
   extern __shared__ __align__(16) uint8_t smem[];
   const Geo &g = A.g;
@@ -166,83 +126,6 @@ __global__ __launch_bounds__(64) void k_site_energy(SiteArgs A) {
 // Bit-exact sequential Metropolis (metropolis.F90:751-891), one wavefront.
 // All lanes run the same control flow; the MT19937 state is in LDS.
 // ------------------------------------------------------------------
-struct SiteXYZ {
-  int x, y, z;  // 0-based doubled-grid coordinates
-};
-
-// bcc_random_site / fcc_random_site (random_site.f90:127-139, 182-193):
-// floor(c*genrand()*n) equals (r*c*n) >> 32 exactly since r*c*n < 2^53.
-__device__ inline SiteXYZ random_site_dev(const Geo &g, uint32_t *st, int &mti) {
-  uint32_t r1 = mt_next(st, mti);
-  int k = (int)(((uint64_t)r1 * (uint64_t)(2 * g.n[2])) >> 32) + 1;
-  SiteXYZ s;
-  if (g.lattice == 2) {
-    uint32_t r2 = mt_next(st, mti);
-    uint32_t r3 = mt_next(st, mti);
-    int i = 2 * (int)(((uint64_t)r2 * (uint64_t)g.n[0]) >> 32) + 2 - (k & 1);
-    int j = 2 * (int)(((uint64_t)r3 * (uint64_t)g.n[1]) >> 32) + 2 - (k & 1);
-    s.x = i - 1; s.y = j - 1; s.z = k - 1;
-  } else {
-    uint32_t r2 = mt_next(st, mti);
-    uint32_t r3 = mt_next(st, mti);
-    int i = (int)(((uint64_t)r2 * (uint64_t)(2 * g.n[0])) >> 32) + 1;
-    int j = 2 * (int)(((uint64_t)r3 * (uint64_t)g.n[1]) >> 32) + 1 + pmod(i - (k & 1), 2);
-    s.x = i - 1; s.y = j - 1; s.z = k - 1;
-  }
-  return s;
-}
-
-// random_site.f90:38-61 nearest-neighbour tables (one draw)
-__constant__ int8_t kRsBcc[8][3] = {{1, 1, 1},   {1, 1, -1},  {1, -1, 1},  {1, -1, -1},
-                                    {-1, 1, 1},  {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
-__constant__ int8_t kRsFcc[12][3] = {{0, 1, 1},  {0, 1, -1},  {0, -1, 1}, {0, -1, -1},
-                                     {1, 1, 0},  {1, -1, 0},  {1, 0, 1},  {1, 0, -1},
-                                     {-1, 1, 0}, {-1, -1, 0}, {-1, 0, 1}, {-1, 0, -1}};
-
-__device__ inline SiteXYZ random_nbr_dev(const Geo &g, SiteXYZ s, uint32_t *st, int &mti) {
-  uint32_t r = mt_next(st, mti);
-  const int8_t *d;
-  if (g.lattice == 2)
-    d = kRsBcc[((uint64_t)r * 8u) >> 32];
-  else
-    d = kRsFcc[((uint64_t)r * 12u) >> 32];
-  SiteXYZ t;
-  t.x = pmod(s.x + d[0], g.m[0]);
-  t.y = pmod(s.y + d[1], g.m[1]);
-  t.z = pmod(s.z + d[2], g.m[2]);
-  return t;
-}
-
-// Whole-lattice energy by the wavefront: 0.5 * sum of site energies, each
-// site's packed neighbour counts contracted with V_ex in FP64.
-__device__ inline double wave_total_energy(const int8_t *lat, const Geo &g, const int32_t *off,
-                                           const double *vex, double *red) {
-  const int cells = g.n[0] * g.n[1] * g.n[2];
-  const int nsites = g.nsub * cells;
-  double part = 0.0;
-  for (int i = threadIdx.x; i < nsites; i += blockDim.x) {
-    int s = i / cells;
-    int r = i - s * cells;
-    int cz = r / (g.n[0] * g.n[1]);
-    r -= cz * g.n[0] * g.n[1];
-    int cy = r / g.n[0];
-    int cx = r - cy * g.n[0];
-    int addr = padded_addr(g, s, cx, cy, cz);
-    int a = lat[addr];
-    const int32_t *o = off + s * g.z;
-    for (int n = 0; n < g.R; ++n) {
-      uint64_t c = 0;
-      for (int t = g.shell_start[n]; t < g.shell_start[n + 1]; ++t)
-        c += (uint64_t)1 << (8 * (lat[addr + o[t]] - 1));
-      for (int b = 0; b < g.S; ++b) {
-        uint32_t k = (uint32_t)(c >> (8 * b)) & 0xffu;
-        part += (double)k * vex[(a - 1) + g.S * (b + g.S * n)];
-      }
-    }
-  }
-  return 0.5 * block_sum(part, red);
-}
-
 __global__ __launch_bounds__(64) void k_metro_seq(SeqArgs A) {
   extern __shared__ __align__(16) uint8_t smem[];
   const Geo &g = A.g;
@@ -252,8 +135,6 @@ __global__ __launch_bounds__(64) void k_metro_seq(SeqArgs A) {
   uint32_t *st = (uint32_t *)(smem + o);
   o += 624 * 4;
   double *vex = (double *)(smem + o);
-  o += sizeof(double) * g.S * g.S * g.R;
-  double *red = (double *)(smem + o);
 
   for (int i = lane; i < 624; i += 64) st[i] = A.mt[i];
   for (int i = lane; i < g.S * g.S * g.R; i += 64) vex[i] = A.vex[i];
@@ -299,7 +180,7 @@ __global__ __launch_bounds__(64) void k_metro_seq(SeqArgs A) {
     }
     if (acc) accepted++;
     if (A.energies && A.sample_every > 0 && trial % A.sample_every == 0) {
-      double e = wave_total_energy(lat, g, A.off, vex, red);
+      const double e = ref_total_energy(lat, g, A.off, vex);
       if (lane == 0) A.energies[sample_k] = e;
       sample_k++;
     }

@@ -1,0 +1,478 @@
+// bwl_walkers.hip -- sequential walkers on the device: nested-sampling runs,
+// Wang-Landau walker sweeps and window entry, and the windowed pair-count
+// histogram behind the radial densities.
+//
+// Each nested-sampling run and each Wang-Landau walker is one wavefront
+// that consumes its own MT19937 stream exactly as the reference's serial
+// code does (one MPI rank per walker there), with site and pair energies in
+// the reference's FP64 order, so every walker's trajectory is bit-identical
+// to the reference's.  Throughput comes from running many walkers (one per
+// CU) in one launch.
+#include <hip/hip_runtime.h>
+
+#include "bwl_kernels.h"
+#include "bwl_seq.h"
+
+namespace bwl {
+
+namespace {
+
+// Swap the species of two sites (both images of each, all lanes help).
+__device__ inline void swap_sites(int8_t *lat, const Geo &g, SiteXYZ a, int suba, int8_t va,
+                                  SiteXYZ b, int subb, int8_t vb) {
+  const int lane = threadIdx.x & 63;
+  __syncthreads();
+  write_site(lat, g, suba, a.x >> 1, a.y >> 1, a.z >> 1, vb, lane, 64);
+  write_site(lat, g, subb, b.x >> 1, b.y >> 1, b.z >> 1, va, lane, 64);
+  __syncthreads();
+}
+
+// bin_index (wang-landau.F90:515-523): truncation toward zero, +1.
+__device__ inline int wl_bin_index(double e, const double *edges, int bins) {
+  const double range = edges[bins] - edges[0];
+  return (int)(((e - edges[0]) / range) * (double)bins) + 1;
+}
+
+__device__ inline double genrand_dev(uint32_t *st, int &mti) {
+  return (double)mt_next(st, mti) * (1.0 / 4294967296.0);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------
+// Nested sampling: the whole serial loop of nested_sampling.f90:78-200 for
+// one run.  Walker lattices stay in HBM; the walker being walked is staged
+// in LDS.  Walker energies live in LDS.
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const NsArgs &A = args[blockIdx.x];
+  const Geo &g = A.g;
+  const int lane = threadIdx.x;
+  int8_t *lat = (int8_t *)smem;
+  size_t o = (g.lds_bytes + 15) & ~15;
+  uint32_t *st = (uint32_t *)(smem + o);
+  o += 624 * 4;
+  double *vex = (double *)(smem + o);
+  o += sizeof(double) * g.S * g.S * g.R;
+  double *E = (double *)(smem + o);
+
+  for (int i = lane; i < 624; i += 64) st[i] = A.mt[i];
+  for (int i = lane; i < g.S * g.S * g.R; i += 64) vex[i] = A.vex[i];
+  int mti = A.mt_index[0];
+  const int nw = A.n_walkers;
+  const size_t cb = (size_t)g.compact_bytes;
+
+  // walker energies: total_energy + genrand()*1e-8 (nested_sampling.f90:95,
+  // 1e-8 is a REAL(4) literal there)
+  for (int w = 0; w < nw; ++w) {
+    __syncthreads();
+    stage_lattice(lat, A.walkers + cb * w, g, 0);
+    __syncthreads();
+    const double e = ref_total_energy(lat, g, A.off, vex);
+    if (lane == 0) E[w] = e + A.rnde[w] * (double)1e-8f;
+  }
+  __syncthreads();
+
+  long long trials = 0, accepted = 0;
+  int status = 0;
+  int extra = 0, n_acc = 0;
+  const int half = (int)((float)nw / 2.0f);
+  for (int it = 1; it <= A.n_iter && status == 0; ++it) {
+    // maxloc: the first maximum (nested_sampling.f90:117)
+    double best = -INFINITY;
+    int bi = 0x7fffffff;  // no walker seen
+    for (int w = lane; w < nw; w += 64)
+      if (bi == 0x7fffffff || E[w] > best) {
+        best = E[w];
+        bi = w;
+      }
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+      const double ob = __shfl_xor(best, m);
+      const int oi = __shfl_xor(bi, m);
+      if (oi != 0x7fffffff && (bi == 0x7fffffff || ob > best || (ob == best && oi < bi))) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    const int imax = bi;
+    const double limit = best;
+    if (lane == 0) A.e_limits[it - 1] = limit;
+    // walk-length adaptation (nested_sampling.f90:129-144)
+    if (it % half == 0) {
+      if (((float)n_acc < (float)A.n_at * 0.05f) && (extra < A.n_steps * 100)) extra += A.n_steps;
+    }
+    // clone a random walker over the discarded one (nested_sampling.f90:149-152)
+    const double rnd = genrand_dev(st, mti);
+    int irnd = (int)ceil(rnd * (double)nw);
+    if (irnd < 1) irnd = 1;
+    const int src = irnd - 1;
+    __syncthreads();
+    stage_lattice(lat, A.walkers + cb * src, g, 0);
+    __syncthreads();
+    double ecur = E[src];
+    n_acc = 0;
+    // the walk (nested_sampling.f90:157-192)
+    for (int stp = 1; stp <= A.n_steps + extra; ++stp) {
+      const SiteXYZ r1 = random_site_dev(g, st, mti);
+      const int a1 = padded_addr_xyz(g, r1.x, r1.y, r1.z);
+      const int s1 = lat[a1];
+      SiteXYZ r2;
+      int a2, s2;
+      long long redraws = 0;
+      do {
+        r2 = random_site_dev(g, st, mti);
+        a2 = padded_addr_xyz(g, r2.x, r2.y, r2.z);
+        s2 = lat[a2];
+      } while (s1 == s2 && ++redraws < A.max_redraws);
+      if (s1 == s2) {
+        status = 1;
+        break;
+      }
+      const int sub1 = sub_of(g.lattice, r1.x & 1, r1.y & 1);
+      const int sub2 = sub_of(g.lattice, r2.x & 1, r2.y & 1);
+      const double eu = pair_energy_lanes(lat, g, A.off, vex, a1, sub1, a2, sub2, nullptr);
+      swap_sites(lat, g, r1, sub1, (int8_t)s1, r2, sub2, (int8_t)s2);
+      const double es = pair_energy_lanes(lat, g, A.off, vex, a1, sub1, a2, sub2, nullptr);
+      const double de = es - eu;
+      ++trials;
+      if (ecur + de < limit) {
+        ecur = ecur + de;
+        ++n_acc;
+        ++accepted;
+      } else {
+        swap_sites(lat, g, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
+      }
+    }
+    __syncthreads();
+    if (lane == 0) E[imax] = ecur;
+    unstage_lattice(lat, A.walkers + cb * imax, g, 0);
+    __syncthreads();
+  }
+  for (int w = lane; w < nw; w += 64) A.E[w] = E[w];
+  for (int i = lane; i < 624; i += 64) A.mt[i] = st[i];
+  if (lane == 0) {
+    A.mt_index[0] = mti;
+    A.counters[0] = trials;
+    A.counters[1] = accepted;
+    A.counters[2] = status;
+  }
+}
+
+// ------------------------------------------------------------------
+// Wang-Landau `sweeps` (wang-landau.F90:539-632) for one walker, without the
+// window average (a separate kernel does that after every walker's call).
+// The walker's ln g and visit histogram are staged in LDS.
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ args) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const WlArgs &A = args[blockIdx.x];
+  const Geo &g = A.g;
+  const int lane = threadIdx.x;
+  const int hb = A.end_idx - A.start_idx + 1;
+  int8_t *lat = (int8_t *)smem;
+  size_t o = (g.lds_bytes + 15) & ~15;
+  uint32_t *st = (uint32_t *)(smem + o);
+  o += 624 * 4;
+  double *vex = (double *)(smem + o);
+  o += sizeof(double) * g.S * g.S * g.R;
+  double *lng = (double *)(smem + o);
+  o += sizeof(double) * A.bins;
+  double *edges = (double *)(smem + o);
+  o += sizeof(double) * (A.bins + 1);
+  double *hist = (double *)(smem + o);
+
+  for (int i = lane; i < 624; i += 64) st[i] = A.mt[i];
+  for (int i = lane; i < g.S * g.S * g.R; i += 64) vex[i] = A.vex[i];
+  for (int i = lane; i < A.bins; i += 64) lng[i] = A.lng[i];
+  for (int i = lane; i <= A.bins; i += 64) edges[i] = A.bin_edges[i];
+  for (int i = lane; i < hb; i += 64) hist[i] = A.hist[i];
+  stage_lattice(lat, A.compact, g, 0);
+  __syncthreads();
+  int mti = A.mt_index[0];
+  const int bins = A.bins;
+  const long long cadence = (long long)(int)(0.02 * (double)(float)g.n_atoms);
+  double e_un = ref_total_energy(lat, g, A.off, vex);
+  long long accepted = 0;
+  const long long total = (long long)A.mc_sweeps * g.n_atoms;
+  for (long long i = 1; i <= total; ++i) {
+    const SiteXYZ r1 = random_site_dev(g, st, mti);
+    const SiteXYZ r2 = A.nbr_swap ? random_nbr_dev(g, r1, st, mti) : random_site_dev(g, st, mti);
+    const int a1 = padded_addr_xyz(g, r1.x, r1.y, r1.z);
+    const int a2 = padded_addr_xyz(g, r2.x, r2.y, r2.z);
+    const int sub1 = sub_of(g.lattice, r1.x & 1, r1.y & 1);
+    const int sub2 = sub_of(g.lattice, r2.x & 1, r2.y & 1);
+    const int s1 = lat[a1], s2 = lat[a2];
+    // The reference also evaluates pair_energy and swaps when the species
+    // are equal; neither changes the state, so both are skipped here.
+    double e_sw = e_un;
+    if (s1 != s2) {
+      const double pu = pair_energy_lanes(lat, g, A.off, vex, a1, sub1, a2, sub2, nullptr);
+      swap_sites(lat, g, r1, sub1, (int8_t)s1, r2, sub2, (int8_t)s2);
+      const double ps = pair_energy_lanes(lat, g, A.off, vex, a1, sub1, a2, sub2, nullptr);
+      e_sw = e_un - pu + ps;
+    }
+    const int ibin = wl_bin_index(e_un, edges, bins);
+    int jbin = wl_bin_index(e_sw, edges, bins);
+    const bool hit = cadence > 0 && (i % cadence) == 0;
+    if (jbin > A.start_idx - 1 && jbin < A.end_idx + 1) {
+      const double li = (ibin >= 1 && ibin <= bins) ? lng[ibin - 1] : 0.0;
+      const double u = genrand_dev(st, mti);
+      if (log_lt(u, li - lng[jbin - 1])) {
+        ++accepted;
+        e_un = e_sw;
+      } else {
+        if (s1 != s2) swap_sites(lat, g, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
+        jbin = ibin;
+      }
+    } else {
+      jbin = ibin;
+      if (s1 != s2) swap_sites(lat, g, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
+    }
+    // a walker outside its window would index outside the reference's
+    // arrays; such updates are dropped
+    if (lane == 0) {
+      if (hit && jbin >= A.start_idx && jbin <= A.end_idx) hist[jbin - A.start_idx] += 1.0;
+      if (jbin >= 1 && jbin <= bins) lng[jbin - 1] = lng[jbin - 1] + A.wl_f;
+    }
+    __syncthreads();
+  }
+  unstage_lattice(lat, A.compact, g, 0);
+  for (int i = lane; i < A.bins; i += 64) A.lng[i] = lng[i];
+  for (int i = lane; i < hb; i += 64) A.hist[i] = hist[i];
+  for (int i = lane; i < 624; i += 64) A.mt[i] = st[i];
+  if (lane == 0) {
+    A.mt_index[0] = mti;
+    A.accepted[0] = accepted;
+  }
+}
+
+// Window average after `sweeps` (wang-landau.F90:628-631): ln g and the
+// visit histogram summed over a window's walkers in walker order, divided
+// by the number of walkers, and written back to every walker of the window.
+// One thread per (window, bin).
+__global__ void k_wl_average(double *lng, double *hist, int n_windows, int wpw, int bins,
+                             int hist_stride) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = bins + hist_stride;
+  if (t >= n_windows * per) return;
+  const int w = t / per;
+  const int b = t - w * per;
+  double *base;
+  int stride;
+  if (b < bins) {
+    base = lng + (size_t)w * wpw * bins + b;
+    stride = bins;
+  } else {
+    base = hist + (size_t)w * wpw * hist_stride + (b - bins);
+    stride = hist_stride;
+  }
+  double s = base[0];
+  for (int k = 1; k < wpw; ++k) s = s + base[(size_t)k * stride];
+  s = s / (double)wpw;
+  for (int k = 0; k < wpw; ++k) base[(size_t)k * stride] = s;
+}
+
+// ------------------------------------------------------------------
+// enter_energy_window (wang-landau.F90:643-741) for one walker.
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__ args) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const WlEnterArgs &A = args[blockIdx.x];
+  const Geo &g = A.g;
+  const int lane = threadIdx.x;
+  int8_t *lat = (int8_t *)smem;
+  size_t o = (g.lds_bytes + 15) & ~15;
+  uint32_t *st = (uint32_t *)(smem + o);
+  o += 624 * 4;
+  double *vex = (double *)(smem + o);
+
+  for (int i = lane; i < 624; i += 64) st[i] = A.mt[i];
+  for (int i = lane; i < g.S * g.S * g.R; i += 64) vex[i] = A.vex[i];
+  stage_lattice(lat, A.compact, g, 0);
+  __syncthreads();
+  int mti = A.mt_index[0];
+  const double target = (A.min_e + A.max_e) / 2.0;
+  const double cond = fabs(A.max_e - A.min_e) * 0.1;
+  double e_un = A.init_e ? ref_total_energy(lat, g, A.off, vex) : A.e_unswapped[0];
+  long long isteps = A.i_steps[0];
+  long long trials = A.trials[0];
+  long long made = 0;
+  bool skip = A.resume[0] != 0;
+  int status = 0;
+  for (;;) {
+    if (!skip) {
+      ++isteps;
+      if (isteps % A.period == 0) {
+        isteps = 0;
+        status = 1;  // the host re-runs initial_setup, then resumes here
+        break;
+      }
+    }
+    skip = false;
+    if (e_un < A.max_e - cond && e_un > A.min_e + cond) {
+      e_un = ref_total_energy(lat, g, A.off, vex);
+      if (e_un < A.max_e - cond && e_un > A.min_e + cond) {
+        status = 0;
+        break;
+      }
+      continue;
+    }
+    if (made >= A.budget) {
+      status = 2;
+      skip = true;  // resume before this iteration's trial
+      break;
+    }
+    ++made;
+    ++trials;
+    const SiteXYZ r1 = random_site_dev(g, st, mti);
+    const SiteXYZ r2 = random_site_dev(g, st, mti);
+    const int a1 = padded_addr_xyz(g, r1.x, r1.y, r1.z);
+    const int a2 = padded_addr_xyz(g, r2.x, r2.y, r2.z);
+    const int s1 = lat[a1], s2 = lat[a2];
+    if (s1 != s2) {
+      const int sub1 = sub_of(g.lattice, r1.x & 1, r1.y & 1);
+      const int sub2 = sub_of(g.lattice, r2.x & 1, r2.y & 1);
+      const double pu = pair_energy_lanes(lat, g, A.off, vex, a1, sub1, a2, sub2, nullptr);
+      swap_sites(lat, g, r1, sub1, (int8_t)s1, r2, sub2, (int8_t)s2);
+      const double ps = pair_energy_lanes(lat, g, A.off, vex, a1, sub1, a2, sub2, nullptr);
+      const double e_sw = e_un - pu + ps;
+      const double de = ((e_sw - target) * (e_sw - target) - (e_un - target) * (e_un - target)) /
+                        A.denom;
+      const double u = genrand_dev(st, mti);
+      if (log_lt(u, -de))
+        e_un = e_sw;
+      else
+        swap_sites(lat, g, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
+    }
+  }
+  __syncthreads();
+  unstage_lattice(lat, A.compact, g, 0);
+  for (int i = lane; i < 624; i += 64) A.mt[i] = st[i];
+  if (lane == 0) {
+    A.mt_index[0] = mti;
+    A.e_unswapped[0] = e_un;
+    A.i_steps[0] = isteps;
+    A.trials[0] = trials;
+    A.resume[0] = (status == 1 || status == 2) ? 1 : 0;
+    A.status[0] = status;
+  }
+}
+
+// ------------------------------------------------------------------
+// Windowed pair counts of radial_densities: for every occupied site of
+// species a and every window offset whose minimum-image distance falls on
+// shell l, the species b at that offset: H(a, b, l) += 1.  The lattice is
+// read from HBM (it is small and stays in L2); counts are packed 8 bits per
+// species per shell and flushed to a per-block LDS histogram, then added to
+// the global one with 64-bit atomics (integer sums, so the result does not
+// depend on the order).
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_asro(AsroArgs A) {
+  __shared__ unsigned int lh[kMaxSpecies * kMaxSpecies * kMaxAsroShells];
+  const Geo &g = A.g;
+  const int S = g.S;
+  const int hsize = S * S * A.wc;
+  for (int i = threadIdx.x; i < hsize; i += blockDim.x) lh[i] = 0;
+  __syncthreads();
+  const int cells = g.n[0] * g.n[1] * g.n[2];
+  const int nsites = g.nsub * cells;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nsites;
+       idx += gridDim.x * blockDim.x) {
+    const int s = idx / cells;
+    int r = idx - s * cells;
+    const int cz = r / (g.n[0] * g.n[1]);
+    r -= cz * g.n[0] * g.n[1];
+    const int cy = r / g.n[0];
+    const int cx = r - cy * g.n[0];
+    int bx, by, bz;
+    sub_bits(g.lattice, s, bx, by, bz);
+    const int x = 2 * cx + bx, y = 2 * cy + by, z = 2 * cz + bz;
+    const int a = A.compact[idx];
+    if (a <= 0) continue;
+    for (int l = 0; l < A.wc; ++l) {
+      uint64_t c = 0;
+      int k = 0;
+      for (int t = A.shell_start[l]; t < A.shell_start[l + 1]; ++t) {
+        int xx = x + A.off[3 * t], yy = y + A.off[3 * t + 1], zz = z + A.off[3 * t + 2];
+        xx = xx < 0 ? xx + g.m[0] : (xx >= g.m[0] ? xx - g.m[0] : xx);
+        yy = yy < 0 ? yy + g.m[1] : (yy >= g.m[1] ? yy - g.m[1] : yy);
+        zz = zz < 0 ? zz + g.m[2] : (zz >= g.m[2] ? zz - g.m[2] : zz);
+        const int b = A.compact[compact_addr(g, sub_of(g.lattice, xx & 1, yy & 1), xx >> 1, yy >> 1,
+                                             zz >> 1)];
+        c += (uint64_t)1 << (8 * (b - 1));
+        if (++k == 255 || t + 1 == A.shell_start[l + 1]) {
+          for (int bb = 0; bb < S; ++bb) {
+            const unsigned q = (unsigned)(c >> (8 * bb)) & 0xffu;
+            if (q) atomicAdd(&lh[(a - 1) + S * (bb + S * l)], q);
+          }
+          c = 0;
+          k = 0;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < hsize; i += blockDim.x)
+    if (lh[i]) atomicAdd(&A.hist[i], (unsigned long long)lh[i]);
+}
+
+// ------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------
+size_t ns_lds_bytes(const Geo &g, int n_walkers) {
+  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + sizeof(double) * g.S * g.S * g.R +
+         sizeof(double) * (size_t)n_walkers;
+}
+
+hipError_t launch_ns(const NsArgs *args_dev, int n, size_t lds, hipStream_t s) {
+  hipError_t e = hipFuncSetAttribute((const void *)k_ns,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_ns, dim3(n), dim3(64), lds, s, args_dev);
+  return hipGetLastError();
+}
+
+size_t wl_lds_bytes(const Geo &g, int bins, int window_bins) {
+  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + sizeof(double) * g.S * g.S * g.R +
+         sizeof(double) * (size_t)(2 * bins + 1 + window_bins);
+}
+
+hipError_t launch_wl_sweeps(const WlArgs *args_dev, int n, size_t lds, hipStream_t s) {
+  hipError_t e = hipFuncSetAttribute((const void *)k_wl_sweeps,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_wl_sweeps, dim3(n), dim3(64), lds, s, args_dev);
+  return hipGetLastError();
+}
+
+hipError_t launch_wl_average(double *lng, double *hist, int n_windows, int wpw, int bins,
+                             int hist_stride, hipStream_t s) {
+  const int total = n_windows * (bins + hist_stride);
+  hipLaunchKernelGGL(k_wl_average, dim3((total + 255) / 256), dim3(256), 0, s, lng, hist,
+                     n_windows, wpw, bins, hist_stride);
+  return hipGetLastError();
+}
+
+size_t wl_enter_lds_bytes(const Geo &g) {
+  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + sizeof(double) * g.S * g.S * g.R;
+}
+
+hipError_t launch_wl_enter(const WlEnterArgs *args_dev, int n, size_t lds, hipStream_t s) {
+  hipError_t e = hipFuncSetAttribute((const void *)k_wl_enter,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_wl_enter, dim3(n), dim3(64), lds, s, args_dev);
+  return hipGetLastError();
+}
+
+hipError_t launch_asro(const AsroArgs &a, hipStream_t s) {
+  const int nsites = a.g.nsub * a.g.n[0] * a.g.n[1] * a.g.n[2];
+  int blocks = (nsites + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(k_asro, dim3(blocks), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace bwl

@@ -11,6 +11,7 @@ methods carry the reference's procedure names:
     initial_setup                 initial_setup       (initialise.F90:434-617)
     mc_step / mc_steps            setup%mc_step       (metropolis.F90:751-891)
     metropolis_simulated_annealing  the temperature loop (metropolis.F90:193-437)
+    lattice_shells / radial_densities  analytics.f90:205-404
 
 Arrays use the reference's Fortran layouts (config (2n1,2n2,2n3) int8,
 V_ex (S,S,R) float64, both order='F').  Errors raise BrawlError with the
@@ -168,6 +169,18 @@ class Lattice:
 
     def mc_step(self, beta: float, nbr_swap=False) -> int:
         return self.mc_steps(beta, 1, nbr_swap=nbr_swap)[0]
+
+    # -- analytics ----------------------------------------------------------
+    def lattice_shells(self, wc_range: int) -> np.ndarray:
+        out = np.zeros(int(wc_range))
+        check(self._L.bwl_lattice_shells(self._h, int(wc_range), _p(out, C.c_double)))
+        return out
+
+    def radial_densities(self, wc_range: int) -> np.ndarray:
+        """rho(S, S, wc_range) of the current configuration (Fortran order)."""
+        rho = np.zeros((self.S, self.S, int(wc_range)), order="F")
+        check(self._L.bwl_radial_densities(self._h, int(wc_range), _p(rho, C.c_double)))
+        return rho
 
     # -- parallel sampler -------------------------------------------------
     def metropolis_simulated_annealing(self, *, T, delta_T=0.0, T_steps=1, n_sweeps,

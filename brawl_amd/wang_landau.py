@@ -1,0 +1,237 @@
+"""Wang-Landau walkers over the HIP C-ABI (wang-landau.F90).
+
+`WangLandau` holds num_windows x walkers_per_window walkers on one device
+(walker k is the reference's MPI rank k, in window k // walkers_per_window,
+drawing from MT19937 seed 110179 + 11*k).  Its methods are the reference's
+per-rank routines run for every walker at once:
+
+    enter_windows   enter_energy_window      (wang-landau.F90:643-741)
+    sweeps          sweeps + window average  (wang-landau.F90:539-632)
+    dos_average     dos_average              (wang-landau.F90:1127-1133)
+    dos_combine     dos_combine              (wang-landau.F90:1147-1194)
+
+`run` is the main loop of wl_main (wang-landau.F90:198-306): sweeps until
+every walker's window histogram is flat, then halve ln f, average and
+stitch ln g, until ln f falls below the tolerance.  Not carried over:
+pre_sampling (:757-838), replica_exchange (:1392-1501), window load
+balancing (mpi_window_optimise, :1211-1328), rho_of_E sampling and per-bin
+configuration stores -- the parts that exist to balance MPI ranks or write
+auxiliary output.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import LatticeDesc, WlParams, check
+from .lattice import LATTICE_NAMES
+
+
+def _p(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+@dataclass
+class WlRunResult:
+    lng: np.ndarray          # combined ln g over all bins (min subtracted)
+    bin_edges: np.ndarray
+    iterations: int          # ln f halvings done
+    sweeps_calls: int
+    trials: int
+    kernel_ms: float
+
+
+class WangLandau:
+    def __init__(self, lattice, n, v_ex, *, bins, num_windows, walkers_per_window,
+                 bin_overlap, energy_min, energy_max, mc_sweeps=1, nbr_swap=False,
+                 rank_offset=0):
+        if isinstance(lattice, str):
+            lattice = LATTICE_NAMES[lattice]
+        v = np.asfortranarray(np.asarray(v_ex, dtype=np.float64))
+        self.S, _, self.R = v.shape
+        self.n = tuple(int(x) for x in n)
+        self.grid = tuple(2 * x for x in self.n)
+        self.lattice = int(lattice)
+        self.n_atoms = (4 if self.lattice == _lib.FCC else 2) * self.n[0] * self.n[1] * self.n[2]
+        self.v_ex = v
+        self.params = WlParams(int(bins), int(num_windows), int(walkers_per_window),
+                               float(bin_overlap), float(energy_min), float(energy_max),
+                               int(bool(nbr_swap)), int(mc_sweeps), int(rank_offset))
+        L = _lib.load()
+        d = LatticeDesc(int(lattice), self.n[0], self.n[1], self.n[2], self.S, self.R)
+        h = C.c_void_p()
+        check(L.bwl_wl_create(C.byref(d), _p(v, C.c_double), C.byref(self.params), C.byref(h)))
+        self._h, self._L = h, L
+        nw, hs = C.c_int32(), C.c_int32()
+        check(L.bwl_wl_n_walkers(h, C.byref(nw)))
+        check(L.bwl_wl_hist_stride(h, C.byref(hs)))
+        self.n_walkers, self.hist_stride = int(nw.value), int(hs.value)
+        self.bins, self.num_windows = int(bins), int(num_windows)
+        self.walkers_per_window = int(walkers_per_window)
+        w = np.zeros(2 * self.num_windows, dtype=np.int32)
+        e = np.zeros(self.bins + 1)
+        check(L.bwl_wl_windows(h, _p(w, C.c_int32), _p(e, C.c_double)))
+        self.windows = w.reshape(self.num_windows, 2)   # 1-based (start, end) bins
+        self.bin_edges = e
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.bwl_wl_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- walkers ------------------------------------------------------------
+    def window_of(self, k: int) -> int:
+        return k // self.walkers_per_window
+
+    def initial_setup(self, species_numbers=None, concentrations=None):
+        sn = None if species_numbers is None else np.ascontiguousarray(species_numbers, dtype=np.int64)
+        cc = None if concentrations is None else np.ascontiguousarray(concentrations, dtype=np.float64)
+        check(self._L.bwl_wl_initial_setup(self._h, None if sn is None else _p(sn, C.c_int64),
+                                           None if cc is None else _p(cc, C.c_double)))
+
+    def set_walker(self, k: int, cfg):
+        cfg = np.asfortranarray(cfg, dtype=np.int8)
+        check(self._L.bwl_wl_set_walker(self._h, int(k), _p(cfg, C.c_int8)))
+
+    def get_walker(self, k: int) -> np.ndarray:
+        out = np.zeros(self.grid, dtype=np.int8, order="F")
+        check(self._L.bwl_wl_get_walker(self._h, int(k), _p(out, C.c_int8)))
+        return out
+
+    def genrand_state(self, k: int):
+        mt = np.zeros(624, dtype=np.uint32)
+        mti = C.c_int32()
+        check(self._L.bwl_wl_genrand_get_state(self._h, int(k), _p(mt, C.c_uint32), C.byref(mti)))
+        return mt, int(mti.value)
+
+    def set_genrand_state(self, k: int, mt, mti: int):
+        mt = np.ascontiguousarray(mt, dtype=np.uint32)
+        check(self._L.bwl_wl_genrand_set_state(self._h, int(k), _p(mt, C.c_uint32), int(mti)))
+
+    def lng_hist(self):
+        lng = np.zeros((self.n_walkers, self.bins))
+        hist = np.zeros((self.n_walkers, self.hist_stride))
+        check(self._L.bwl_wl_get_lng(self._h, _p(lng, C.c_double), _p(hist, C.c_double)))
+        return lng, hist
+
+    def set_lng_hist(self, lng=None, hist=None):
+        lp = hp = None
+        if lng is not None:
+            lng = np.ascontiguousarray(lng, dtype=np.float64)
+            assert lng.shape == (self.n_walkers, self.bins)
+            lp = _p(lng, C.c_double)
+        if hist is not None:
+            hist = np.ascontiguousarray(hist, dtype=np.float64)
+            assert hist.shape == (self.n_walkers, self.hist_stride)
+            hp = _p(hist, C.c_double)
+        check(self._L.bwl_wl_set_lng(self._h, lp, hp))
+
+    # -- the reference's routines ------------------------------------------
+    def enter_windows(self, max_trials: int = 10**9):
+        tr = np.zeros(self.n_walkers, dtype=np.int64)
+        en = np.zeros(self.n_walkers, dtype=np.int32)
+        check(self._L.bwl_wl_enter_windows(self._h, int(max_trials), _p(tr, C.c_int64),
+                                           _p(en, C.c_int32)))
+        return tr, en.astype(bool)
+
+    def sweeps(self, wl_f: float, average: bool = True):
+        acc = np.zeros(self.n_walkers, dtype=np.int64)
+        ms = C.c_double()
+        check(self._L.bwl_wl_sweeps(self._h, float(wl_f), int(bool(average)), _p(acc, C.c_int64),
+                                    C.byref(ms)))
+        return acc, float(ms.value)
+
+    def dos_average(self, lng: np.ndarray) -> np.ndarray:
+        """Mean of each window's ln g over its walkers, summed in walker order."""
+        out = lng.copy()
+        wpw = self.walkers_per_window
+        for w in range(self.num_windows):
+            rows = lng[w * wpw:(w + 1) * wpw]
+            s = rows[0].copy()
+            for r in rows[1:]:
+                s = s + r
+            out[w * wpw:(w + 1) * wpw] = s / float(wpw)
+        return out
+
+    def dos_combine(self, lng: np.ndarray) -> np.ndarray:
+        """Stitch the windows' ln g where their slopes agree best
+        (wang-landau.F90:1147-1194); returns the combined ln g (min 0)."""
+        wpw = self.walkers_per_window
+        comb = lng[0].copy()
+        beta_index = 0
+        for i in range(2, self.num_windows + 1):
+            buf = lng[(i - 1) * wpw]
+            start, end = int(self.windows[i - 1, 0]), int(self.windows[i - 1, 1])
+            beta_diff = np.finfo(np.float64).max
+            for j in range(0, int(self.windows[i - 2, 1]) - start):
+                bo = comb[start + j] - comb[start + j - 1]      # 1-based (s+j+1) - (s+j)
+                bm = buf[start + j] - buf[start + j - 1]
+                if abs(bo - bm) < beta_diff:
+                    beta_diff = abs(bo - bm)
+                    beta_index = start + j + 1
+            for j in range(beta_index, end + 1):
+                comb[j - 1] = buf[j - 1] + comb[beta_index - 1] - buf[beta_index - 1]
+        return comb - comb.min()
+
+    def flatness(self, hist: np.ndarray) -> np.ndarray:
+        """min(hist)/(sum(hist)/window bins) per walker (wang-landau.F90:222)."""
+        out = np.zeros(self.n_walkers)
+        for k in range(self.n_walkers):
+            w = self.window_of(k)
+            nb = int(self.windows[w, 1] - self.windows[w, 0] + 1)
+            h = hist[k, :nb]
+            s = h.sum()
+            out[k] = h.min() / (s / nb) if s > 0 else 0.0
+        return out
+
+    def run(self, *, wl_f=1.0, tolerance=1e-6, flatness=0.8, max_sweeps_calls=None,
+            enter_max_trials=10**9):
+        """wl_main's loop (see module docstring for what is left out)."""
+        _, entered = self.enter_windows(enter_max_trials)
+        if not entered.all():
+            raise RuntimeError("some walkers did not reach their energy window")
+        lng, hist = self.lng_hist()
+        self.set_lng_hist(np.zeros_like(lng), np.zeros_like(hist))
+        converged = np.zeros(self.n_walkers, dtype=bool)
+        calls = iters = 0
+        ms = 0.0
+        acc_total = 0
+        trials_per_call = self.params.mc_sweeps * self.n_atoms * self.n_walkers
+        while wl_f > tolerance:
+            if max_sweeps_calls is not None and calls >= max_sweeps_calls:
+                break
+            acc, k_ms = self.sweeps(wl_f, average=True)
+            calls += 1
+            ms += k_ms
+            acc_total += int(acc.sum())
+            lng, hist = self.lng_hist()
+            converged |= self.flatness(hist) > flatness
+            if converged.all():
+                converged[:] = False
+                wl_f = wl_f * 0.5
+                lng = self.dos_average(lng)
+                comb = self.dos_combine(lng)
+                self.set_lng_hist(np.tile(comb, (self.n_walkers, 1)), np.zeros_like(hist))
+                iters += 1
+        lng, _ = self.lng_hist()
+        return WlRunResult(lng[0].copy(), self.bin_edges.copy(), iters, calls,
+                           calls * trials_per_call, ms)
+
+
+__all__ = ["WangLandau", "WlRunResult"]

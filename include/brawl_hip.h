@@ -164,6 +164,109 @@ int bwl_metropolis_anneal_batch(int32_t n, bwl_ctx *const *ctxs,
                                 double *const *acceptance_of_T,
                                 bwl_run_stats *stats, double *makespan_ms);
 
+/* ---------------------------------------------------------------- */
+/* Radial densities (analytics.f90:205-404)                          */
+/* ---------------------------------------------------------------- */
+/* lattice_shells (analytics.f90:205-275): the first wc_range distinct
+ * REAL(4) distances (doubled-grid units) from site (1,1,1) to the occupied
+ * sites, 0 first.  wc_range 1..16. */
+int bwl_lattice_shells(bwl_ctx *ctx, int32_t wc_range, double *shells);
+/* radial_densities (analytics.f90:293-404) of the context's configuration:
+ * rho(a, b, l) = number of species-b sites at shell-l distance (minimum
+ * image, inside the reference's +-min(n,5) doubled-cell window, counted
+ * with the window's multiplicity) from species-a sites, divided by the
+ * number of species-a sites.  rho is FP64 (S, S, wc_range), Fortran
+ * layout.  Pair counts are integer on the device, so the result is
+ * bit-identical to the reference's. */
+int bwl_radial_densities(bwl_ctx *ctx, int32_t wc_range, double *rho);
+
+/* ---------------------------------------------------------------- */
+/* Nested sampling (nested_sampling.f90:45-206)                      */
+/* ---------------------------------------------------------------- */
+typedef struct {
+  int32_t n_walkers;  /* >= 2 */
+  int32_t n_steps;    /* walk length (trials per iteration before adaptation) */
+  int32_t n_iter;     /* iterations = energy limits written */
+  int32_t reserved;
+} bwl_ns_params;
+
+/* The serial nested-sampling run of the reference on the device: walkers
+ * are made by initial_setup from the context's MT19937 stream (plus one
+ * draw each for the energy jitter), then each iteration discards the
+ * highest-energy walker, clones a random one over it and walks it with
+ * lattice-wide swaps of different species accepted iff E + dE < E_limit,
+ * with the reference's walk-length adaptation.  Consumes the context's
+ * stream exactly as the reference does, so e_limits (n_iter values) are
+ * bit-identical to the reference's for the same stream.  The context's
+ * own configuration is not used or changed.  stats: trials and accepted
+ * walk moves. */
+int bwl_nested_sampling(bwl_ctx *ctx, const bwl_ns_params *p, const int64_t *species_numbers,
+                        const double *concentrations, double *e_limits, bwl_run_stats *stats);
+/* n independent runs (one wavefront each, each on its own context and
+ * stream of random numbers) in one launch; each run's result is the same
+ * as alone.  species_numbers[i] / concentrations[i] may be NULL as for
+ * bwl_initial_setup. */
+int bwl_nested_sampling_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_ns_params *params,
+                              const int64_t *const *species_numbers,
+                              const double *const *concentrations, double *const *e_limits,
+                              bwl_run_stats *stats, double *makespan_ms);
+
+/* ---------------------------------------------------------------- */
+/* Wang-Landau walkers (wang-landau.F90)                             */
+/* ---------------------------------------------------------------- */
+/* The reference's wl_params fields the walker moves read
+ * (derived_types.f90 wl_params; read in io.f90). */
+typedef struct {
+  int32_t bins;
+  int32_t num_windows;
+  int32_t walkers_per_window; /* the reference's MPI ranks per window */
+  float bin_overlap;          /* REAL(4), as read */
+  float energy_min;           /* meV/atom, REAL(4) */
+  float energy_max;
+  int32_t nbr_swap;           /* mc_select: 0 lattice-wide, 1 nearest neighbour */
+  int32_t mc_sweeps;          /* sweeps per `sweeps` call */
+  uint32_t rank_offset;       /* walker k draws from MT seed 110179 + 11*(rank_offset+k) */
+} bwl_wl_params;
+
+typedef struct bwl_wl bwl_wl;
+
+/* A set of num_windows * walkers_per_window walkers (walker k in window
+ * k / walkers_per_window) with windows and bin edges from divide_range,
+ * create_overlap and create_energy_bins (wang-landau.F90:855-992). */
+int bwl_wl_create(const bwl_lattice_desc *desc, const double *v_ex, const bwl_wl_params *p,
+                  bwl_wl **out);
+int bwl_wl_destroy(bwl_wl *wl);
+int bwl_wl_n_walkers(const bwl_wl *wl, int32_t *n_walkers);
+/* windows: (start, end) 1-based bins per window (2*num_windows); bin_edges:
+ * bins+1 energies in Ry for the whole lattice.  Either may be NULL. */
+int bwl_wl_windows(const bwl_wl *wl, int32_t *windows, double *bin_edges);
+/* Width of the per-walker histogram rows (the widest window). */
+int bwl_wl_hist_stride(const bwl_wl *wl, int32_t *stride);
+/* initial_setup for every walker from its own stream. */
+int bwl_wl_initial_setup(bwl_wl *wl, const int64_t *species_numbers, const double *concentrations);
+int bwl_wl_set_walker(bwl_wl *wl, int32_t walker, const int8_t *config);
+int bwl_wl_get_walker(bwl_wl *wl, int32_t walker, int8_t *config);
+int bwl_wl_genrand_get_state(bwl_wl *wl, int32_t walker, uint32_t *mt, int32_t *mti);
+int bwl_wl_genrand_set_state(bwl_wl *wl, int32_t walker, const uint32_t *mt, int32_t mti);
+/* ln g (walkers x bins) and visit histograms (walkers x stride), FP64.
+ * Either pointer may be NULL. */
+int bwl_wl_get_lng(bwl_wl *wl, double *lng, double *hist);
+int bwl_wl_set_lng(bwl_wl *wl, const double *lng, const double *hist);
+/* enter_energy_window (wang-landau.F90:643-741) for every walker, each on
+ * its own: biased swaps towards the middle of its window until its
+ * recounted energy is inside the window's inner 80%, re-running
+ * initial_setup every n_atoms*250 iterations as the reference does.  The
+ * reference's hand-off of the first walker's configuration to the rest of
+ * its window (MPI timing) is not made.  Stops a walker after max_trials
+ * trials; entered[k] = 1 if walker k is inside. */
+int bwl_wl_enter_windows(bwl_wl *wl, int64_t max_trials, int64_t *trials, int32_t *entered);
+/* One `sweeps` call (wang-landau.F90:539-632) for every walker at once
+ * (mc_sweeps*n_atoms trials each), then, if average != 0, each window's ln g
+ * and histograms replaced by their mean over the window's walkers, summed
+ * in walker order (the reference's MPI_Allreduce, :628-631).  accepted
+ * (optional) per walker; kernel_ms the device time. */
+int bwl_wl_sweeps(bwl_wl *wl, double wl_f, int32_t average, int64_t *accepted, double *kernel_ms);
+
 #ifdef __cplusplus
 }
 #endif
