@@ -1,0 +1,141 @@
+"""TEST INFRASTRUCTURE ONLY: the reference's own routines, compiled from the
+reference's Fortran/C sources by oracle/ref.mk into oracle/_ref/ (never
+committed).  Used by tests/ to check the C restatement (oracle.py) against
+the reference itself on inputs its golden files do not cover.
+
+available() is False when the reference build is absent (the reference
+sources are not present, or the build was not run); callers skip then.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import multiprocessing as mp
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(_HERE, "_ref")
+LIB_PATH = os.path.join(REF_DIR, "libbrawl_ref.so")
+DROP_IN = os.path.join(REF_DIR, "libdrop_in.so")
+REF_SRC = "/root/reference/src"
+_lib = None
+
+
+def build() -> bool:
+    """Build oracle/_ref when the reference sources are present; returns
+    whether the build is available afterwards."""
+    if os.path.isdir(REF_SRC):
+        subprocess.check_call(["make", "-s", "-f", os.path.join(_HERE, "ref.mk")])
+    return available()
+
+
+def available() -> bool:
+    return os.path.exists(LIB_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        i32, i8p, dp, i64p, i32p = C.c_int32, P(C.c_int8), P(C.c_double), P(C.c_int64), P(C.c_int32)
+        sig = {
+            "ref_total_energy": (C.c_double, [i32] * 6 + [dp, i8p]),
+            "ref_site_energies": (None, [i32] * 6 + [dp, i8p, dp]),
+            "ref_initial_setup": (None, [i32] * 6 + [i64p, i8p]),
+            "ref_random_sites": (None, [i32] * 6 + [i32p]),
+            "ref_radial_densities": (None, [i32] * 6 + [i8p, dp, dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _isolated(fn):
+    """Run fn in a forked child: the reference stops the whole process (a
+    Fortran STOP exits with status 0) on inputs it rejects, which must not
+    end the caller."""
+    def wrapper(*args, **kw):
+        ctx = mp.get_context("fork")
+        rd, wr = ctx.Pipe(duplex=False)
+
+        def child():
+            try:
+                wr.send(("ok", fn(*args, **kw)))
+            except BaseException as e:  # report, never hang the parent
+                wr.send(("err", repr(e)))
+            wr.close()
+
+        p = ctx.Process(target=child)
+        p.start()
+        wr.close()
+        try:
+            status, val = rd.recv()
+        except EOFError:
+            p.join()
+            raise RuntimeError(f"reference routine {fn.__name__} ended the process "
+                               f"(exit code {p.exitcode}): it rejected its input") from None
+        p.join()
+        if status != "ok":
+            raise RuntimeError(f"reference routine {fn.__name__} failed: {val}")
+        return val
+    wrapper.__name__ = fn.__name__
+    wrapper.__doc__ = fn.__doc__
+    return wrapper
+
+
+def _p(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _cfg(cfg):
+    cfg = np.asfortranarray(cfg, dtype=np.int8)
+    return cfg, _p(cfg, C.c_int8)
+
+
+@_isolated
+def total_energy(lattice, n, v_ex, cfg) -> float:
+    v = np.asfortranarray(v_ex, dtype=np.float64)
+    S, _, R = v.shape
+    c, cp = _cfg(cfg)
+    return lib().ref_total_energy(lattice, *n, S, R, _p(v, C.c_double), cp)
+
+
+@_isolated
+def site_energies(lattice, n, v_ex, cfg, n_atoms) -> np.ndarray:
+    """setup%nbr_energy of every occupied site, sites in z, y, x order."""
+    v = np.asfortranarray(v_ex, dtype=np.float64)
+    S, _, R = v.shape
+    c, cp = _cfg(cfg)
+    out = np.zeros(n_atoms)
+    lib().ref_site_energies(lattice, *n, S, R, _p(v, C.c_double), cp, _p(out, C.c_double))
+    return out
+
+
+@_isolated
+def initial_setup(lattice, n, species_numbers, rank=0) -> np.ndarray:
+    sn = np.ascontiguousarray(species_numbers, dtype=np.int64)
+    cfg = np.zeros(tuple(2 * x for x in n), dtype=np.int8, order="F")
+    lib().ref_initial_setup(lattice, *n, len(sn), rank, _p(sn, C.c_int64), _p(cfg, C.c_int8))
+    return cfg
+
+
+@_isolated
+def random_sites(lattice, n, count, rank=0) -> np.ndarray:
+    out = np.zeros((count, 4), dtype=np.int32)
+    lib().ref_random_sites(lattice, *n, rank, count, _p(out, C.c_int32))
+    return out
+
+
+@_isolated
+def radial_densities(lattice, n, S, wc, cfg):
+    c, cp = _cfg(cfg)
+    shells = np.zeros(wc)
+    rho = np.zeros((S, S, wc), order="F")
+    lib().ref_radial_densities(lattice, *n, S, wc, cp, _p(shells, C.c_double), _p(rho, C.c_double))
+    return shells, rho

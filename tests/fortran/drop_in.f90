@@ -1,0 +1,147 @@
+!> drop_in -- BraWl host code that rebinds its energy and MC-step
+!> procedure pointers to the GPU adaptors (brawl_hip_adaptors) and checks
+!> them against the reference's own routines on the same inputs.  Built as
+!> a shared library (oracle/_ref/libdrop_in.so) whose C entry point
+!> tests/test_gpu_fortran.py calls in-process:
+!>   int drop_in_run(int lattice, int n, int S, int R, int n_trials,
+!>                   const char *out_file)
+!> It writes `key value` lines to out_file and returns the first error
+!> status of the GPU library (0 if none).
+module drop_in
+
+  use iso_c_binding
+  use kinds, only : real64, int64
+  use constants, only : array_int
+  use shared_data, only : config, V_ex
+  use derived_types, only : run_params
+  use c_functions, only : f90_init_genrand
+  use initialise, only : initialise_function_pointers, initial_setup
+  use brawl_hip
+  use brawl_hip_adaptors
+  implicit none
+
+contains
+
+  function drop_in_run(lattice, n, ns, nr, n_trials, path) result(status) bind(C, name="drop_in_run")
+  integer(c_int), value :: lattice, n, ns, nr, n_trials
+  character(kind=c_char), intent(in) :: path(*)
+  integer(c_int) :: status
+  type(run_params) :: setup
+  character(len=512) :: out_file
+  integer :: i, j, k, a, b, s, u, q, mism, seed
+  integer(array_int), allocatable :: cfg_ref(:,:,:,:), cfg_gpu(:,:,:,:)
+  real(real64) :: e_ref, e_gpu, es_ref, es_gpu
+  integer :: acc
+  integer(int64) :: acc_total
+
+  if (lattice == 1) then
+    setup%lattice = 'fcc'
+  else
+    setup%lattice = 'bcc'
+  end if
+  out_file = ''
+  do i = 1, len(out_file)
+    if (path(i) == c_null_char) exit
+    out_file(i:i) = path(i)
+  end do
+  bwl_stop_on_error = .false.
+  bwl_first_error = BWL_OK
+  if (allocated(config)) deallocate(config)
+  if (allocated(V_ex)) deallocate(V_ex)
+
+  ! the reference's set-up of a static-seed run on rank 0
+  setup%n_1 = n
+  setup%n_2 = n
+  setup%n_3 = n
+  setup%n_species = ns
+  setup%interaction_range = nr
+  setup%wc_range = 2
+  setup%static_seed = .true.
+  ! species numbers first: initialise_function_pointers checks them
+  if (trim(setup%lattice) == 'fcc') then
+    setup%n_atoms = 4*n*n*n
+  else
+    setup%n_atoms = 2*n*n*n
+  end if
+  allocate(setup%species_numbers(ns), setup%species_concentrations(ns))
+  setup%species_numbers = int(setup%n_atoms / ns, int64)
+  setup%species_numbers(1) = setup%species_numbers(1) + int(setup%n_atoms - ns*(setup%n_atoms/ns), int64)
+  setup%species_concentrations = 0.0_real64
+  call initialise_function_pointers(setup)
+  allocate(config(setup%n_basis, 2*n, 2*n, 2*n))
+  config = 0_array_int
+  allocate(V_ex(ns, ns, nr))
+  ! a symmetric, zero-mean-free deterministic interaction table (Ry)
+  do s = 1, nr
+    do b = 1, ns
+      do a = 1, ns
+        V_ex(a, b, s) = 1.0e-3_real64 * sin(real(a*b + 3*s, real64)) / real(s, real64)
+      end do
+    end do
+  end do
+  seed = f90_init_genrand(0_c_int, 0_c_int, 0_c_int)
+  call initial_setup(setup, config)
+
+  open(newunit=u, file=trim(out_file), status='replace', action='write')
+
+  ! 1. the reference's energies
+  e_ref = setup%full_energy(config)
+  ! 2. rebind to the GPU and evaluate the same configuration
+  call bwl_attach(setup)
+  setup%full_energy => bwl_full_energy
+  e_gpu = setup%full_energy(config)
+  write(u, '(a,1x,es26.17e3)') 'e_ref', e_ref
+  write(u, '(a,1x,es26.17e3)') 'e_gpu', e_gpu
+
+  ! site energies through the `neighbour` pointer, both bindings
+  mism = 0
+  q = 0
+  do k = 1, 2*n
+    do j = 1, 2*n
+      do i = 1, 2*n
+        if (config(1, i, j, k) == 0_array_int) cycle
+        q = q + 1
+        if (mod(q, 7) /= 1) cycle
+        call initialise_function_pointers(setup)
+        es_ref = setup%nbr_energy(config, 1, i, j, k)
+        setup%nbr_energy => bwl_nbr_energy
+        es_gpu = setup%nbr_energy(config, 1, i, j, k)
+        if (transfer(es_ref, 0_int64) /= transfer(es_gpu, 0_int64)) mism = mism + 1
+      end do
+    end do
+  end do
+  write(u, '(a,1x,i0)') 'site_energy_checked', (q + 6) / 7
+  write(u, '(a,1x,i0)') 'site_energy_mismatches', mism
+
+  ! 3. initial_setup from the device stream seeded as the reference's
+  allocate(cfg_ref(size(config,1), size(config,2), size(config,3), size(config,4)))
+  allocate(cfg_gpu(size(config,1), size(config,2), size(config,3), size(config,4)))
+  cfg_ref = config
+  call bwl_seed_stream(0)
+  call bwl_initial_setup_gpu(setup, cfg_gpu)
+  write(u, '(a,1x,i0)') 'initial_setup_mismatches', count(cfg_ref /= cfg_gpu)
+
+  ! 4. MC steps through the rebound `monte_carlo` pointer, continuing the
+  ! stream; the test replays the same trials on the oracle
+  setup%mc_step => bwl_mc_step_lattice
+  acc_total = 0
+  do q = 1, n_trials
+    acc = setup%mc_step(cfg_gpu, 1.0_real64 / (600.0_real64 * 8.167333262e-5_real64 / 13.605693122990_real64))
+    acc_total = acc_total + acc
+  end do
+  write(u, '(a,1x,i0)') 'mc_accepted', acc_total
+  write(u, '(a)', advance='no') 'mc_config'
+  do k = 1, 2*n
+    do j = 1, 2*n
+      do i = 1, 2*n
+        write(u, '(1x,i0)', advance='no') cfg_gpu(1, i, j, k)
+      end do
+    end do
+  end do
+  write(u, *)
+  close(u)
+  call bwl_detach()
+  status = bwl_first_error
+  end function drop_in_run
+
+end module drop_in

@@ -1,0 +1,90 @@
+"""The oracle (C restatement) against the reference's own Fortran routines,
+compiled here from the reference's sources (oracle/ref.mk).  Covers inputs
+the golden files do not: every shell count of both lattices (bcc shells
+7-10, fcc 5-6), non-cubic boxes, more species, other ranks' streams and
+wider radial-density ranges.  CPU only; skipped when the reference build
+is absent."""
+import numpy as np
+import pytest
+
+from oracle import ref as R
+
+pytestmark = pytest.mark.skipif(not R.available(), reason="reference build (oracle/_ref) absent")
+
+SHAPES = [
+    # lattice, n, S, shells
+    (1, (2, 2, 2), 3, 6),
+    (1, (3, 2, 4), 4, 5),
+    (1, (4, 4, 4), 2, 1),
+    (2, (3, 3, 3), 3, 10),
+    (2, (4, 3, 5), 5, 9),
+    (2, (2, 2, 2), 2, 7),
+    (2, (5, 5, 5), 4, 8),
+    (1, (5, 4, 3), 8, 4),
+]
+
+
+def _config(orc, lat, n, S, seed):
+    from brawl_amd import workloads
+    v = workloads.random_v(S, [20.0 / k for k in range(1, 11)], seed)
+    o = orc.OracleLattice(lat, n, v[:, :, :1])
+    N = o.n_atoms
+    rng = np.random.default_rng(seed)
+    return workloads.dirichlet_counts(N, S, rng), N
+
+
+@pytest.mark.parametrize("lat,n,S,Rs", SHAPES)
+def test_energies_bit_exact_against_reference(oracle, lat, n, S, Rs):
+    from brawl_amd import workloads
+    v = workloads.random_v(S, [25.0 / k for k in range(1, Rs + 1)], 11 * Rs + S)
+    o = oracle.OracleLattice(lat, n, v)
+    N = o.n_atoms
+    counts = workloads.dirichlet_counts(N, S, np.random.default_rng(N + S))
+    cfg = o.initial_setup(oracle.Generator(110179 + 7), species_numbers=counts)
+    assert o.total_energy(cfg) == R.total_energy(lat, n, v, cfg)
+    ref_sites = R.site_energies(lat, n, v, cfg, N)
+    ours = []
+    for k in range(1, 2 * n[2] + 1):
+        for j in range(1, 2 * n[1] + 1):
+            for i in range(1, 2 * n[0] + 1):
+                if cfg[i - 1, j - 1, k - 1]:
+                    ours.append(o.site_energy(cfg, i, j, k))
+    assert np.array_equal(np.array(ours), ref_sites)
+
+
+@pytest.mark.parametrize("lat,n,S,Rs", SHAPES)
+@pytest.mark.parametrize("rank", [0, 3])
+def test_initial_setup_bit_exact_against_reference(oracle, lat, n, S, Rs, rank):
+    from brawl_amd import workloads
+    o = oracle.OracleLattice(lat, n, np.zeros((S, S, 1)))
+    N = o.n_atoms
+    counts = workloads.dirichlet_counts(N, S, np.random.default_rng(N * (rank + 1)))
+    cfg = o.initial_setup(oracle.Generator(rank=rank), species_numbers=counts)
+    assert np.array_equal(cfg, R.initial_setup(lat, n, counts, rank))
+
+
+@pytest.mark.parametrize("lat,n", [(1, (3, 4, 5)), (2, (4, 2, 3)), (1, (8, 8, 8)), (2, (16, 16, 16))])
+def test_random_sites_against_reference(oracle, lat, n):
+    o = oracle.OracleLattice(lat, n, np.zeros((1, 1, 1)))
+    gen = oracle.Generator(rank=2)
+    ours = np.array([o.random_site(gen) for _ in range(2000)], dtype=np.int32)
+    ref = R.random_sites(lat, n, 2000, rank=2)
+    assert np.all(ref[:, 0] == 1)
+    assert np.array_equal(ours, ref[:, 1:])
+
+
+@pytest.mark.parametrize("lat,n,S,wc", [(1, (4, 4, 4), 3, 3), (1, (6, 6, 6), 5, 6), (2, (8, 8, 8), 4, 8),
+                                        (2, (3, 5, 4), 2, 5), (1, (2, 2, 2), 2, 2), (2, (6, 6, 6), 3, 12)])
+def test_radial_densities_bit_exact_against_reference(oracle, lat, n, S, wc):
+    from brawl_amd import workloads
+    o = oracle.OracleLattice(lat, n, np.zeros((S, S, 1)))
+    N = o.n_atoms
+    counts = workloads.dirichlet_counts(N, S, np.random.default_rng(5 * N + wc))
+    cfg = o.initial_setup(oracle.Generator(110179 + wc), species_numbers=counts)
+    shells_ref, rho_ref = R.radial_densities(lat, n, S, wc, cfg)
+    shells = o.lattice_shells(cfg, wc)
+    assert np.array_equal(shells, shells_ref)
+    rho = o.radial_densities(cfg, wc, shells)
+    fin = np.isfinite(rho_ref)
+    assert np.array_equal(np.isfinite(rho), fin)
+    assert np.array_equal(rho[fin], rho_ref[fin])
