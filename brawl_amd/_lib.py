@@ -31,7 +31,9 @@ EXPORTS = (
     "bwl_wl_n_walkers", "bwl_wl_windows", "bwl_wl_hist_stride", "bwl_wl_initial_setup",
     "bwl_wl_set_walker", "bwl_wl_get_walker", "bwl_wl_genrand_get_state",
     "bwl_wl_genrand_set_state", "bwl_wl_get_lng", "bwl_wl_set_lng", "bwl_wl_enter_windows",
-    "bwl_wl_sweeps",
+    "bwl_wl_sweeps", "bwl_server_create", "bwl_server_destroy", "bwl_server_submit_anneal",
+    "bwl_server_submit_ns", "bwl_server_submit_wl", "bwl_server_progress", "bwl_server_poll",
+    "bwl_server_info", "bwl_server_drain", "bwl_server_memory",
 )
 
 
@@ -60,6 +62,17 @@ class WlParams(C.Structure):
                 ("walkers_per_window", C.c_int32), ("bin_overlap", C.c_float),
                 ("energy_min", C.c_float), ("energy_max", C.c_float), ("nbr_swap", C.c_int32),
                 ("mc_sweeps", C.c_int32), ("rank_offset", C.c_uint32)]
+
+
+class ServerParams(C.Structure):
+    _fields_ = [("device_bytes", C.c_int64), ("max_launches", C.c_int32), ("reserved", C.c_int32)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("state", C.c_int32), ("error", C.c_int32),
+                ("reserved", C.c_int32), ("trials", C.c_int64), ("accepted", C.c_int64),
+                ("t_submit_ms", C.c_double), ("t_admit_ms", C.c_double), ("t_done_ms", C.c_double),
+                ("kernel_ms", C.c_double)]
 
 
 class BrawlError(RuntimeError):
@@ -122,6 +135,19 @@ def load():
         "bwl_wl_set_lng": (C.c_int, [vp, dp, dp]),
         "bwl_wl_enter_windows": (C.c_int, [vp, C.c_int64, i64p, i32p]),
         "bwl_wl_sweeps": (C.c_int, [vp, C.c_double, C.c_int32, i64p, dp]),
+        "bwl_server_create": (C.c_int, [P(ServerParams), P(vp)]),
+        "bwl_server_destroy": (C.c_int, [vp]),
+        "bwl_server_submit_anneal": (C.c_int, [vp, P(LatticeDesc), dp, i8p, P(AnnealParams), dp, dp,
+                                               dp, i8p, i64p]),
+        "bwl_server_submit_ns": (C.c_int, [vp, P(LatticeDesc), dp, C.c_uint32, P(NsParams), i64p, dp,
+                                           dp, i64p]),
+        "bwl_server_submit_wl": (C.c_int, [vp, P(LatticeDesc), dp, P(WlParams), i64p, dp, C.c_double,
+                                           dp, dp, i64p]),
+        "bwl_server_progress": (C.c_int, [vp, i32p, i32p]),
+        "bwl_server_poll": (C.c_int, [vp, i64p, C.c_int32, i32p]),
+        "bwl_server_info": (C.c_int, [vp, C.c_int64, P(RequestInfo)]),
+        "bwl_server_drain": (C.c_int, [vp]),
+        "bwl_server_memory": (C.c_int, [vp, i64p, i64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -14,10 +14,6 @@ namespace bwl {
 thread_local std::string g_err;
 }
 
-namespace {
-int c0dev(bwl_ctx *const *ctxs) { return ctxs[0]->device; }
-}  // namespace
-
 extern "C" {
 
 const char *bwl_last_error(void) { return g_err.c_str(); }
@@ -74,15 +70,17 @@ int bwl_create(const bwl_lattice_desc *desc, const double *v_ex, bwl_ctx **out) 
 int bwl_destroy(bwl_ctx *c) {
   if (!c) return BWL_OK;
   (void)hipSetDevice(c->device);
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
-  (void)hipFree(c->d_compact);
-  (void)hipFree(c->d_ref);
-  (void)hipFree(c->d_off);
-  (void)hipFree(c->d_vex);
-  (void)hipFree(c->d_scratch);
+  if (c->stream && !c->borrowed) (void)hipStreamSynchronize(c->stream);
+  if (!c->borrowed) {
+    (void)hipFree(c->d_compact);
+    (void)hipFree(c->d_ref);
+    (void)hipFree(c->d_off);
+    (void)hipFree(c->d_vex);
+  }
+  if (c->d_scratch) (void)hipFree(c->d_scratch);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream && !c->borrowed) (void)hipStreamDestroy(c->stream);
   delete c;
   return BWL_OK;
 }
@@ -294,23 +292,102 @@ int bwl_metropolis_anneal(bwl_ctx *c, const bwl_anneal_params *p, double *energi
   return bwl_metropolis_anneal_batch(1, cs, p, e, cv, ac, stats, nullptr);
 }
 
-int bwl_metropolis_anneal_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_anneal_params *params,
-                                double *const *energies_of_T, double *const *C_of_T,
-                                double *const *acceptance_of_T, bwl_run_stats *stats,
-                                double *makespan_ms) {
-  if (n < 1 || !ctxs || !params || !energies_of_T || !C_of_T || !acceptance_of_T)
-    return fail(BWL_ERR_ARG, "null argument");
-  struct Item {
-    int idx;
-    const void *fn;
-    int threads;
-    size_t lds;
-    ParArgs a;
-    ParPlan plan;
-    size_t off_muls, off_out, off_cnt;
-    double work;
-  };
-  std::vector<Item> items((size_t)n);
+}  // extern "C"
+
+namespace bwl {
+
+size_t ctx_device_bytes(const bwl_lattice_desc &d) {
+  Geo g;
+  std::vector<int32_t> off;
+  std::string err;
+  if (make_geo(d, g, off, err)) return 0;
+  const size_t ref = (size_t)g.m[0] * g.m[1] * g.m[2];
+  auto a = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  return a((size_t)g.compact_bytes) + a(ref) + a(sizeof(int32_t) * off.size()) +
+         a(sizeof(double) * (size_t)g.S * g.S * g.R);
+}
+
+int ctx_create_borrowed(const bwl_lattice_desc &d, const double *v_ex, char *mem, size_t bytes,
+                        hipStream_t stream, bwl_ctx **out) {
+  *out = nullptr;
+  bwl_ctx *c = new bwl_ctx();
+  c->desc = d;
+  int rc = make_geo(d, c->g, c->off, g_err);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  if (bytes < ctx_device_bytes(d)) {
+    delete c;
+    return fail(BWL_ERR_ARG, "borrowed block too small for the lattice");
+  }
+  const Geo &g = c->g;
+  c->vex.assign(v_ex, v_ex + (size_t)g.S * g.S * g.R);
+  c->borrowed = true;
+  if (hipGetDevice(&c->device) != hipSuccess) c->device = 0;
+  auto a = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  char *q = mem;
+  c->d_compact = (int8_t *)q;
+  q += a((size_t)g.compact_bytes);
+  c->d_ref = (int8_t *)q;
+  q += a((size_t)g.m[0] * g.m[1] * g.m[2]);
+  c->d_off = (int32_t *)q;
+  q += a(sizeof(int32_t) * c->off.size());
+  c->d_vex = (double *)q;
+  c->stream = stream;
+  bool ok = hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess &&
+            hipMemcpyAsync(c->d_off, c->off.data(), sizeof(int32_t) * c->off.size(),
+                           hipMemcpyHostToDevice, stream) == hipSuccess &&
+            hipMemcpyAsync(c->d_vex, c->vex.data(), sizeof(double) * c->vex.size(),
+                           hipMemcpyHostToDevice, stream) == hipSuccess;
+  if (!ok) {
+    bwl_destroy(c);
+    return fail(BWL_ERR_DEVICE, "borrowed context set-up failed");
+  }
+  c->mt.seed(110179u);
+  *out = c;
+  return BWL_OK;
+}
+
+}  // namespace bwl
+
+// ----------------------------------------------------------------------
+// Batched anneal in three steps (used by bwl_metropolis_anneal_batch and by
+// the request server): plan on the host, upload + launch on streams, collect
+// after completion.
+// ----------------------------------------------------------------------
+namespace bwl {
+
+int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int &threads,
+                  size_t &lds) {
+  const int lim = lds_limit();
+  int rc = plan_parallel(g, plan, g_err);
+  if (rc) return rc;
+  int max_pairs = 0;
+  for (const RoundVar &v : plan.vars) max_pairs = v.n2 > max_pairs ? v.n2 : max_pairs;
+  // fewest passes over a round's pairs, then the fewest threads for that
+  const int passes = (max_pairs + 511) / 512;
+  threads = (((max_pairs + passes - 1) / passes + 63) / 64) * 64;
+  if (threads > 512) threads = 512;
+  if (threads < 64) threads = 64;
+  bool use_nat = make_nat(g, nat);
+  lds = par_lds_bytes(g, use_nat ? &nat : nullptr, threads);
+  if (use_nat && lds > (size_t)lim) {
+    use_nat = false;
+    lds = par_lds_bytes(g, nullptr, threads);
+  }
+  if (lds > (size_t)lim) return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
+  // packed counts: the natural kernel uses 8-bit fields (u32 holds 4
+  // species), the per-sublattice kernel 6-bit fields (u32 holds 5)
+  const bool wide = use_nat ? g.S > 4 : g.S > 5;
+  fn = par_kernel_for(g.lattice, g.R, g.n[0], use_nat, wide);
+  if (!fn) return fail(BWL_ERR_UNSUPPORTED, "no kernel for this lattice");
+  return BWL_OK;
+}
+
+int anneal_plan(int n, bwl_ctx *const *ctxs, const bwl_anneal_params *params, AnnealPlan &P) {
+  P.items.assign((size_t)n, AnnealPlan::Item());
+  P.groups.clear();
   size_t total = 0;
   const int lim = lds_limit();
   for (int i = 0; i < n; ++i) {
@@ -320,34 +397,16 @@ int bwl_metropolis_anneal_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_annea
     if (i > 0 && c->device != ctxs[0]->device)
       return fail(BWL_ERR_ARG, "all chains of a batch must be on one device");
     const bwl_anneal_params *p = &params[i];
-    if (!energies_of_T[i] || !C_of_T[i] || !acceptance_of_T[i]) return fail(BWL_ERR_ARG, "null output");
     if (p->T_steps < 1 || p->n_sweeps < 1 || p->burn_in_sweeps < 0)
       return fail(BWL_ERR_ARG, "T_steps and n_sweeps must be positive");
-    Item &it = items[(size_t)i];
+    AnnealPlan::Item &it = P.items[(size_t)i];
     it.idx = i;
     const Geo &g = c->g;
-    rc = plan_parallel(g, it.plan, g_err);
-    if (rc) return rc;
-    int max_pairs = 0;
-    for (const RoundVar &v : it.plan.vars) max_pairs = v.n2 > max_pairs ? v.n2 : max_pairs;
-    // fewest passes over a round's pairs, then the fewest threads for that
-    const int passes = (max_pairs + 511) / 512;
-    int threads = (((max_pairs + passes - 1) / passes + 63) / 64) * 64;
-    if (threads > 512) threads = 512;
-    if (threads < 64) threads = 64;
     NatGeo nat;
-    bool use_nat = make_nat(g, nat);
-    size_t lds = par_lds_bytes(g, use_nat ? &nat : nullptr, threads);
-    if (use_nat && lds > (size_t)lim) {
-      use_nat = false;
-      lds = par_lds_bytes(g, nullptr, threads);
-    }
-    if (lds > (size_t)lim) return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
-    // packed counts: the natural kernel uses 8-bit fields (u32 holds 4
-    // species), the per-sublattice kernel 6-bit fields (u32 holds 5)
-    const bool wide = use_nat ? g.S > 4 : g.S > 5;
-    it.fn = par_kernel_for(g.lattice, g.R, g.n[0], use_nat, wide);
-    if (!it.fn) return fail(BWL_ERR_UNSUPPORTED, "no kernel for this lattice");
+    int threads = 0;
+    size_t lds = 0;
+    rc = anneal_kernel(g, it.plan, nat, it.fn, threads, lds);
+    if (rc) return rc;
     it.threads = threads;
     it.lds = lds;
     ParArgs &a = it.a;
@@ -379,11 +438,11 @@ int bwl_metropolis_anneal_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_annea
     a.chain_id = 0;  // the stream depends on the seed and position only, not on the batch
     a.phase0 = c->phase;
     a.e_trace = nullptr;
-    const size_t T = (size_t)p->T_steps;
+    it.T = (size_t)p->T_steps;
     it.off_muls = total;
     total += 256 * ((it.plan.muls.size() * 4 + 255) / 256);
     it.off_out = total;
-    total += 256 * ((3 * T * sizeof(double) + 255) / 256);
+    total += 256 * ((3 * it.T * sizeof(double) + 255) / 256);
     it.off_cnt = total;
     total += 256;
     it.work = (double)g.n_atoms * (double)p->T_steps * (double)(p->n_sweeps + p->burn_in_sweeps);
@@ -395,30 +454,22 @@ int bwl_metropolis_anneal_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_annea
   std::vector<int> order((size_t)n);
   for (int i = 0; i < n; ++i) order[(size_t)i] = i;
   std::sort(order.begin(), order.end(), [&](int x, int y) {
-    const Item &A = items[(size_t)x], &B = items[(size_t)y];
+    const AnnealPlan::Item &A = P.items[(size_t)x], &B = P.items[(size_t)y];
     if (A.fn != B.fn) return A.fn < B.fn;
     return A.work > B.work;
   });
-  struct Group {
-    const void *fn;
-    int threads;
-    size_t lds;
-    std::vector<int> members;
-    size_t off_args;
-  };
-  std::vector<Group> groups;
   for (int i : order) {
-    const Item &it = items[(size_t)i];
-    if (groups.empty() || groups.back().fn != it.fn)
-      groups.push_back(Group{it.fn, it.threads, 0, {}, 0});
-    Group &gr = groups.back();
+    const AnnealPlan::Item &it = P.items[(size_t)i];
+    if (P.groups.empty() || P.groups.back().fn != it.fn)
+      P.groups.push_back(AnnealPlan::Group{it.fn, it.threads, 0, {}, 0});
+    AnnealPlan::Group &gr = P.groups.back();
     gr.members.push_back(i);
     if (it.threads > gr.threads) gr.threads = it.threads;
   }
   // LDS per launch: the largest member's need at the group's block size
-  for (Group &gr : groups)
+  for (AnnealPlan::Group &gr : P.groups)
     for (int i : gr.members) {
-      const Item &it = items[(size_t)i];
+      const AnnealPlan::Item &it = P.items[(size_t)i];
       const size_t need = it.lds + sizeof(double) * (size_t)(gr.threads - it.threads) / 64;
       if (need > gr.lds) gr.lds = need;
     }
@@ -426,66 +477,117 @@ int bwl_metropolis_anneal_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_annea
   // own: ask each block for more than half the LDS so no two share a CU
   // (the largest chains set the makespan; sharing a CU would halve them).
   int n_cu = 0;
-  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c0dev(ctxs)) != hipSuccess)
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctxs[0]->device) !=
+      hipSuccess)
     n_cu = 0;
   if (n_cu > 0 && n <= n_cu) {
     const size_t exclusive = (size_t)lim / 2 + 1024;
-    for (Group &gr : groups)
+    for (AnnealPlan::Group &gr : P.groups)
       if (gr.lds < exclusive && exclusive <= (size_t)lim) gr.lds = exclusive;
   }
-  for (Group &gr : groups) {
+  for (AnnealPlan::Group &gr : P.groups) {
     gr.off_args = total;
     total += 256 * ((sizeof(ParArgs) * gr.members.size() + 255) / 256);
   }
-  bwl_ctx *c0 = ctxs[0];
-  void *scratch = nullptr;
-  if (hipMalloc(&scratch, total) != hipSuccess) return fail(BWL_ERR_NOMEM, "device allocation failed");
-  char *base = (char *)scratch;
-  struct Freer {
-    void *p;
-    ~Freer() { (void)hipFree(p); }
-  } freer{scratch};
-  std::vector<ParArgs> host_args;
-  for (int i = 0; i < n; ++i) {
-    Item &it = items[(size_t)i];
-    const size_t T = (size_t)params[i].T_steps;
+  P.scratch_bytes = total;
+  return BWL_OK;
+}
+
+int anneal_upload(AnnealPlan &P, char *base, hipStream_t s) {
+  P.base = base;
+  P.host_args.clear();
+  for (AnnealPlan::Item &it : P.items) {
     it.a.muls = (const uint32_t *)(base + it.off_muls);
     it.a.energies_of_T = (double *)(base + it.off_out);
-    it.a.C_of_T = it.a.energies_of_T + T;
-    it.a.acc_of_T = it.a.C_of_T + T;
+    it.a.C_of_T = it.a.energies_of_T + it.T;
+    it.a.acc_of_T = it.a.C_of_T + it.T;
     it.a.counters = (long long *)(base + it.off_cnt);
     HIP_OK(hipMemcpyAsync(base + it.off_muls, it.plan.muls.data(), it.plan.muls.size() * 4,
-                          hipMemcpyHostToDevice, c0->stream));
+                          hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemsetAsync(it.a.counters, 0, 3 * sizeof(long long), s));
   }
-  for (Group &gr : groups) {
-    host_args.clear();
-    for (int i : gr.members) host_args.push_back(items[(size_t)i].a);
-    HIP_OK(hipMemcpyAsync(base + gr.off_args, host_args.data(), sizeof(ParArgs) * host_args.size(),
-                          hipMemcpyHostToDevice, c0->stream));
+  for (AnnealPlan::Group &gr : P.groups) {
+    std::vector<ParArgs> host;
+    for (int i : gr.members) host.push_back(P.items[(size_t)i].a);
+    P.host_args.push_back(std::move(host));
+    HIP_OK(hipMemcpyAsync(base + gr.off_args, P.host_args.back().data(),
+                          sizeof(ParArgs) * P.host_args.back().size(), hipMemcpyHostToDevice, s));
   }
-  for (int i = 0; i < n; ++i) {
-    long long zero[3] = {0, 0, 0};
-    HIP_OK(hipMemcpyAsync(items[(size_t)i].a.counters, zero, sizeof(zero), hipMemcpyHostToDevice,
-                          c0->stream));
+  return BWL_OK;
+}
+
+int anneal_launch(const AnnealPlan &P, const hipStream_t *streams) {
+  for (size_t k = 0; k < P.groups.size(); ++k) {
+    const AnnealPlan::Group &gr = P.groups[k];
+    HIP_OK(launch_metro_par_fn(gr.fn, (const ParArgs *)(P.base + gr.off_args),
+                               (int)gr.members.size(), gr.lds, gr.threads, streams[k]));
   }
-  // launch every group on its own stream so groups overlap on the device
-  std::vector<hipStream_t> streams;
-  for (size_t k = 0; k < groups.size(); ++k) {
-    hipStream_t s = nullptr;
-    if (k == 0) {
-      s = c0->stream;
-    } else if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
-      for (hipStream_t t : streams) (void)hipStreamDestroy(t);
-      return fail(BWL_ERR_DEVICE, "stream creation failed");
+  return BWL_OK;
+}
+
+int anneal_collect(const AnnealPlan &P, bwl_ctx *const *ctxs, double *const *energies_of_T,
+                   double *const *C_of_T, double *const *acceptance_of_T, bwl_run_stats *stats,
+                   double kernel_ms, hipStream_t s) {
+  const size_t n = P.items.size();
+  std::vector<std::vector<double>> out(n);
+  std::vector<long long> cnt(3 * n);
+  for (size_t i = 0; i < n; ++i) {
+    const AnnealPlan::Item &it = P.items[i];
+    out[i].resize(3 * it.T);
+    HIP_OK(hipMemcpyAsync(out[i].data(), it.a.energies_of_T, 3 * it.T * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&cnt[3 * i], it.a.counters, 3 * sizeof(long long), hipMemcpyDeviceToHost, s));
+  }
+  HIP_OK(hipStreamSynchronize(s));
+  for (size_t i = 0; i < n; ++i) {
+    const size_t T = P.items[i].T;
+    if (energies_of_T && energies_of_T[i]) memcpy(energies_of_T[i], out[i].data(), T * sizeof(double));
+    if (C_of_T && C_of_T[i]) memcpy(C_of_T[i], out[i].data() + T, T * sizeof(double));
+    if (acceptance_of_T && acceptance_of_T[i])
+      memcpy(acceptance_of_T[i], out[i].data() + 2 * T, T * sizeof(double));
+    if (stats) {
+      stats[i].trials = cnt[3 * i + 2];
+      stats[i].accepted = cnt[3 * i + 1];
+      stats[i].kernel_ms = kernel_ms;
     }
-    if (k > 0) streams.push_back(s);
+    ctxs[i]->phase = (unsigned long long)cnt[3 * i];
   }
+  return BWL_OK;
+}
+
+}  // namespace bwl
+
+extern "C" {
+
+int bwl_metropolis_anneal_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_anneal_params *params,
+                                double *const *energies_of_T, double *const *C_of_T,
+                                double *const *acceptance_of_T, bwl_run_stats *stats,
+                                double *makespan_ms) {
+  if (n < 1 || !ctxs || !params || !energies_of_T || !C_of_T || !acceptance_of_T)
+    return fail(BWL_ERR_ARG, "null argument");
+  for (int i = 0; i < n; ++i)
+    if (!energies_of_T[i] || !C_of_T[i] || !acceptance_of_T[i]) return fail(BWL_ERR_ARG, "null output");
+  AnnealPlan P;
+  int rc = anneal_plan(n, ctxs, params, P);
+  if (rc) return rc;
+  bwl_ctx *c0 = ctxs[0];
+  DevBuf scratch;
+  if (hipMalloc(&scratch.p, P.scratch_bytes) != hipSuccess)
+    return fail(BWL_ERR_NOMEM, "device allocation failed");
+  // every group on its own stream so groups overlap on the device
+  std::vector<hipStream_t> streams;
   struct StreamFreer {
     std::vector<hipStream_t> *v;
     ~StreamFreer() {
-      for (hipStream_t t : *v) (void)hipStreamDestroy(t);
+      for (size_t k = 1; k < v->size(); ++k) (void)hipStreamDestroy((*v)[k]);
     }
   } sfree{&streams};
+  streams.push_back(c0->stream);
+  for (size_t k = 1; k < P.groups.size(); ++k) {
+    hipStream_t s = nullptr;
+    HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    streams.push_back(s);
+  }
   hipEvent_t ready = nullptr;
   HIP_OK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
   struct EventFreer {
@@ -493,18 +595,16 @@ int bwl_metropolis_anneal_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_annea
     ~EventFreer() { (void)hipEventDestroy(e); }
   } efree{ready};
   HIP_OK(hipEventRecord(c0->ev0, c0->stream));
+  rc = anneal_upload(P, (char *)scratch.p, c0->stream);
+  if (rc) return rc;
   HIP_OK(hipEventRecord(ready, c0->stream));
-  for (size_t k = 0; k < groups.size(); ++k) {
-    hipStream_t s = k == 0 ? c0->stream : streams[k - 1];
-    if (k > 0) HIP_OK(hipStreamWaitEvent(s, ready, 0));
-    const Group &gr = groups[k];
-    HIP_OK(launch_metro_par_fn(gr.fn, (const ParArgs *)(base + gr.off_args), (int)gr.members.size(),
-                               gr.lds, gr.threads, s));
-  }
-  for (size_t k = 1; k < groups.size(); ++k) {
+  for (size_t k = 1; k < streams.size(); ++k) HIP_OK(hipStreamWaitEvent(streams[k], ready, 0));
+  rc = anneal_launch(P, streams.data());
+  if (rc) return rc;
+  for (size_t k = 1; k < streams.size(); ++k) {
     hipEvent_t done = nullptr;
     HIP_OK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
-    HIP_OK(hipEventRecord(done, streams[k - 1]));
+    HIP_OK(hipEventRecord(done, streams[k]));
     HIP_OK(hipStreamWaitEvent(c0->stream, done, 0));
     (void)hipEventDestroy(done);
   }
@@ -513,24 +613,7 @@ int bwl_metropolis_anneal_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_annea
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, c0->ev0, c0->ev1);
   if (makespan_ms) *makespan_ms = ms;
-  for (int i = 0; i < n; ++i) {
-    const Item &it = items[(size_t)i];
-    const size_t T = (size_t)params[i].T_steps;
-    std::vector<double> out(3 * T);
-    long long cnt[3];
-    HIP_OK(hipMemcpy(out.data(), it.a.energies_of_T, 3 * T * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(cnt, it.a.counters, sizeof(cnt), hipMemcpyDeviceToHost));
-    memcpy(energies_of_T[i], out.data(), T * sizeof(double));
-    memcpy(C_of_T[i], out.data() + T, T * sizeof(double));
-    memcpy(acceptance_of_T[i], out.data() + 2 * T, T * sizeof(double));
-    if (stats) {
-      stats[i].trials = cnt[2];
-      stats[i].accepted = cnt[1];
-      stats[i].kernel_ms = ms;
-    }
-    ctxs[i]->phase = (unsigned long long)cnt[0];
-  }
-  return BWL_OK;
+  return anneal_collect(P, ctxs, energies_of_T, C_of_T, acceptance_of_T, stats, ms, c0->stream);
 }
 
 }  // extern "C"

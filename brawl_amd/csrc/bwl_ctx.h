@@ -46,6 +46,7 @@ struct bwl_ctx {
   bwl::MT19937 mt;
   unsigned long long phase = 0;  // parallel-sampler round counter
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool borrowed = false;  // device memory and stream belong to someone else
 };
 
 namespace bwl {
@@ -76,6 +77,51 @@ inline int lds_limit() {
     return v;
   return kLdsBytes;
 }
+
+// A context whose device buffers live in caller-provided memory (`bytes` >=
+// ctx_device_bytes) and whose uploads go on `stream`; bwl_destroy frees
+// neither.
+size_t ctx_device_bytes(const bwl_lattice_desc &d);
+int ctx_create_borrowed(const bwl_lattice_desc &d, const double *v_ex, char *mem, size_t bytes,
+                        hipStream_t stream, bwl_ctx **out);
+
+// A batch of anneals prepared on the host (anneal_plan), its arguments and
+// buffers placed in caller-provided device memory (anneal_upload), its
+// launches enqueued one group per stream (anneal_launch), and its outputs
+// copied out after completion (anneal_collect).
+struct AnnealPlan {
+  struct Item {
+    int idx = 0;
+    const void *fn = nullptr;
+    int threads = 0;
+    size_t lds = 0;
+    ParArgs a;
+    ParPlan plan;
+    size_t T = 0, off_muls = 0, off_out = 0, off_cnt = 0;
+    double work = 0.0;
+  };
+  struct Group {
+    const void *fn;
+    int threads;
+    size_t lds;
+    std::vector<int> members;
+    size_t off_args;
+  };
+  std::vector<Item> items;
+  std::vector<Group> groups;
+  std::vector<std::vector<ParArgs>> host_args;  // kept alive until the uploads finish
+  size_t scratch_bytes = 0;
+  char *base = nullptr;
+};
+// The parallel-sampler kernel variant, block size and LDS for a lattice.
+int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int &threads,
+                  size_t &lds);
+int anneal_plan(int n, bwl_ctx *const *ctxs, const bwl_anneal_params *params, AnnealPlan &P);
+int anneal_upload(AnnealPlan &P, char *scratch_dev, hipStream_t s);
+int anneal_launch(const AnnealPlan &P, const hipStream_t *streams);
+int anneal_collect(const AnnealPlan &P, bwl_ctx *const *ctxs, double *const *energies_of_T,
+                   double *const *C_of_T, double *const *acceptance_of_T, bwl_run_stats *stats,
+                   double kernel_ms, hipStream_t s);
 
 // Device buffer freed at scope exit.
 struct DevBuf {

@@ -267,6 +267,72 @@ int bwl_wl_enter_windows(bwl_wl *wl, int64_t max_trials, int64_t *trials, int32_
  * (optional) per walker; kernel_ms the device time. */
 int bwl_wl_sweeps(bwl_wl *wl, double wl_f, int32_t average, int64_t *accepted, double *kernel_ms);
 
+/* ---------------------------------------------------------------- */
+/* Request server (continuous batching on one device)                */
+/* ---------------------------------------------------------------- */
+/* No counterpart in the reference (there, one MPI rank runs one chain).
+ * Requests are submitted with host inputs and queue on the host;
+ * bwl_server_progress retires finished launches (results written to the
+ * callers' buffers, device memory returned) and admits queued requests in
+ * arrival order -- skipping ahead past any that do not fit -- while the
+ * device-memory budget and launch slots allow.  Requests admitted together
+ * are grouped by kernel variant, one launch per group on its own stream, so
+ * work admitted at different times overlaps on the device.  Each request
+ * keeps its own random stream, so its result is the same as when run alone
+ * (bwl_metropolis_anneal on a fresh context with the same seed, or
+ * bwl_nested_sampling after bwl_genrand_seed(seed)).  Output buffers must
+ * stay valid until the request is done.  One host thread per server. */
+typedef struct {
+  int64_t device_bytes;  /* admission budget: one device arena of this size */
+  int32_t max_launches;  /* launch slots (streams), 1..64 */
+  int32_t reserved;
+} bwl_server_params;
+
+typedef struct {
+  int32_t kind;          /* 1 anneal, 2 nested sampling, 3 Wang-Landau */
+  int32_t state;         /* 0 queued, 1 running, 2 done, 3 failed */
+  int32_t error;         /* status of a failed request */
+  int32_t reserved;
+  int64_t trials, accepted;
+  double t_submit_ms, t_admit_ms, t_done_ms;  /* server clock */
+  double kernel_ms;      /* device time of the launch the request ran in */
+} bwl_request_info;
+
+typedef struct bwl_server bwl_server;
+
+int bwl_server_create(const bwl_server_params *p, bwl_server **out);
+int bwl_server_destroy(bwl_server *s);
+/* A Metropolis anneal (as bwl_metropolis_anneal) of `config` (reference
+ * layout); config_out (optional) receives the final configuration. */
+int bwl_server_submit_anneal(bwl_server *s, const bwl_lattice_desc *desc, const double *v_ex,
+                             const int8_t *config, const bwl_anneal_params *p,
+                             double *energies_of_T, double *C_of_T, double *acceptance_of_T,
+                             int8_t *config_out, int64_t *id);
+/* A nested-sampling run (as bwl_nested_sampling) from MT19937 seed `seed`;
+ * its walkers are set up on the host at submission. */
+int bwl_server_submit_ns(bwl_server *s, const bwl_lattice_desc *desc, const double *v_ex,
+                         uint32_t seed, const bwl_ns_params *p, const int64_t *species_numbers,
+                         const double *concentrations, double *e_limits, int64_t *id);
+/* A short Wang-Landau run: num_windows x walkers_per_window walkers (walker
+ * k seeded as rank rank_offset + k, set up by initial_setup), ln g and
+ * histograms starting at 0, one `sweeps` call at modification factor wl_f,
+ * then the window average (as bwl_wl_initial_setup + bwl_wl_sweeps(wl_f,
+ * 1)).  lng receives num_windows x bins values (each window's averaged
+ * ln g), hist num_windows x bins values (each window's averaged visit
+ * histogram in its first end-start+1 entries; the rest is left as is). */
+int bwl_server_submit_wl(bwl_server *s, const bwl_lattice_desc *desc, const double *v_ex,
+                         const bwl_wl_params *p, const int64_t *species_numbers,
+                         const double *concentrations, double wl_f, double *lng, double *hist,
+                         int64_t *id);
+/* Non-blocking: retire finished launches, admit what fits. */
+int bwl_server_progress(bwl_server *s, int32_t *n_running, int32_t *n_queued);
+/* Ids of requests finished since the last poll (up to max_ids). */
+int bwl_server_poll(bwl_server *s, int64_t *ids, int32_t max_ids, int32_t *n);
+int bwl_server_info(bwl_server *s, int64_t id, bwl_request_info *info);
+/* Blocking: progress until every request is done. */
+int bwl_server_drain(bwl_server *s);
+int bwl_server_memory(const bwl_server *s, int64_t *used_bytes, int64_t *total_bytes);
+
 #ifdef __cplusplus
 }
 #endif
