@@ -13,6 +13,14 @@ lattice, through the C-ABI.
 times), the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters
 into bytes.
 
+The served form (config 5): a Poisson stream of mixed requests (Metropolis
+anneals, nested-sampling runs, short Wang-Landau runs) submitted at their
+arrival times to the request server (continuous batching under a device
+memory budget), with per-class p50/p99 latency from arrival to completion;
+the same requests are then run one at a time through the plain entry points
+for the plain rate and to check that every served result equals its solo
+run.
+
 Prints ONE JSON line.  With --gpus N > 1 it exits: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -27,6 +35,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
+# hardware queues per process (HIP's default is 4): concurrent launches of
+# the request server each get a queue of their own (set before HIP starts)
+os.environ["GPU_MAX_HW_QUEUES"] = "32"
 
 
 def log(*a):
@@ -117,6 +128,122 @@ def batched(steps: int, warmup: int):
             "device_rate": trials / (np.mean(mks) / 1e3), "makespan_ms": float(np.mean(mks))}
 
 
+def _served_submit(srv, r):
+    p = r.params
+    if r.kind == "anneal":
+        return srv.submit_anneal(r.lattice, r.n, r.v_ex, r.config, T=p["T"], n_sweeps=p["n_sweeps"],
+                                 seed=r.seed)
+    if r.kind == "ns":
+        return srv.submit_ns(r.lattice, r.n, r.v_ex, r.seed, n_walkers=p["n_walkers"],
+                             n_steps=p["n_steps"], n_iter=p["n_iter"], species_numbers=r.counts)
+    return srv.submit_wl(r.lattice, r.n, r.v_ex, bins=p["bins"], num_windows=p["num_windows"],
+                         walkers_per_window=p["walkers_per_window"], bin_overlap=p["bin_overlap"],
+                         energy_min=p["energy_min"], energy_max=p["energy_max"],
+                         mc_sweeps=p["mc_sweeps"], wl_f=p["wl_f"], rank_offset=p["rank_offset"],
+                         species_numbers=r.counts)
+
+
+def _plain_run(r):
+    """The same request alone through the plain entry points; returns
+    (outputs, trials, seconds of the run call, seconds of device time)."""
+    from brawl_amd import Lattice, nested_sampling, wang_landau
+    p = r.params
+    if r.kind == "anneal":
+        with Lattice(r.lattice, r.n, r.v_ex) as L:
+            L.set_config(r.config)
+            t0 = time.perf_counter()
+            res = L.metropolis_simulated_annealing(T=p["T"], n_sweeps=p["n_sweeps"], seed=r.seed)
+            dt = time.perf_counter() - t0
+            return (dict(energies_of_T=res.energies_of_T, C_of_T=res.C_of_T,
+                         acceptance_of_T=res.acceptance_of_T), res.trials, dt, res.kernel_ms / 1e3)
+    if r.kind == "ns":
+        with Lattice(r.lattice, r.n, r.v_ex) as L:
+            L.seed(r.seed)
+            t0 = time.perf_counter()
+            lim, st = nested_sampling.nested_sampling(L, n_walkers=p["n_walkers"], n_steps=p["n_steps"],
+                                                      n_iter=p["n_iter"], species_numbers=r.counts)
+            return dict(e_limits=lim), st.trials, time.perf_counter() - t0, st.kernel_ms / 1e3
+    wl = wang_landau.WangLandau(r.lattice, r.n, r.v_ex, bins=p["bins"], num_windows=p["num_windows"],
+                                walkers_per_window=p["walkers_per_window"], bin_overlap=p["bin_overlap"],
+                                energy_min=p["energy_min"], energy_max=p["energy_max"],
+                                mc_sweeps=p["mc_sweeps"], rank_offset=p["rank_offset"])
+    t0 = time.perf_counter()
+    wl.initial_setup(species_numbers=r.counts)
+    _, k_ms = wl.sweeps(p["wl_f"], average=True)
+    dt = time.perf_counter() - t0
+    lng, _ = wl.lng_hist()
+    out = dict(lng=lng[:: p["walkers_per_window"]].copy())
+    wl.close()
+    trials = p["num_windows"] * p["walkers_per_window"] * p["mc_sweeps"] * r.n_atoms
+    return out, trials, dt, k_ms / 1e3
+
+
+def served(args):
+    """Config 5 through the request server, then the same requests plain."""
+    from brawl_amd import workloads
+    from brawl_amd.serve import Server
+    reqs = workloads.config5(args.served_requests, args.served_rate)
+    log(f"[bench] served: {len(reqs)} requests at {args.served_rate:g}/s")
+    with Server(device_bytes=args.served_budget_mb << 20, max_launches=args.served_slots,
+                batch_wait_us=args.served_wait_us) as srv:
+        ids, late = [], []
+        t0 = time.perf_counter()
+        for r in reqs:
+            while time.perf_counter() - t0 < r.arrival_s:
+                srv.progress()
+            late.append(time.perf_counter() - t0 - r.arrival_s)   # submission after arrival
+            ids.append(_served_submit(srv, r))
+            srv.progress()
+        srv.drain()
+        wall = time.perf_counter() - t0
+        res = [srv.result(i) for i in ids]
+        peak_mem = srv.memory()[1]
+    trials = sum(x["trials"] for x in res)
+    first = min(x["t_submit_ms"] for x in res)
+    last = max(x["t_done_ms"] for x in res)
+    makespan_s = (last - first) / 1e3
+    classes = {}
+    for r, x, lt in zip(reqs, res, late):
+        lat_ms = x["t_done_ms"] - x["t_submit_ms"] + lt * 1e3
+        c = classes.setdefault(r.kind, dict(lat=[], trials=0, n=0, k=[]))
+        c["lat"].append(lat_ms)
+        c["k"].append(x["kernel_ms"])
+        c["trials"] += x["trials"]
+        c["n"] += 1
+    log(f"[bench] served: {trials} trials, makespan {makespan_s * 1e3:.1f} ms, wall {wall * 1e3:.1f} ms")
+    # plain: one request at a time, and the served results checked against it
+    plain_s, plain_dev_s, plain_trials, mismatch = 0.0, 0.0, 0, 0
+    for r, x in zip(reqs, res):
+        out, tr, dt, dev = _plain_run(r)
+        plain_s += dt
+        plain_dev_s += dev
+        plain_trials += tr
+        same = tr == x["trials"] and all(np.array_equal(v, x[k]) for k, v in out.items())
+        mismatch += 0 if same else 1
+    log(f"[bench] plain over the same requests: {plain_trials} trials in {plain_s:.2f} s "
+        f"({plain_dev_s:.2f} s device), {mismatch} results differ")
+    per_class = {k: dict(requests=c["n"], trials=c["trials"],
+                         p50_ms=float(np.percentile(c["lat"], 50)),
+                         p99_ms=float(np.percentile(c["lat"], 99)),
+                         launch_device_ms_mean=float(np.mean(c["k"]))) for k, c in classes.items()}
+    value = trials / makespan_s
+    # the plain rate counts device time only (the most favourable to it)
+    plain_rate = plain_trials / plain_dev_s
+    return {"workload": f"{len(reqs)} requests, Poisson arrivals at {args.served_rate:g}/s: 60% Metropolis "
+                        "anneals (fcc L 8-28 / bcc L 8-32, S 2-5, 3 shells, 100-500 sweeps), 25% nested "
+                        "sampling (quinary fcc 6^3, 128-640 walkers, walk length 100, 200 iterations), "
+                        "15% short Wang-Landau runs (bcc L 6-10, 4-8 walkers, 20 sweeps)",
+            "value": value, "unit": "swap trials/s", "trials": trials,
+            "makespan_ms": makespan_s * 1e3, "wall_ms": wall * 1e3,
+            "latency_from_arrival": per_class,
+            "plain_same_requests": plain_rate,
+            "plain_same_requests_wall": plain_trials / plain_s, "vs_plain": value / plain_rate,
+            "results_differing_from_solo": mismatch,
+            "server": {"device_bytes": peak_mem, "launch_slots": args.served_slots,
+                       "batch_wait_us": args.served_wait_us,
+                       "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +253,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--only-served", action="store_true",
+                    help="run only the served workload (for profiling it)")
+    ap.add_argument("--served-requests", type=int, default=240)
+    ap.add_argument("--served-rate", type=float, default=400.0)
+    ap.add_argument("--served-slots", type=int, default=32)
+    ap.add_argument("--served-budget-mb", type=int, default=1024)
+    ap.add_argument("--served-wait-us", type=int, default=5000)
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -138,6 +273,9 @@ def main():
     from brawl_amd import Lattice, workloads
     from brawl_amd import _lib
 
+    if args.only_served:
+        print(json.dumps({"served": served(args)}))
+        return
     chain = workloads.config2()
     log(f"[bench] workload {chain.name}: N={chain.n_atoms} S={chain.S} R={chain.R}")
     # start configuration: equiatomic random occupation (reference initial_setup)
@@ -205,6 +343,8 @@ def main():
     if not args.no_batched:
         out["batched"] = batched(max(1, args.steps), min(1, args.warmup))
         out["batched"]["vs_plain"] = out["batched"]["value"] / value
+    if not args.no_served:
+        out["served"] = served(args)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(chain, args.cpu_budget)

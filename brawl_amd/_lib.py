@@ -65,7 +65,8 @@ class WlParams(C.Structure):
 
 
 class ServerParams(C.Structure):
-    _fields_ = [("device_bytes", C.c_int64), ("max_launches", C.c_int32), ("reserved", C.c_int32)]
+    _fields_ = [("device_bytes", C.c_int64), ("max_launches", C.c_int32),
+                ("batch_wait_us", C.c_int32)]
 
 
 class RequestInfo(C.Structure):

@@ -256,7 +256,13 @@ int bwl_nested_sampling_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_ns_para
   }
   HIP_OK(hipMemcpyAsync(base, args.data(), sizeof(NsArgs) * (size_t)n, hipMemcpyHostToDevice, c0->stream));
   HIP_OK(hipEventRecord(c0->ev0, c0->stream));
-  HIP_OK(launch_ns((const NsArgs *)base, n, lds, c0->stream));
+  int max_w = 0;
+  size_t lds_e = 0;
+  for (int i = 0; i < n; ++i) {
+    max_w = std::max(max_w, params[i].n_walkers);
+    lds_e = std::max(lds_e, ns_energies_lds_bytes(ctxs[i]->g));
+  }
+  HIP_OK(launch_ns((const NsArgs *)base, n, lds, max_w, lds_e, c0->stream));
   HIP_OK(hipEventRecord(c0->ev1, c0->stream));
   HIP_OK(hipStreamSynchronize(c0->stream));
   float ms = 0.f;

@@ -76,7 +76,7 @@ struct NsArgs {
   const double *vex;
   int8_t *walkers;        // n_walkers compact lattices, g.compact_bytes apart
   const double *rnde;     // per walker: the genrand() drawn after its set-up
-  double *E;              // out: final walker energies
+  double *E;              // walker energies: initial (k_ns_energies), then final
   uint32_t *mt;           // 624 words, in/out
   int32_t *mt_index;      // in/out
   int32_t n_walkers, n_steps, n_iter, n_at;
@@ -151,7 +151,11 @@ hipError_t launch_metro_par(const ParArgs *args_dev, int n_chains, int lattice, 
                             bool nat, size_t lds, int threads, bool wide, hipStream_t s);
 
 size_t ns_lds_bytes(const Geo &g, int n_walkers);
-hipError_t launch_ns(const NsArgs *args_dev, int n, size_t lds, hipStream_t s);
+// Initial walker energies (all walkers at once), then the runs (one
+// wavefront each).
+hipError_t launch_ns(const NsArgs *args_dev, int n, size_t lds, int max_walkers, size_t lds_energies,
+                     hipStream_t s);
+size_t ns_energies_lds_bytes(const Geo &g);
 size_t wl_lds_bytes(const Geo &g, int bins, int window_bins);
 hipError_t launch_wl_sweeps(const WlArgs *args_dev, int n, size_t lds, hipStream_t s);
 hipError_t launch_wl_average(double *lng, double *hist, int n_windows, int walkers_per_window,

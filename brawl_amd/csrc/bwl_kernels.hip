@@ -66,7 +66,9 @@ __global__ __launch_bounds__(512) void k_histogram(const HistArgs *__restrict__ 
   const int hsize = g.S * g.S * g.R;
   const int nw = blockDim.x >> 6;
   uint32_t *hw = (uint32_t *)(smem + ((g.lds_bytes + 15) & ~15));
+  int32_t *offl = (int32_t *)(hw + hsize * nw);
   for (int i = threadIdx.x; i < hsize * nw; i += blockDim.x) hw[i] = 0;
+  stage_offsets(offl, A.off, g);
   stage_lattice(lat, A.compact, g, 0);
   __syncthreads();
   uint32_t *mine = hw + (threadIdx.x >> 6) * hsize;
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(512) void k_histogram(const HistArgs *__restrict__ 
     int addr = padded_addr(g, s, cx, cy, cz);
     int a = lat[addr];
     if (a <= 0) continue;
-    const int32_t *off = A.off + s * g.z;
+    const int32_t *off = offl + s * g.z;
     for (int n = 0; n < g.R; ++n) {
       // 8-bit packed counts per species (a shell has at most 32 entries)
       uint64_t c = 0;
@@ -107,7 +109,11 @@ __global__ __launch_bounds__(64) void k_site_energy(SiteArgs A) {
   const Geo &g = A.g;
   int8_t *lat = (int8_t *)smem;
   double *vex = (double *)(smem + ((g.lds_bytes + 15) & ~15));
+  int32_t *offl = (int32_t *)(vex + g.S * g.S * g.R);
   for (int i = threadIdx.x; i < g.S * g.S * g.R; i += blockDim.x) vex[i] = A.vex[i];
+  stage_offsets(offl, A.off, g);
+  PairLanes pl;
+  init_pair_lanes(g, pl);
   stage_lattice(lat, A.compact, g, 0);
   __syncthreads();
   double eA = 0.0;
@@ -115,7 +121,7 @@ __global__ __launch_bounds__(64) void k_site_energy(SiteArgs A) {
   int addrB = padded_addr_xyz(g, A.x[1], A.y[1], A.z[1]);
   int subA = sub_of(g.lattice, A.x[0] & 1, A.y[0] & 1);
   int subB = sub_of(g.lattice, A.x[1] & 1, A.y[1] & 1);
-  double e = pair_energy_lanes(lat, g, A.off, vex, addrA, subA, addrB, subB, &eA);
+  double e = pair_energy_lanes(lat, g, offl, vex, pl, addrA, subA, addrB, subB, &eA);
   if (threadIdx.x == 0) {
     A.out[0] = eA;
     A.out[1] = e;
@@ -135,9 +141,14 @@ __global__ __launch_bounds__(64) void k_metro_seq(SeqArgs A) {
   uint32_t *st = (uint32_t *)(smem + o);
   o += 624 * 4;
   double *vex = (double *)(smem + o);
+  o += sizeof(double) * g.S * g.S * g.R;
+  int32_t *offl = (int32_t *)(smem + o);
 
   for (int i = lane; i < 624; i += 64) st[i] = A.mt[i];
   for (int i = lane; i < g.S * g.S * g.R; i += 64) vex[i] = A.vex[i];
+  stage_offsets(offl, A.off, g);
+  PairLanes pl;
+  init_pair_lanes(g, pl);
   stage_lattice(lat, A.compact, g, 0);
   __syncthreads();
   int mti = A.mt_index[0];
@@ -157,12 +168,12 @@ __global__ __launch_bounds__(64) void k_metro_seq(SeqArgs A) {
     if (sp1 == sp2) {
       acc = true;  // metropolis.F90:774-777: counted as accepted, no draw
     } else {
-      double e_un = pair_energy_lanes(lat, g, A.off, vex, addr1, sub1, addr2, sub2, nullptr);
+      double e_un = pair_energy_lanes(lat, g, offl, vex, pl, addr1, sub1, addr2, sub2, nullptr);
       __syncthreads();
       write_site(lat, g, sub1, s1.x >> 1, s1.y >> 1, s1.z >> 1, (int8_t)sp2, lane, 64);
       write_site(lat, g, sub2, s2.x >> 1, s2.y >> 1, s2.z >> 1, (int8_t)sp1, lane, 64);
       __syncthreads();
-      double e_sw = pair_energy_lanes(lat, g, A.off, vex, addr1, sub1, addr2, sub2, nullptr);
+      double e_sw = pair_energy_lanes(lat, g, offl, vex, pl, addr1, sub1, addr2, sub2, nullptr);
       double de = e_sw - e_un;
       if (de < 0.0) {
         acc = true;
@@ -180,7 +191,7 @@ __global__ __launch_bounds__(64) void k_metro_seq(SeqArgs A) {
     }
     if (acc) accepted++;
     if (A.energies && A.sample_every > 0 && trial % A.sample_every == 0) {
-      const double e = ref_total_energy(lat, g, A.off, vex);
+      const double e = ref_total_energy(lat, g, offl, vex);
       if (lane == 0) A.energies[sample_k] = e;
       sample_k++;
     }
@@ -214,7 +225,8 @@ hipError_t launch_unpack(const int8_t *compact, int8_t *ref, const Geo &g, hipSt
 }
 
 size_t hist_lds_bytes(const Geo &g, int threads) {
-  return (size_t)((g.lds_bytes + 15) & ~15) + sizeof(uint32_t) * g.S * g.S * g.R * (threads / 64);
+  return (size_t)((g.lds_bytes + 15) & ~15) + sizeof(uint32_t) * g.S * g.S * g.R * (threads / 64) +
+         sizeof(int32_t) * g.nsub * g.z;
 }
 
 hipError_t launch_histogram(const HistArgs *args_dev, int n, const Geo &g_max, int threads,
@@ -228,7 +240,8 @@ hipError_t launch_histogram(const HistArgs *args_dev, int n, const Geo &g_max, i
 }
 
 size_t site_lds_bytes(const Geo &g) {
-  return (size_t)((g.lds_bytes + 15) & ~15) + sizeof(double) * g.S * g.S * g.R;
+  return (size_t)((g.lds_bytes + 15) & ~15) + sizeof(double) * g.S * g.S * g.R +
+         sizeof(int32_t) * g.nsub * g.z;
 }
 
 hipError_t launch_site_energy(const SiteArgs &a, hipStream_t s) {
@@ -241,7 +254,8 @@ hipError_t launch_site_energy(const SiteArgs &a, hipStream_t s) {
 }
 
 size_t seq_lds_bytes(const Geo &g) {
-  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + sizeof(double) * g.S * g.S * g.R + 64;
+  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + sizeof(double) * g.S * g.S * g.R +
+         sizeof(int32_t) * g.nsub * g.z + 64;
 }
 
 hipError_t launch_metro_seq(const SeqArgs &a, hipStream_t s) {

@@ -24,26 +24,149 @@ __device__ inline double shell_sum_lane(const int8_t *lat, const Geo &g, const i
   return e;
 }
 
-// Energy of sites A and B (lanes [0,R) and [R,2R)); returns e_A + e_B on
-// every lane, each in reference order.
-__device__ inline double pair_energy_lanes(const int8_t *lat, const Geo &g, const int32_t *off,
-                                           const double *vex, int addrA, int subA,
-                                           int addrB, int subB, double *eA_out) {
-  const int lane = threadIdx.x & 63;
-  double v = 0.0;
-  if (lane < 2 * g.R) {
-    bool isA = lane < g.R;
-    int n = isA ? lane : lane - g.R;
-    int addr = isA ? addrA : addrB;
-    v = shell_sum_lane(lat, g, off, vex, addr, isA ? subA : subB, lat[addr], n);
+// Value of `v` in lane `l` (l uniform across the wavefront).
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Copy the [nsub][z] neighbour-offset table into LDS (`dst`).
+__device__ inline void stage_offsets(int32_t *dst, const int32_t *__restrict__ src, const Geo &g) {
+  for (int i = threadIdx.x; i < g.nsub * g.z; i += blockDim.x) dst[i] = src[i];
+}
+
+// Per-lane constants of pair_energy_lanes, computed once per kernel: the
+// shell of each lane's entries and, per 64-entry pass, the mask of entries
+// that end a shell.  Entries 0..z-1 are site A's neighbours in list order,
+// z..2z-1 site B's.
+struct PairLanes {
+  unsigned long long end_mask[6];
+  int shell[6];
+  int passes;
+  int fixed;  // compile-time summation variant (0: generic loop)
+};
+
+// First list entry of shell n (0-based) of a lattice, at compile time.
+template <int LAT>
+__host__ __device__ constexpr int shell_first(int n) {
+  int s = 0;
+  for (int k = 0; k < n; ++k) s += LAT == 1 ? kFccShellSize[k] : kBccShellSize[k];
+  return s;
+}
+
+// The ordered sum of pair_energy_lanes with the shell structure known at
+// compile time: straight-line readlane/add code, no loop or branches.
+template <int LAT, int R>
+__device__ __forceinline__ double fixed_pair_sum(const double (&vals)[6], double *eA_out) {
+  constexpr int z = shell_first<LAT>(R);
+  double eA = 0.0, eB = 0.0;
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    double es = 0.0;
+#pragma unroll
+    for (int t = shell_first<LAT>(n); t < shell_first<LAT>(n + 1); ++t)
+      es = es + readlane_d(vals[t >> 6], t & 63);
+    eA = n == 0 ? es : eA + es;
   }
-  double eA = __shfl(v, 0), eB = __shfl(v, g.R);
-  for (int n = 1; n < g.R; ++n) {
-    eA = eA + __shfl(v, n);
-    eB = eB + __shfl(v, g.R + n);
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    double es = 0.0;
+#pragma unroll
+    for (int t = z + shell_first<LAT>(n); t < z + shell_first<LAT>(n + 1); ++t)
+      es = es + readlane_d(vals[t >> 6], t & 63);
+    eB = n == 0 ? es : eB + es;
   }
   if (eA_out) *eA_out = eA;
   return eA + eB;
+}
+
+__device__ inline void init_pair_lanes(const Geo &g, PairLanes &pl) {
+  const int lane = threadIdx.x & 63;
+  const int z = g.z;
+  int bounds[kMaxShells + 1];
+#pragma unroll
+  for (int i = 0; i <= kMaxShells; ++i) bounds[i] = g.shell_start[i];
+  pl.passes = (2 * z + 63) / 64;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int t = 64 * k + lane;
+    const int tt = t < z ? t : t - z;
+    int n = 0;
+    bool last = false;
+#pragma unroll
+    for (int i = 1; i <= kMaxShells; ++i) {
+      if (i <= g.R && tt >= bounds[i]) n = i;
+      if (i <= g.R && tt + 1 == bounds[i]) last = true;
+    }
+    pl.shell[k] = n;
+    pl.end_mask[k] = __ballot(t < 2 * z && last);
+  }
+  const int key = g.lattice * 16 + g.R;
+  pl.fixed = (key == 16 + 2 || key == 16 + 3 || key == 16 + 4 || key == 32 + 2 || key == 32 + 3 ||
+              key == 32 + 4)
+                 ? key
+                 : 0;
+}
+
+// Energies of sites A and B in the reference's FP64 order (pair_energy,
+// bw_hamiltonian.f90:99-114): every lane gathers neighbour V entries (one
+// per 64-entry pass), then the entries are added in list order from 0.0
+// per shell, shells left to right per site, and e_A + e_B -- on every lane,
+// from the lanes' values.  `off` is the offset table and `vex` V_ex, both
+// in LDS.
+__device__ inline double pair_energy_lanes(const int8_t *lat, const Geo &g, const int32_t *off,
+                                           const double *vex, const PairLanes &pl, int addrA,
+                                           int subA, int addrB, int subB, double *eA_out) {
+  const int lane = threadIdx.x & 63;
+  const int z = g.z, S = g.S;
+  const int spA = lat[addrA], spB = lat[addrB];
+  double vals[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    vals[k] = 0.0;
+    const int t = 64 * k + lane;
+    if (k < pl.passes && t < 2 * z) {
+      const bool isA = t < z;
+      const int tt = isA ? t : t - z;
+      const int addr = isA ? addrA : addrB, sub = isA ? subA : subB, sp = isA ? spA : spB;
+      const int nb = lat[addr + off[sub * z + tt]];
+      vals[k] = vex[(sp - 1) + S * ((nb - 1) + S * pl.shell[k])];
+    }
+  }
+  switch (pl.fixed) {
+    case 16 + 2: return fixed_pair_sum<1, 2>(vals, eA_out);
+    case 16 + 3: return fixed_pair_sum<1, 3>(vals, eA_out);
+    case 16 + 4: return fixed_pair_sum<1, 4>(vals, eA_out);
+    case 32 + 2: return fixed_pair_sum<2, 2>(vals, eA_out);
+    case 32 + 3: return fixed_pair_sum<2, 3>(vals, eA_out);
+    case 32 + 4: return fixed_pair_sum<2, 4>(vals, eA_out);
+    default: break;
+  }
+  double eA = 0.0, e = 0.0, es = 0.0;
+  bool first = true;
+  int t = 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if (k >= pl.passes) break;
+    const int lim = 2 * z - 64 * k < 64 ? 2 * z - 64 * k : 64;
+    const unsigned long long em = pl.end_mask[k];
+    for (int l = 0; l < lim; ++l, ++t) {
+      es = es + readlane_d(vals[k], l);
+      if ((em >> l) & 1ull) {
+        e = first ? es : e + es;
+        es = 0.0;
+        first = false;
+        if (t == z - 1) {
+          eA = e;
+          first = true;
+        }
+      }
+    }
+  }
+  if (eA_out) *eA_out = eA;
+  return eA + e;
 }
 
 struct SiteXYZ {
@@ -123,14 +246,6 @@ __device__ inline double wave_total_energy(const int8_t *lat, const Geo &g, cons
   return 0.5 * block_sum(part, red);
 }
 
-
-// Value of `v` in lane `l` (l uniform across the wavefront).
-__device__ __forceinline__ double readlane_d(double v, int l) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
 
 // total_energy (bw_hamiltonian.f90:58-81) in the reference's order: sites in
 // doubled-grid order (x fastest, then y, then z), empty cells skipped, site

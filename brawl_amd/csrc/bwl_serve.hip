@@ -152,6 +152,7 @@ struct bwl_server {
   std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
   std::vector<int64_t> done_unpolled;
   int64_t next_id = 1;
+  double batch_wait_ms = 0.0;
   std::chrono::steady_clock::time_point t0;
 };
 
@@ -380,7 +381,13 @@ int launch_ns_group(bwl_server *s, std::vector<Request *> &grp, int slot) {
   HIP_OK(hipEventCreate(&L->ev0));
   HIP_OK(hipEventCreate(&L->ev1));
   HIP_OK(hipEventRecord(L->ev0, st));
-  HIP_OK(launch_ns((const NsArgs *)L->scratch, (int)grp.size(), lds, st));
+  int max_w = 0;
+  size_t lds_e = 0;
+  for (Request *r : grp) {
+    max_w = std::max(max_w, r->np.n_walkers);
+    lds_e = std::max(lds_e, ns_energies_lds_bytes(r->g));
+  }
+  HIP_OK(launch_ns((const NsArgs *)L->scratch, (int)grp.size(), lds, max_w, lds_e, st));
   HIP_OK(hipEventRecord(L->ev1, st));
   s->slot_busy[(size_t)slot] = true;
   s->running.push_back(std::move(L));
@@ -483,8 +490,14 @@ int fail_request(bwl_server *s, Request *r, int rc) {
 }
 
 // Admit queued requests while memory and slots last, grouped by variant.
+// With a batching window, nothing is admitted until the oldest queued
+// request has waited that long (so requests arriving close together share
+// launches) unless no launch is running at all.
 int admit(bwl_server *s) {
   if (s->queue.empty() || free_slot(s) < 0) return BWL_OK;
+  if (s->batch_wait_ms > 0.0 && !s->running.empty() &&
+      now_ms(s->t0) - s->queue.front()->t_submit < s->batch_wait_ms)
+    return BWL_OK;
   // candidates in arrival order; a request that does not fit is skipped
   std::vector<Request *> taken;
   std::deque<Request *> keep;
@@ -577,8 +590,10 @@ extern "C" {
 int bwl_server_create(const bwl_server_params *p, bwl_server **out) {
   if (!p || !out) return fail(BWL_ERR_ARG, "null argument");
   *out = nullptr;
-  if (p->device_bytes < (1 << 20) || p->max_launches < 1 || p->max_launches > 64)
-    return fail(BWL_ERR_ARG, "device_bytes >= 1 MiB and 1 <= max_launches <= 64 are required");
+  if (p->device_bytes < (1 << 20) || p->max_launches < 1 || p->max_launches > 64 ||
+      p->batch_wait_us < 0)
+    return fail(BWL_ERR_ARG,
+                "device_bytes >= 1 MiB, 1 <= max_launches <= 64 and batch_wait_us >= 0 are required");
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return fail(BWL_ERR_DEVICE, "no HIP device available");
   auto s = std::make_unique<bwl_server>();
@@ -595,6 +610,7 @@ int bwl_server_create(const bwl_server_params *p, bwl_server **out) {
     s->streams.push_back(st);
     s->slot_busy.push_back(false);
   }
+  s->batch_wait_ms = p->batch_wait_us / 1000.0;
   s->t0 = std::chrono::steady_clock::now();
   *out = s.release();
   return BWL_OK;

@@ -40,6 +40,31 @@ __device__ inline double genrand_dev(uint32_t *st, int &mti) {
 }  // namespace
 
 // ------------------------------------------------------------------
+// Initial walker energies of nested-sampling runs, all walkers of all runs
+// at once (block (w, run)): total_energy + genrand()*1e-8
+// (nested_sampling.f90:95; 1e-8 is a REAL(4) literal there), the total in
+// the reference's summation order.
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_ns_energies(const NsArgs *__restrict__ args) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const NsArgs &A = args[blockIdx.y];
+  const int w = blockIdx.x;
+  if (w >= A.n_walkers) return;
+  const Geo &g = A.g;
+  int8_t *lat = (int8_t *)smem;
+  size_t o = (g.lds_bytes + 15) & ~15;
+  double *vex = (double *)(smem + o);
+  o += sizeof(double) * g.S * g.S * g.R;
+  int32_t *offl = (int32_t *)(smem + o);
+  for (int i = threadIdx.x; i < g.S * g.S * g.R; i += 64) vex[i] = A.vex[i];
+  stage_offsets(offl, A.off, g);
+  stage_lattice(lat, A.walkers + (size_t)g.compact_bytes * w, g, 0);
+  __syncthreads();
+  const double e = ref_total_energy(lat, g, offl, vex);
+  if (threadIdx.x == 0) A.E[w] = e + A.rnde[w] * (double)1e-8f;
+}
+
+// ------------------------------------------------------------------
 // Nested sampling: the whole serial loop of nested_sampling.f90:78-200 for
 // one run.  Walker lattices stay in HBM; the walker being walked is staged
 // in LDS.  Walker energies live in LDS.
@@ -56,22 +81,20 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
   double *vex = (double *)(smem + o);
   o += sizeof(double) * g.S * g.S * g.R;
   double *E = (double *)(smem + o);
+  o += sizeof(double) * A.n_walkers;
+  int32_t *offl = (int32_t *)(smem + o);
 
   for (int i = lane; i < 624; i += 64) st[i] = A.mt[i];
   for (int i = lane; i < g.S * g.S * g.R; i += 64) vex[i] = A.vex[i];
+  stage_offsets(offl, A.off, g);
+  PairLanes pl;
+  init_pair_lanes(g, pl);
   int mti = A.mt_index[0];
   const int nw = A.n_walkers;
   const size_t cb = (size_t)g.compact_bytes;
 
-  // walker energies: total_energy + genrand()*1e-8 (nested_sampling.f90:95,
-  // 1e-8 is a REAL(4) literal there)
-  for (int w = 0; w < nw; ++w) {
-    __syncthreads();
-    stage_lattice(lat, A.walkers + cb * w, g, 0);
-    __syncthreads();
-    const double e = ref_total_energy(lat, g, A.off, vex);
-    if (lane == 0) E[w] = e + A.rnde[w] * (double)1e-8f;
-  }
+  // walker energies, computed beforehand by k_ns_energies
+  for (int w = lane; w < nw; w += 64) E[w] = A.E[w];
   __syncthreads();
 
   long long trials = 0, accepted = 0;
@@ -132,9 +155,9 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
       }
       const int sub1 = sub_of(g.lattice, r1.x & 1, r1.y & 1);
       const int sub2 = sub_of(g.lattice, r2.x & 1, r2.y & 1);
-      const double eu = pair_energy_lanes(lat, g, A.off, vex, a1, sub1, a2, sub2, nullptr);
+      const double eu = pair_energy_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, nullptr);
       swap_sites(lat, g, r1, sub1, (int8_t)s1, r2, sub2, (int8_t)s2);
-      const double es = pair_energy_lanes(lat, g, A.off, vex, a1, sub1, a2, sub2, nullptr);
+      const double es = pair_energy_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, nullptr);
       const double de = es - eu;
       ++trials;
       if (ecur + de < limit) {
@@ -182,9 +205,14 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
   double *edges = (double *)(smem + o);
   o += sizeof(double) * (A.bins + 1);
   double *hist = (double *)(smem + o);
+  o += sizeof(double) * hb;
+  int32_t *offl = (int32_t *)(smem + o);
 
   for (int i = lane; i < 624; i += 64) st[i] = A.mt[i];
   for (int i = lane; i < g.S * g.S * g.R; i += 64) vex[i] = A.vex[i];
+  stage_offsets(offl, A.off, g);
+  PairLanes pl;
+  init_pair_lanes(g, pl);
   for (int i = lane; i < A.bins; i += 64) lng[i] = A.lng[i];
   for (int i = lane; i <= A.bins; i += 64) edges[i] = A.bin_edges[i];
   for (int i = lane; i < hb; i += 64) hist[i] = A.hist[i];
@@ -193,7 +221,7 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
   int mti = A.mt_index[0];
   const int bins = A.bins;
   const long long cadence = (long long)(int)(0.02 * (double)(float)g.n_atoms);
-  double e_un = ref_total_energy(lat, g, A.off, vex);
+  double e_un = ref_total_energy(lat, g, offl, vex);
   long long accepted = 0;
   const long long total = (long long)A.mc_sweeps * g.n_atoms;
   for (long long i = 1; i <= total; ++i) {
@@ -208,9 +236,9 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
     // are equal; neither changes the state, so both are skipped here.
     double e_sw = e_un;
     if (s1 != s2) {
-      const double pu = pair_energy_lanes(lat, g, A.off, vex, a1, sub1, a2, sub2, nullptr);
+      const double pu = pair_energy_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, nullptr);
       swap_sites(lat, g, r1, sub1, (int8_t)s1, r2, sub2, (int8_t)s2);
-      const double ps = pair_energy_lanes(lat, g, A.off, vex, a1, sub1, a2, sub2, nullptr);
+      const double ps = pair_energy_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, nullptr);
       e_sw = e_un - pu + ps;
     }
     const int ibin = wl_bin_index(e_un, edges, bins);
@@ -287,15 +315,20 @@ __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__
   uint32_t *st = (uint32_t *)(smem + o);
   o += 624 * 4;
   double *vex = (double *)(smem + o);
+  o += sizeof(double) * g.S * g.S * g.R;
+  int32_t *offl = (int32_t *)(smem + o);
 
   for (int i = lane; i < 624; i += 64) st[i] = A.mt[i];
   for (int i = lane; i < g.S * g.S * g.R; i += 64) vex[i] = A.vex[i];
+  stage_offsets(offl, A.off, g);
+  PairLanes pl;
+  init_pair_lanes(g, pl);
   stage_lattice(lat, A.compact, g, 0);
   __syncthreads();
   int mti = A.mt_index[0];
   const double target = (A.min_e + A.max_e) / 2.0;
   const double cond = fabs(A.max_e - A.min_e) * 0.1;
-  double e_un = A.init_e ? ref_total_energy(lat, g, A.off, vex) : A.e_unswapped[0];
+  double e_un = A.init_e ? ref_total_energy(lat, g, offl, vex) : A.e_unswapped[0];
   long long isteps = A.i_steps[0];
   long long trials = A.trials[0];
   long long made = 0;
@@ -312,7 +345,7 @@ __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__
     }
     skip = false;
     if (e_un < A.max_e - cond && e_un > A.min_e + cond) {
-      e_un = ref_total_energy(lat, g, A.off, vex);
+      e_un = ref_total_energy(lat, g, offl, vex);
       if (e_un < A.max_e - cond && e_un > A.min_e + cond) {
         status = 0;
         break;
@@ -334,9 +367,9 @@ __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__
     if (s1 != s2) {
       const int sub1 = sub_of(g.lattice, r1.x & 1, r1.y & 1);
       const int sub2 = sub_of(g.lattice, r2.x & 1, r2.y & 1);
-      const double pu = pair_energy_lanes(lat, g, A.off, vex, a1, sub1, a2, sub2, nullptr);
+      const double pu = pair_energy_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, nullptr);
       swap_sites(lat, g, r1, sub1, (int8_t)s1, r2, sub2, (int8_t)s2);
-      const double ps = pair_energy_lanes(lat, g, A.off, vex, a1, sub1, a2, sub2, nullptr);
+      const double ps = pair_energy_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, nullptr);
       const double e_sw = e_un - pu + ps;
       const double de = ((e_sw - target) * (e_sw - target) - (e_un - target) * (e_un - target)) /
                         A.denom;
@@ -423,20 +456,31 @@ __global__ __launch_bounds__(256) void k_asro(AsroArgs A) {
 // ------------------------------------------------------------------
 size_t ns_lds_bytes(const Geo &g, int n_walkers) {
   return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + sizeof(double) * g.S * g.S * g.R +
-         sizeof(double) * (size_t)n_walkers;
+         sizeof(double) * (size_t)n_walkers + sizeof(int32_t) * g.nsub * g.z;
 }
 
-hipError_t launch_ns(const NsArgs *args_dev, int n, size_t lds, hipStream_t s) {
-  hipError_t e = hipFuncSetAttribute((const void *)k_ns,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+hipError_t launch_ns(const NsArgs *args_dev, int n, size_t lds, int max_walkers, size_t lds_energies,
+                     hipStream_t s) {
+  hipError_t e = hipFuncSetAttribute((const void *)k_ns_energies,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_energies);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_ns_energies, dim3(max_walkers, n), dim3(64), lds_energies, s, args_dev);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void *)k_ns, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_ns, dim3(n), dim3(64), lds, s, args_dev);
   return hipGetLastError();
 }
 
+size_t ns_energies_lds_bytes(const Geo &g) {
+  return (size_t)((g.lds_bytes + 15) & ~15) + sizeof(double) * g.S * g.S * g.R +
+         sizeof(int32_t) * g.nsub * g.z;
+}
+
 size_t wl_lds_bytes(const Geo &g, int bins, int window_bins) {
   return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + sizeof(double) * g.S * g.S * g.R +
-         sizeof(double) * (size_t)(2 * bins + 1 + window_bins);
+         sizeof(double) * (size_t)(2 * bins + 1 + window_bins) + sizeof(int32_t) * g.nsub * g.z;
 }
 
 hipError_t launch_wl_sweeps(const WlArgs *args_dev, int n, size_t lds, hipStream_t s) {
@@ -456,7 +500,8 @@ hipError_t launch_wl_average(double *lng, double *hist, int n_windows, int wpw, 
 }
 
 size_t wl_enter_lds_bytes(const Geo &g) {
-  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + sizeof(double) * g.S * g.S * g.R;
+  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + sizeof(double) * g.S * g.S * g.R +
+         sizeof(int32_t) * g.nsub * g.z;
 }
 
 hipError_t launch_wl_enter(const WlEnterArgs *args_dev, int n, size_t lds, hipStream_t s) {

@@ -46,11 +46,11 @@ class Request:
 
 
 class Server:
-    def __init__(self, device_bytes: int = 256 << 20, max_launches: int = 8):
+    def __init__(self, device_bytes: int = 256 << 20, max_launches: int = 8, batch_wait_us: int = 0):
         self._L = _lib.load()
         h = C.c_void_p()
-        check(self._L.bwl_server_create(C.byref(ServerParams(int(device_bytes), int(max_launches), 0)),
-                                        C.byref(h)))
+        p = ServerParams(int(device_bytes), int(max_launches), int(batch_wait_us))
+        check(self._L.bwl_server_create(C.byref(p), C.byref(h)))
         self._h = h
         self.requests: dict[int, Request] = {}
 

@@ -124,3 +124,79 @@ def config3(n_chains: int = 256, seed: int = MASTER_SEED, n_sweeps: int = 500):
         T = float(rng.uniform(200.0, 2000.0))
         out.append(Chain(lat, (L, L, L), v, counts, T, 0.0, 1, n_sweeps, s, f"chain{c}"))
     return out
+
+
+def random_config(lattice: int, n, counts, rng: np.random.Generator) -> np.ndarray:
+    """A uniformly random occupation with exact species counts, in the
+    reference's layout (2n1, 2n2, 2n3) int8, Fortran order."""
+    m = tuple(2 * x for x in n)
+    x, y, z = np.meshgrid(*[np.arange(k) for k in m], indexing="ij")
+    occ = ((x + y + z) % 2 == 0) if lattice == FCC else ((x % 2 == y % 2) & (y % 2 == z % 2))
+    species = np.repeat(np.arange(1, len(counts) + 1, dtype=np.int8), counts)
+    rng.shuffle(species)
+    cfg = np.zeros(m, dtype=np.int8, order="F")
+    cfg[occ] = species
+    return np.asfortranarray(cfg)
+
+
+@dataclass
+class ServedRequest:
+    kind: str            # "anneal", "ns", "wl"
+    arrival_s: float
+    lattice: int
+    n: tuple
+    v_ex: np.ndarray
+    seed: int
+    params: dict
+    counts: np.ndarray = None
+    config: np.ndarray = None
+
+    @property
+    def n_atoms(self):
+        return n_atoms_of(self.lattice, self.n)
+
+
+def config5(n_requests: int = 240, rate_hz: float = 400.0, seed: int = MASTER_SEED):
+    """Served mixed stream, Poisson arrivals at rate_hz requests/s:
+    60 % Metropolis (fcc or bcc; bcc L ~ U{8..32}, fcc L ~ U{8..28}, the
+    largest fcc box whose 3-shell lattice fits one workgroup's LDS;
+    S ~ U{2..5}, Dirichlet(1) counts, 3 shells of 25/n meV, T ~ U[200,
+    2000] K, U{100..500} sweeps), 25 % nested sampling (quinary fcc 6^3,
+    4 shells of 30/n meV, U{128..640} walkers, walk length 100, 200
+    iterations), 15 % short Wang-Landau runs (bcc L ~ U{6..10}, S = 4, 2
+    shells of 30/n meV, one window of 50 bins over [-10, 10] meV/atom,
+    U{4..8} walkers, 20 sweeps at ln f = 1)."""
+    rng = np.random.Generator(np.random.PCG64(derive_seed(seed, 5)))
+    t = 0.0
+    out = []
+    for r in range(n_requests):
+        t += float(rng.exponential(1.0 / rate_hz))
+        s = derive_seed(seed, 50000 + r)
+        u = float(rng.uniform())
+        if u < 0.60:
+            lat = FCC if rng.integers(0, 2) == 0 else BCC
+            L = int(rng.integers(8, 29 if lat == FCC else 33))
+            S = int(rng.integers(2, 6))
+            v = random_v(S, [25.0 / k for k in range(1, 4)], s ^ 0x5A5A)
+            N = n_atoms_of(lat, (L, L, L))
+            counts = dirichlet_counts(N, S, rng)
+            cfg = random_config(lat, (L, L, L), counts, rng)
+            out.append(ServedRequest("anneal", t, lat, (L, L, L), v, s,
+                                     dict(T=float(rng.uniform(200.0, 2000.0)),
+                                          n_sweeps=int(rng.integers(100, 501))), counts, cfg))
+        elif u < 0.85:
+            S = 5
+            v = random_v(S, [30.0 / k for k in range(1, 5)], s ^ 0x5A5A)
+            out.append(ServedRequest("ns", t, FCC, (6, 6, 6), v, s & 0xFFFFFFFF,
+                                     dict(n_walkers=int(rng.integers(128, 641)), n_steps=100,
+                                          n_iter=200), equal_counts(864, S)))
+        else:
+            L = int(rng.integers(6, 11))
+            v = random_v(4, [30.0, 15.0], s ^ 0x5A5A)
+            out.append(ServedRequest("wl", t, BCC, (L, L, L), v, s,
+                                     dict(bins=50, num_windows=1,
+                                          walkers_per_window=int(rng.integers(4, 9)),
+                                          bin_overlap=0.1, energy_min=-10.0, energy_max=10.0,
+                                          mc_sweeps=20, wl_f=1.0, rank_offset=int(s % 1000)),
+                                     equal_counts(n_atoms_of(BCC, (L, L, L)), 4)))
+    return out

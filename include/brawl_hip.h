@@ -285,7 +285,9 @@ int bwl_wl_sweeps(bwl_wl *wl, double wl_f, int32_t average, int64_t *accepted, d
 typedef struct {
   int64_t device_bytes;  /* admission budget: one device arena of this size */
   int32_t max_launches;  /* launch slots (streams), 1..64 */
-  int32_t reserved;
+  int32_t batch_wait_us; /* batching window: while launches run, admit only
+                            once the oldest queued request has waited this
+                            long (0: admit at once) */
 } bwl_server_params;
 
 typedef struct {
