@@ -54,6 +54,18 @@ def bytes_per_trial(z: int) -> int:
 # x1.0): the lattice is read once into LDS and written back once.
 TRAFFIC_BYTES = int(round((51.125 * 2.0 + 65.21875) * 1024))
 PLAIN_KERNEL = "k_metro_nat<bcc, R=4, 32-bit counts, 32 cells>"
+# radial densities (alpha^(1..4)) sampled every 10 sweeps of each T step
+ASRO_EVERY = 10
+
+
+def _alpha_summary(res, chain):
+    """Warren-Cowley alpha^(n)(a,b) at the last temperature, shells 1..R."""
+    from brawl_amd.lattice import warren_cowley
+    z = {1: [12, 6, 24, 12, 24, 8], 2: [8, 6, 12, 24, 8, 6, 24, 24, 24, 32]}[chain.lattice][: chain.R]
+    conc = chain.counts / chain.counts.sum()
+    a = warren_cowley(res.rho_of_T[:, :, :, -1], z, conc)
+    return [[[round(float(a[i, j, n]), 6) for j in range(chain.S)] for i in range(chain.S)]
+            for n in range(1, chain.R + 1)]
 
 
 def calibrate_copy():
@@ -290,7 +302,7 @@ def main():
         lat.set_config(start)
         return lat.metropolis_simulated_annealing(T=chain.T, delta_T=chain.delta_T,
                                                   T_steps=chain.T_steps, n_sweeps=chain.n_sweeps,
-                                                  seed=seed)
+                                                  seed=seed, asro_every=ASRO_EVERY)
 
     for w in range(args.warmup):
         r = step(1000 + w)
@@ -326,7 +338,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "plain: quaternary bcc anneal, bcc 32^3 cells (65536 sites), "
-                               "S=4 equiatomic, 4 shells, 3000->100 K in 50 steps x 200 sweeps",
+                               "S=4 equiatomic, 4 shells, 3000->100 K in 50 steps x 200 sweeps, "
+                               "radial densities sampled every 10 sweeps",
                    "form": "plain (one chain per call)", "trials_per_step": trials_per_launch,
                    "bytes_per_trial": bpt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s",
@@ -337,7 +350,8 @@ def main():
                     "E_per_atom_last_T": float(last.energies_of_T[-1]),
                     "C_max": float(np.max(last.C_of_T)),
                     "acceptance_first_last": [float(last.acceptance_of_T[0]),
-                                              float(last.acceptance_of_T[-1])]},
+                                              float(last.acceptance_of_T[-1])],
+                    "alpha_last_T_shells_1_to_4": _alpha_summary(last, chain)},
     }
     lat.close()
     if not args.no_batched:

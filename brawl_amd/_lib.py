@@ -26,7 +26,7 @@ EXPORTS = (
     "bwl_set_config", "bwl_get_config", "bwl_pair_histogram", "bwl_total_energy",
     "bwl_site_energy", "bwl_pair_energy", "bwl_genrand_seed", "bwl_genrand_get_state",
     "bwl_genrand_set_state", "bwl_initial_setup", "bwl_mc_steps", "bwl_metropolis_anneal",
-    "bwl_metropolis_anneal_batch", "bwl_lattice_shells", "bwl_radial_densities",
+    "bwl_metropolis_anneal_batch", "bwl_metropolis_anneal_rho", "bwl_lattice_shells", "bwl_radial_densities",
     "bwl_nested_sampling", "bwl_nested_sampling_batch", "bwl_wl_create", "bwl_wl_destroy",
     "bwl_wl_n_walkers", "bwl_wl_windows", "bwl_wl_hist_stride", "bwl_wl_initial_setup",
     "bwl_wl_set_walker", "bwl_wl_get_walker", "bwl_wl_genrand_get_state",
@@ -45,7 +45,7 @@ class LatticeDesc(C.Structure):
 class AnnealParams(C.Structure):
     _fields_ = [("T", C.c_double), ("delta_T", C.c_double), ("T_steps", C.c_int32),
                 ("n_sweeps", C.c_int32), ("burn_in_sweeps", C.c_int32),
-                ("reserved", C.c_int32), ("seed", C.c_uint64)]
+                ("asro_every", C.c_int32), ("seed", C.c_uint64)]
 
 
 class RunStats(C.Structure):
@@ -117,6 +117,7 @@ def load():
         "bwl_metropolis_anneal": (C.c_int, [vp, P(AnnealParams), dp, dp, dp, P(RunStats)]),
         "bwl_metropolis_anneal_batch": (C.c_int, [C.c_int32, P(vp), P(AnnealParams), P(dp), P(dp),
                                                   P(dp), P(RunStats), dp]),
+        "bwl_metropolis_anneal_rho": (C.c_int, [vp, P(AnnealParams), dp, dp, dp, dp, P(RunStats)]),
         "bwl_lattice_shells": (C.c_int, [vp, C.c_int32, dp]),
         "bwl_radial_densities": (C.c_int, [vp, C.c_int32, dp]),
         "bwl_nested_sampling": (C.c_int, [vp, P(NsParams), i64p, dp, dp, P(RunStats)]),

@@ -287,9 +287,38 @@ int bwl_mc_steps(bwl_ctx *c, double beta, int32_t nbr_swap, int64_t n_trials, in
 
 int bwl_metropolis_anneal(bwl_ctx *c, const bwl_anneal_params *p, double *energies_of_T,
                           double *C_of_T, double *acceptance_of_T, bwl_run_stats *stats) {
+  if (!p) return fail(BWL_ERR_ARG, "null argument");
+  bwl_anneal_params q = *p;
+  q.asro_every = 0;
   bwl_ctx *cs[1] = {c};
   double *e[1] = {energies_of_T}, *cv[1] = {C_of_T}, *ac[1] = {acceptance_of_T};
-  return bwl_metropolis_anneal_batch(1, cs, p, e, cv, ac, stats, nullptr);
+  return bwl_metropolis_anneal_batch(1, cs, &q, e, cv, ac, stats, nullptr);
+}
+
+int bwl_metropolis_anneal_rho(bwl_ctx *c, const bwl_anneal_params *p, double *energies_of_T,
+                              double *C_of_T, double *acceptance_of_T, double *rho_of_T,
+                              bwl_run_stats *stats) {
+  if (!c || !p || !energies_of_T || !C_of_T || !acceptance_of_T || !rho_of_T)
+    return fail(BWL_ERR_ARG, "null argument");
+  if (p->asro_every < 1) return fail(BWL_ERR_ARG, "asro_every must be positive");
+  bwl_ctx *cs[1] = {c};
+  AnnealPlan P;
+  int rc = anneal_plan(1, cs, p, P);
+  if (rc) return rc;
+  DevBuf scratch;
+  if (hipMalloc(&scratch.p, P.scratch_bytes) != hipSuccess)
+    return fail(BWL_ERR_NOMEM, "device allocation failed");
+  HIP_OK(hipEventRecord(c->ev0, c->stream));
+  rc = anneal_upload(P, (char *)scratch.p, c->stream);
+  if (rc) return rc;
+  rc = anneal_launch(P, &c->stream);
+  if (rc) return rc;
+  HIP_OK(hipEventRecord(c->ev1, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  double *e[1] = {energies_of_T}, *cv[1] = {C_of_T}, *ac[1] = {acceptance_of_T}, *rh[1] = {rho_of_T};
+  return anneal_collect(P, cs, e, cv, ac, stats, ms, c->stream, rh);
 }
 
 }  // extern "C"
@@ -359,7 +388,7 @@ int ctx_create_borrowed(const bwl_lattice_desc &d, const double *v_ex, char *mem
 namespace bwl {
 
 int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int &threads,
-                  size_t &lds) {
+                  size_t &lds, bool *natural) {
   const int lim = lds_limit();
   int rc = plan_parallel(g, plan, g_err);
   if (rc) return rc;
@@ -382,6 +411,7 @@ int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int
   const bool wide = use_nat ? g.S > 4 : g.S > 5;
   fn = par_kernel_for(g.lattice, g.R, g.n[0], use_nat, wide);
   if (!fn) return fail(BWL_ERR_UNSUPPORTED, "no kernel for this lattice");
+  if (natural) *natural = use_nat;
   return BWL_OK;
 }
 
@@ -405,8 +435,20 @@ int anneal_plan(int n, bwl_ctx *const *ctxs, const bwl_anneal_params *params, An
     NatGeo nat;
     int threads = 0;
     size_t lds = 0;
-    rc = anneal_kernel(g, it.plan, nat, it.fn, threads, lds);
+    bool natural = false;
+    rc = anneal_kernel(g, it.plan, nat, it.fn, threads, lds, &natural);
     if (rc) return rc;
+    if (p->asro_every < 0 || p->asro_every > p->n_sweeps)
+      return fail(BWL_ERR_ARG, "asro_every must be in 0..n_sweeps");
+    if (p->asro_every > 0) {
+      if (!natural)
+        return fail(BWL_ERR_UNSUPPORTED,
+                    "radial densities in the anneal need a cubic supercell held in lattice coordinates");
+      const int w = std::min(std::min(g.n[0], g.n[1]), std::min(g.n[2], 5));
+      if (g.rcut > w)
+        return fail(BWL_ERR_UNSUPPORTED,
+                    "an interaction shell reaches outside the radial-density window (supercell too small)");
+    }
     it.threads = threads;
     it.lds = lds;
     ParArgs &a = it.a;
@@ -438,6 +480,8 @@ int anneal_plan(int n, bwl_ctx *const *ctxs, const bwl_anneal_params *params, An
     a.chain_id = 0;  // the stream depends on the seed and position only, not on the batch
     a.phase0 = c->phase;
     a.e_trace = nullptr;
+    a.asro_every = p->asro_every;
+    a.rho_of_T = nullptr;
     it.T = (size_t)p->T_steps;
     it.off_muls = total;
     total += 256 * ((it.plan.muls.size() * 4 + 255) / 256);
@@ -445,6 +489,9 @@ int anneal_plan(int n, bwl_ctx *const *ctxs, const bwl_anneal_params *params, An
     total += 256 * ((3 * it.T * sizeof(double) + 255) / 256);
     it.off_cnt = total;
     total += 256;
+    it.rho_n = p->asro_every > 0 ? (size_t)g.S * g.S * (g.R + 1) * it.T : 0;
+    it.off_rho = total;
+    total += 256 * ((it.rho_n * sizeof(double) + 255) / 256);
     it.work = (double)g.n_atoms * (double)p->T_steps * (double)(p->n_sweeps + p->burn_in_sweeps);
   }
   // group chains that run the same kernel (one launch each, the block size
@@ -502,6 +549,7 @@ int anneal_upload(AnnealPlan &P, char *base, hipStream_t s) {
     it.a.C_of_T = it.a.energies_of_T + it.T;
     it.a.acc_of_T = it.a.C_of_T + it.T;
     it.a.counters = (long long *)(base + it.off_cnt);
+    it.a.rho_of_T = it.rho_n ? (double *)(base + it.off_rho) : nullptr;
     HIP_OK(hipMemcpyAsync(base + it.off_muls, it.plan.muls.data(), it.plan.muls.size() * 4,
                           hipMemcpyHostToDevice, s));
     HIP_OK(hipMemsetAsync(it.a.counters, 0, 3 * sizeof(long long), s));
@@ -527,7 +575,7 @@ int anneal_launch(const AnnealPlan &P, const hipStream_t *streams) {
 
 int anneal_collect(const AnnealPlan &P, bwl_ctx *const *ctxs, double *const *energies_of_T,
                    double *const *C_of_T, double *const *acceptance_of_T, bwl_run_stats *stats,
-                   double kernel_ms, hipStream_t s) {
+                   double kernel_ms, hipStream_t s, double *const *rho_of_T) {
   const size_t n = P.items.size();
   std::vector<std::vector<double>> out(n);
   std::vector<long long> cnt(3 * n);
@@ -537,6 +585,9 @@ int anneal_collect(const AnnealPlan &P, bwl_ctx *const *ctxs, double *const *ene
     HIP_OK(hipMemcpyAsync(out[i].data(), it.a.energies_of_T, 3 * it.T * sizeof(double),
                           hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(&cnt[3 * i], it.a.counters, 3 * sizeof(long long), hipMemcpyDeviceToHost, s));
+    if (rho_of_T && rho_of_T[i] && it.rho_n)
+      HIP_OK(hipMemcpyAsync(rho_of_T[i], it.a.rho_of_T, it.rho_n * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
   }
   HIP_OK(hipStreamSynchronize(s));
   for (size_t i = 0; i < n; ++i) {

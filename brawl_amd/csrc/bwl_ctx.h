@@ -97,7 +97,7 @@ struct AnnealPlan {
     size_t lds = 0;
     ParArgs a;
     ParPlan plan;
-    size_t T = 0, off_muls = 0, off_out = 0, off_cnt = 0;
+    size_t T = 0, off_muls = 0, off_out = 0, off_cnt = 0, off_rho = 0, rho_n = 0;
     double work = 0.0;
   };
   struct Group {
@@ -115,13 +115,13 @@ struct AnnealPlan {
 };
 // The parallel-sampler kernel variant, block size and LDS for a lattice.
 int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int &threads,
-                  size_t &lds);
+                  size_t &lds, bool *natural = nullptr);
 int anneal_plan(int n, bwl_ctx *const *ctxs, const bwl_anneal_params *params, AnnealPlan &P);
 int anneal_upload(AnnealPlan &P, char *scratch_dev, hipStream_t s);
 int anneal_launch(const AnnealPlan &P, const hipStream_t *streams);
 int anneal_collect(const AnnealPlan &P, bwl_ctx *const *ctxs, double *const *energies_of_T,
                    double *const *C_of_T, double *const *acceptance_of_T, bwl_run_stats *stats,
-                   double kernel_ms, hipStream_t s);
+                   double kernel_ms, hipStream_t s, double *const *rho_of_T = nullptr);
 
 // Device buffer freed at scope exit.
 struct DevBuf {

@@ -66,6 +66,8 @@ struct ParArgs {
   // outputs
   double *energies_of_T, *C_of_T, *acc_of_T;
   double *e_trace;       // optional [T_steps][n_sweeps]
+  int32_t asro_every;    // sampled sweeps between radial-density samples (0: none)
+  double *rho_of_T;      // (S, S, R+1, T_steps) when asro_every > 0
   long long *counters;   // [0] = next round counter, [1] += accepted, [2] += trials
 };
 

@@ -378,8 +378,13 @@ __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ c
   o += sizeof(int32_t) * ((Z + 3) & ~3);
   double *red = (double *)(smem + o);
   unsigned long long *redu = (unsigned long long *)(smem + o);
+  o += sizeof(double) * (blockDim.x / 64 + 1);
+  // shell pair counts H[a][b][n] summed over the radial-density samples of
+  // the current temperature step
+  unsigned long long *hacc = (unsigned long long *)(smem + o);
 
   for (int i = tid; i < Z; i += blockDim.x) offl[i] = C.nat.off[i];
+  for (int i = tid; i < S * S * R; i += blockDim.x) hacc[i] = 0ull;
   for (int i = tid; i < S * S * R * S1; i += blockDim.x) {
     int s = i % S1, n = (i / S1) % R, be = (i / (S1 * R)) % S, al = i / (S1 * R * S);
     const double *V = C.vex;
@@ -545,9 +550,77 @@ __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ c
           step_Esq = step_Esq + E * E;
           if (C.e_trace) C.e_trace[(long long)t * C.n_sweeps + (sw - C.burn_sweeps)] = E;
         }
+        if (C.asro_every > 0 && (sw - C.burn_sweeps + 1) % C.asro_every == 0) {
+          // radial-density sample: every site's shell counts, summed per
+          // (species, shell) over each wavefront, added to H by lane 0
+          const int nsites = N.D0 * N.D1 * N.D2;
+          const int lane = tid & 63;
+          for (int i0 = active ? (tid & ~63) : nsites; i0 < nsites; i0 += nthr) {
+            const int i = i0 + lane;
+            const bool valid = i < nsites;
+            int sp = -1;
+            CW cnt[R];
+#pragma unroll
+            for (int n = 0; n < R; ++n) cnt[n] = 0;
+            if (valid) {
+              int c = i / (N.D0 * N.D1);
+              int r2 = i - c * N.D0 * N.D1;
+              int b = r2 / N.D0;
+              int a = r2 - b * N.D0;
+              int ad = nat_addr(N, a, b, c);
+              sp = lat[ad] / FIELD;
+              if constexpr (NC > 0)
+                count_c<LAT, R, CW, NC>(lat, ad, cnt);
+              else
+                count_nat<LAT, R, CW>(lat, ad, off, cnt);
+            }
+            for (int x = 0; x < S; ++x) {
+#pragma unroll
+              for (int n = 0; n < R; ++n) {
+                const uint64_t w = sp == x ? (uint64_t)cnt[n] : 0ull;
+                // 8-bit fields to 16-bit fields (a wavefront sum stays < 2^16)
+                uint64_t lo = 0, hi = 0;
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                  lo |= ((w >> (8 * f)) & 0xffull) << (16 * f);
+                  hi |= ((w >> (8 * (f + 4))) & 0xffull) << (16 * f);
+                }
+#pragma unroll
+                for (int m = 32; m > 0; m >>= 1) {
+                  lo += __shfl_xor(lo, m);
+                  if (sizeof(CW) > 4) hi += __shfl_xor(hi, m);
+                }
+                if (lane == 0)
+                  for (int s2 = 0; s2 < S; ++s2) {
+                    const uint64_t v = s2 < 4 ? (lo >> (16 * s2)) & 0xffffull : (hi >> (16 * (s2 - 4))) & 0xffffull;
+                    if (v) atomicAdd(&hacc[(x * S + s2) * R + n], (unsigned long long)v);
+                  }
+              }
+            }
+          }
+        }
       }
     }
     unsigned long long acc_sum = block_sum_u64(acc_local, redu);
+    if (C.asro_every > 0) {
+      // rho(a, b, l) = mean over samples of H(a, b, l) / N_a, shell 0 on-site
+      // (the reference's radial_densities layout, wc_range = R + 1)
+      const int ns_asro = C.n_sweeps / C.asro_every;
+      for (int i = tid; i < S * S * (R + 1); i += blockDim.x) {
+        const int a = i % S, b = (i / S) % S, l = i / (S * S);
+        unsigned long long tot = 0;
+        for (int b2 = 0; b2 < S; ++b2) tot += hacc[(a * S + b2) * R + 0];
+        const double na = (double)(tot / (unsigned long long)(shell_size<LAT>(0) * (ns_asro > 0 ? ns_asro : 1)));
+        double v;
+        if (l == 0)
+          v = (a == b ? na : 0.0) / na;
+        else
+          v = ((double)hacc[(a * S + b) * R + (l - 1)] / na) / (double)ns_asro;
+        C.rho_of_T[(size_t)t * S * S * (R + 1) + i] = v;
+      }
+      __syncthreads();
+      for (int i = tid; i < S * S * R; i += blockDim.x) hacc[i] = 0ull;
+    }
     if (tid == 0) {
       const double ns = (double)C.n_sweeps;
       const double na = (double)g.n_atoms;
@@ -798,7 +871,7 @@ size_t par_lds_bytes(const Geo &g, const NatGeo *nat, int threads) {
   if (nat)
     return (size_t)((nat->lds_bytes + 15) & ~15) + sizeof(double) * g.S * g.S * g.R * S1 +
            sizeof(double) * g.S * g.R * g.S + sizeof(RoundP) * kRoundBatch +
-           sizeof(int32_t) * ((g.z + 3) & ~3) + red;
+           sizeof(int32_t) * ((g.z + 3) & ~3) + red + sizeof(unsigned long long) * g.S * g.S * g.R;
   return (size_t)((g.lds_bytes + 15) & ~15) + sizeof(double) * g.S * g.S * g.R + red +
          sizeof(int32_t) * g.nsub * g.z;
 }

@@ -30,7 +30,7 @@ module brawl_hip
 
   type, bind(C) :: bwl_anneal_params
     real(c_double) :: T, delta_T
-    integer(c_int32_t) :: T_steps, n_sweeps, burn_in_sweeps, reserved
+    integer(c_int32_t) :: T_steps, n_sweeps, burn_in_sweeps, asro_every
     integer(c_int64_t) :: seed
   end type bwl_anneal_params
 
@@ -161,6 +161,15 @@ module brawl_hip
       type(c_ptr), value :: ctx
       type(bwl_anneal_params), intent(in) :: p
       real(c_double), intent(out) :: energies_of_T(*), C_of_T(*), acceptance_of_T(*)
+      type(bwl_run_stats), intent(out) :: stats
+    end function
+
+    integer(c_int) function bwl_metropolis_anneal_rho(ctx, p, energies_of_T, C_of_T, acceptance_of_T, &
+                                                      rho_of_T, stats) bind(C, name="bwl_metropolis_anneal_rho")
+      import :: c_int, c_ptr, c_double, bwl_anneal_params, bwl_run_stats
+      type(c_ptr), value :: ctx
+      type(bwl_anneal_params), intent(in) :: p
+      real(c_double), intent(out) :: energies_of_T(*), C_of_T(*), acceptance_of_T(*), rho_of_T(*)
       type(bwl_run_stats), intent(out) :: stats
     end function
 

@@ -50,6 +50,19 @@ class AnnealResult:
     trials: int
     accepted: int
     kernel_ms: float
+    rho_of_T: np.ndarray = None  # (S, S, R+1, T_steps) radial densities, if sampled
+
+
+def warren_cowley(rho: np.ndarray, shell_sizes, concentrations) -> np.ndarray:
+    """alpha(a, b, l) = 1 - rho(a, b, l) / (z_l c_b) for shells l >= 1 of a
+    radial-density array rho(S, S, wc); shell 0 (on-site) is left as 0."""
+    rho = np.asarray(rho)
+    S = rho.shape[0]
+    a = np.zeros_like(rho)
+    for l in range(1, rho.shape[2]):
+        for b in range(S):
+            a[:, b, l, ...] = 1.0 - rho[:, b, l, ...] / (shell_sizes[l - 1] * concentrations[b])
+    return a
 
 
 class Lattice:
@@ -184,14 +197,24 @@ class Lattice:
 
     # -- parallel sampler -------------------------------------------------
     def metropolis_simulated_annealing(self, *, T, delta_T=0.0, T_steps=1, n_sweeps,
-                                       burn_in_sweeps=0, seed=20261202) -> AnnealResult:
+                                       burn_in_sweeps=0, seed=20261202,
+                                       asro_every=0) -> AnnealResult:
+        """With asro_every > 0, radial densities are averaged over samples
+        taken every asro_every sampled sweeps (result.rho_of_T)."""
         p = AnnealParams(float(T), float(delta_T), int(T_steps), int(n_sweeps),
-                         int(burn_in_sweeps), 0, int(seed) & (2**64 - 1))
+                         int(burn_in_sweeps), int(asro_every), int(seed) & (2**64 - 1))
         e = np.zeros(T_steps); c = np.zeros(T_steps); a = np.zeros(T_steps)
         st = RunStats()
-        check(self._L.bwl_metropolis_anneal(self._h, C.byref(p), _p(e, C.c_double),
-                                            _p(c, C.c_double), _p(a, C.c_double), C.byref(st)))
-        return AnnealResult(e, c, a, int(st.trials), int(st.accepted), float(st.kernel_ms))
+        rho = None
+        if asro_every:
+            rho = np.zeros((self.S, self.S, self.R + 1, T_steps), order="F")
+            check(self._L.bwl_metropolis_anneal_rho(self._h, C.byref(p), _p(e, C.c_double),
+                                                    _p(c, C.c_double), _p(a, C.c_double),
+                                                    _p(rho, C.c_double), C.byref(st)))
+        else:
+            check(self._L.bwl_metropolis_anneal(self._h, C.byref(p), _p(e, C.c_double),
+                                                _p(c, C.c_double), _p(a, C.c_double), C.byref(st)))
+        return AnnealResult(e, c, a, int(st.trials), int(st.accepted), float(st.kernel_ms), rho)
 
 
 def anneal_batch(lattices, params):
@@ -224,5 +247,5 @@ def anneal_batch(lattices, params):
     return res, float(mk.value)
 
 
-__all__ = ["Lattice", "BrawlError", "AnnealResult", "anneal_batch", "beta_of", "FCC", "BCC",
-           "K_B_IN_RY", "RY_TO_EV"]
+__all__ = ["Lattice", "BrawlError", "AnnealResult", "anneal_batch", "beta_of", "warren_cowley",
+           "FCC", "BCC", "K_B_IN_RY", "RY_TO_EV"]

@@ -133,7 +133,8 @@ typedef struct {
   int32_t T_steps;
   int32_t n_sweeps;    /* sweeps per T step (samples per T step) */
   int32_t burn_in_sweeps; /* unsampled sweeps before each T step's samples */
-  int32_t reserved;
+  int32_t asro_every;  /* radial-density sample every asro_every sampled sweeps
+                          (0: none; bwl_metropolis_anneal_rho only) */
   uint64_t seed;
 } bwl_anneal_params;
 
@@ -146,6 +147,19 @@ typedef struct {
 int bwl_metropolis_anneal(bwl_ctx *ctx, const bwl_anneal_params *p,
                           double *energies_of_T, double *C_of_T,
                           double *acceptance_of_T, bwl_run_stats *stats);
+
+/* As bwl_metropolis_anneal, also averaging the radial densities
+ * (analytics.f90:293-404; metropolis.F90:377-384, 439-441) over samples
+ * taken every p->asro_every sampled sweeps (1 <= asro_every <= n_sweeps):
+ * rho_of_T is FP64 (S, S, R+1, T_steps), Fortran layout, the reference's
+ * rho_of_T with wc_range = R+1 (shell 0 on-site, shells 1..R the
+ * interaction shells).  Counted from the device's integer shell pair
+ * counts; equal to radial_densities when every shell offset lies in the
+ * reference's +-min(n,5) window (BWL_ERR_UNSUPPORTED otherwise, and for
+ * supercells that are not cubic). */
+int bwl_metropolis_anneal_rho(bwl_ctx *ctx, const bwl_anneal_params *p, double *energies_of_T,
+                              double *C_of_T, double *acceptance_of_T, double *rho_of_T,
+                              bwl_run_stats *stats);
 
 /* Batched form: n independent chains (contexts on one device, uneven
  * lattice sizes, compositions and schedules) annealed in as few launches

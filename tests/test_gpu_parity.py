@@ -199,33 +199,66 @@ def test_parallel_sampler_conserves_composition_and_energy(hip, oracle):
 
 
 def test_parallel_sampler_statistics_match_reference_sampler(hip, oracle):
-    """The small config at 600 K: <E>/N from the parallel sampler agrees with
-    the reference's sequential sampler (the oracle) within 2 sigma of the
-    difference, both equilibrated from the same start."""
+    """The small config at 600 K: <E>/N, C and the Warren-Cowley alpha^(1)
+    of the unlike pair from the parallel sampler agree with the reference's
+    sequential sampler (the oracle) within 2 sigma of the difference, both
+    equilibrated from the same kind of start; radial densities are sampled
+    every sweep by both."""
     from brawl_amd import workloads
+    from brawl_amd.lattice import warren_cowley
     c = workloads.config1()
     ref = oracle.OracleLattice(c.lattice, c.n, c.v_ex)
-    beta = hip.beta_of(c.T)
+    conc = c.counts / c.counts.sum()
+    z = (12, 6)                         # fcc shells 1 and 2
     # reference sampler (oracle): 6 chains x (200 burn-in + 300 sampled sweeps)
-    e_ref = []
+    e_ref, c_ref, a_ref = [], [], []
     for k in range(6):
         gen = oracle.Generator(110179 + 11 * k)
         cfg = ref.initial_setup(gen, species_numbers=c.counts)
         r = ref.anneal(cfg, gen, T=c.T, n_mc_steps=300 * c.n_atoms, n_sample_steps=c.n_atoms,
-                       burn_in_start=True, n_burn_in_steps=200 * c.n_atoms)
+                       burn_in_start=True, n_burn_in_steps=200 * c.n_atoms, calculate_asro=True,
+                       n_sample_steps_asro=c.n_atoms, wc_range=3)
         e_ref.append(r["energies"][0])
-    e_gpu = []
+        c_ref.append(r["C"][0])
+        a_ref.append(warren_cowley(r["rho"][:, :, :, 0], z, conc)[0, 1, 1])
+    e_gpu, c_gpu, a_gpu = [], [], []
     for k in range(8):
         cfg = ref.initial_setup(oracle.Generator(999 + k), species_numbers=c.counts)
         with hip.Lattice(c.lattice, c.n, c.v_ex) as L:
             L.set_config(cfg)
-            r = L.metropolis_simulated_annealing(T=c.T, n_sweeps=300, burn_in_sweeps=200, seed=k + 1)
+            r = L.metropolis_simulated_annealing(T=c.T, n_sweeps=300, burn_in_sweeps=200, seed=k + 1,
+                                                 asro_every=1)
             e_gpu.append(r.energies_of_T[0])
-    e_ref, e_gpu = np.array(e_ref), np.array(e_gpu)
-    diff = e_gpu.mean() - e_ref.mean()
-    sigma = np.sqrt(e_gpu.var(ddof=1) / len(e_gpu) + e_ref.var(ddof=1) / len(e_ref))
-    _say("<E>/N gpu", e_gpu.mean(), "ref", e_ref.mean(), "diff/sigma", diff / sigma)
-    assert abs(diff) <= 2.0 * sigma + 1e-9
+            c_gpu.append(r.C_of_T[0])
+            a_gpu.append(warren_cowley(r.rho_of_T[:, :, :, 0], z, conc)[0, 1, 1])
+    for name, g_, r_ in (("<E>/N", e_gpu, e_ref), ("C", c_gpu, c_ref), ("alpha1_12", a_gpu, a_ref)):
+        g_, r_ = np.array(g_), np.array(r_)
+        diff = g_.mean() - r_.mean()
+        sigma = np.sqrt(g_.var(ddof=1) / len(g_) + r_.var(ddof=1) / len(r_))
+        _say(name, "gpu", g_.mean(), "ref", r_.mean(), "diff/sigma", diff / sigma)
+        assert abs(diff) <= 2.0 * sigma + 1e-9, name
+
+
+@pytest.mark.parametrize("lat,n,S,R", [(2, (8, 8, 8), 4, 4), (1, (6, 6, 6), 3, 3), (2, (5, 5, 5), 5, 2)])
+def test_anneal_radial_densities_equal_recount(hip, oracle, lat, n, S, R):
+    """One sweep, one sample: the anneal's rho_of_T equals the radial
+    densities of its final configuration (integer counts, bit-exact), and
+    the on-site shell is 1 for every species."""
+    from brawl_amd import workloads
+    v = workloads.random_v(S, [25.0 / k for k in range(1, R + 1)], 31 + S)
+    ref = oracle.OracleLattice(lat, n, v)
+    cfg = ref.initial_setup(oracle.Generator(110179 + S),
+                            species_numbers=workloads.dirichlet_counts(ref.n_atoms, S,
+                                                                       np.random.default_rng(S)))
+    with hip.Lattice(lat, n, v) as L:
+        L.set_config(cfg)
+        r = L.metropolis_simulated_annealing(T=900.0, n_sweeps=1, seed=5, asro_every=1)
+        final = L.get_config()
+        rho_dev = L.radial_densities(R + 1)
+    rho_ref = ref.radial_densities(final, R + 1, ref.lattice_shells(final, R + 1))
+    assert np.array_equal(r.rho_of_T[:, :, :, 0], rho_ref)
+    assert np.array_equal(rho_dev, rho_ref)
+    assert np.all(np.diag(r.rho_of_T[:, :, 0, 0]) == 1.0)
 
 
 @pytest.mark.parametrize("lat,n,R", [(1, (8, 8, 8), 2), (2, (8, 8, 8), 4), (2, (7, 5, 6), 3)])
