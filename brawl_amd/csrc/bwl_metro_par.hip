@@ -347,6 +347,116 @@ __device__ __forceinline__ void site_of(int ax, int pa, int pb, int pc, int n,
 // (128 + c_a - c_b) per byte lane of a packed count word
 __device__ __forceinline__ uint32_t biased_diff(uint32_t a, uint32_t b) { return a + 0x80808080u - b; }
 
+// One radial-density sample of the lattice: H[a][b][n] += number of
+// shell-n neighbours of species b around sites of species a.  Each lane
+// widens its sites' packed 8-bit counts to 16-bit fields and sums them per
+// (species, shell); the lanes of a wavefront are then summed with shuffles
+// and lane 0 adds the totals to H (LDS, 64-bit).  With at most 4 species the
+// per-lane sums stay in registers over all of the lane's sites (flushed
+// before a 16-bit field could overflow); otherwise every 64 sites.
+template <int LAT, int R, typename CW, int NC>
+__device__ __forceinline__ void asro_sample(const int8_t *lat, const ND N, const int32_t *offl, int S,
+                                         int tid, int nthr, bool active, unsigned long long *hacc) {
+  constexpr int FIELD = 8;
+  constexpr int Z = shell_begin<LAT>(R);
+  int off[NC ? 1 : Z];
+#pragma unroll
+  for (int t = 0; t < (NC ? 1 : Z); ++t) off[t] = NC ? 0 : offl[t];
+  constexpr bool kRegs = sizeof(CW) == 4 && (R <= 3 || NC > 0);
+  constexpr int XS = kRegs ? 4 : 8;
+  constexpr int kMaxField = shell_begin<LAT>(R);  // largest count a site can add
+  const int nsites = N.D0 * N.D1 * N.D2;
+  const int lane = tid & 63;
+  uint64_t acc[kRegs ? 4 : 1][kRegs ? R : 1];
+  int pending = 0;
+  auto flush = [&]() {
+    if constexpr (kRegs) {
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int n = 0; n < R; ++n) {
+          uint64_t v = acc[x][n];
+#pragma unroll
+          for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+          if (lane == 0 && x < S)
+            for (int s2 = 0; s2 < S; ++s2) {
+              const unsigned long long f = (v >> (16 * s2)) & 0xffffull;
+              if (f) atomicAdd(&hacc[(x * S + s2) * R + n], f);
+            }
+          acc[x][n] = 0;
+        }
+    }
+  };
+  if constexpr (kRegs) {
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int n = 0; n < R; ++n) acc[x][n] = 0;
+  }
+  for (int i0 = active ? (tid & ~63) : nsites; i0 < nsites; i0 += nthr) {
+    const int i = i0 + lane;
+    const bool valid = i < nsites;
+    int sp = -1;
+    CW cnt[R];
+#pragma unroll
+    for (int n = 0; n < R; ++n) cnt[n] = 0;
+    if (valid) {
+      int c = i / (N.D0 * N.D1);
+      int r2 = i - c * N.D0 * N.D1;
+      int b = r2 / N.D0;
+      int a = r2 - b * N.D0;
+      int ad = nat_addr(N, a, b, c);
+      sp = lat[ad] / FIELD;
+      if constexpr (NC > 0)
+        count_c<LAT, R, CW, NC>(lat, ad, cnt);
+      else
+        count_nat<LAT, R, CW>(lat, ad, off, cnt);
+    }
+    if constexpr (kRegs) {
+#pragma unroll
+      for (int n = 0; n < R; ++n) {
+        const uint64_t w = (uint64_t)cnt[n];
+        const uint64_t e = (w & 0xffull) | ((w & 0xff00ull) << 8) | ((w & 0xff0000ull) << 16) |
+                           ((w & 0xff000000ull) << 24);
+#pragma unroll
+        for (int x = 0; x < 4; ++x) acc[x][n] += sp == x ? e : 0ull;
+      }
+      // a lane's 16-bit field holds at most 65535; the wavefront sum of
+      // fields must too: flush before 64 lanes x pending x kMaxField could
+      if (++pending * 64 * kMaxField > 60000) {
+        flush();
+        pending = 0;
+      }
+    } else {
+      for (int x = 0; x < S && x < XS; ++x) {
+#pragma unroll
+        for (int n = 0; n < R; ++n) {
+          const uint64_t w = sp == x ? (uint64_t)cnt[n] : 0ull;
+          uint64_t lo = 0, hi = 0;
+#pragma unroll
+          for (int f = 0; f < 4; ++f) {
+            lo |= ((w >> (8 * f)) & 0xffull) << (16 * f);
+            hi |= ((w >> (8 * (f + 4))) & 0xffull) << (16 * f);
+          }
+#pragma unroll
+          for (int m = 32; m > 0; m >>= 1) {
+            lo += __shfl_xor(lo, m);
+            hi += __shfl_xor(hi, m);
+          }
+          if (lane == 0)
+            for (int s2 = 0; s2 < S; ++s2) {
+              const uint64_t v = s2 < 4 ? (lo >> (16 * s2)) & 0xffffull : (hi >> (16 * (s2 - 4))) & 0xffffull;
+              if (v) atomicAdd(&hacc[(x * S + s2) * R + n], (unsigned long long)v);
+            }
+        }
+      }
+    }
+  }
+  if constexpr (kRegs) {
+    if (active) flush();
+  }
+}
+
 template <int LAT, int R, typename CW, int NC>
 __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ chains) {
   extern __shared__ __align__(16) uint8_t smem[];
@@ -550,55 +660,8 @@ __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ c
           step_Esq = step_Esq + E * E;
           if (C.e_trace) C.e_trace[(long long)t * C.n_sweeps + (sw - C.burn_sweeps)] = E;
         }
-        if (C.asro_every > 0 && (sw - C.burn_sweeps + 1) % C.asro_every == 0) {
-          // radial-density sample: every site's shell counts, summed per
-          // (species, shell) over each wavefront, added to H by lane 0
-          const int nsites = N.D0 * N.D1 * N.D2;
-          const int lane = tid & 63;
-          for (int i0 = active ? (tid & ~63) : nsites; i0 < nsites; i0 += nthr) {
-            const int i = i0 + lane;
-            const bool valid = i < nsites;
-            int sp = -1;
-            CW cnt[R];
-#pragma unroll
-            for (int n = 0; n < R; ++n) cnt[n] = 0;
-            if (valid) {
-              int c = i / (N.D0 * N.D1);
-              int r2 = i - c * N.D0 * N.D1;
-              int b = r2 / N.D0;
-              int a = r2 - b * N.D0;
-              int ad = nat_addr(N, a, b, c);
-              sp = lat[ad] / FIELD;
-              if constexpr (NC > 0)
-                count_c<LAT, R, CW, NC>(lat, ad, cnt);
-              else
-                count_nat<LAT, R, CW>(lat, ad, off, cnt);
-            }
-            for (int x = 0; x < S; ++x) {
-#pragma unroll
-              for (int n = 0; n < R; ++n) {
-                const uint64_t w = sp == x ? (uint64_t)cnt[n] : 0ull;
-                // 8-bit fields to 16-bit fields (a wavefront sum stays < 2^16)
-                uint64_t lo = 0, hi = 0;
-#pragma unroll
-                for (int f = 0; f < 4; ++f) {
-                  lo |= ((w >> (8 * f)) & 0xffull) << (16 * f);
-                  hi |= ((w >> (8 * (f + 4))) & 0xffull) << (16 * f);
-                }
-#pragma unroll
-                for (int m = 32; m > 0; m >>= 1) {
-                  lo += __shfl_xor(lo, m);
-                  if (sizeof(CW) > 4) hi += __shfl_xor(hi, m);
-                }
-                if (lane == 0)
-                  for (int s2 = 0; s2 < S; ++s2) {
-                    const uint64_t v = s2 < 4 ? (lo >> (16 * s2)) & 0xffffull : (hi >> (16 * (s2 - 4))) & 0xffffull;
-                    if (v) atomicAdd(&hacc[(x * S + s2) * R + n], (unsigned long long)v);
-                  }
-              }
-            }
-          }
-        }
+        if (C.asro_every > 0 && (sw - C.burn_sweeps + 1) % C.asro_every == 0)
+          asro_sample<LAT, R, CW, NC>(lat, N, offl, S, tid, nthr, active, hacc);
       }
     }
     unsigned long long acc_sum = block_sum_u64(acc_local, redu);
