@@ -195,9 +195,11 @@ int bwl_nested_sampling_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_ns_para
     if (2 * c->g.R > 64) return fail(BWL_ERR_UNSUPPORTED, "too many shells");
     lds = std::max(lds, ns_lds_bytes(c->g, p->n_walkers));
     offs[(size_t)i] = total;
+    const size_t ref = (size_t)c->g.m[0] * c->g.m[1] * c->g.m[2];
     total += align256((size_t)c->g.compact_bytes * p->n_walkers) +
              align256(sizeof(double) * 2 * p->n_walkers) + align256(624 * 4 + 64) +
-             align256(sizeof(long long) * 3) + align256(sizeof(double) * p->n_iter);
+             align256(sizeof(long long) * 3) + align256(sizeof(double) * p->n_iter) +
+             align256(ref * p->n_walkers);
   }
   if (lds > (size_t)lim) return fail(BWL_ERR_UNSUPPORTED, "lattice and walker energies do not fit in LDS");
   bwl_ctx *c0 = ctxs[0];
@@ -227,6 +229,8 @@ int bwl_nested_sampling_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_ns_para
     a.counters = (long long *)q;
     q += align256(sizeof(long long) * 3);
     a.e_limits = (double *)q;
+    q += align256(sizeof(double) * p->n_iter);
+    int8_t *staging = (int8_t *)q;  // the walkers in the reference layout
     a.n_walkers = p->n_walkers;
     a.n_steps = p->n_steps;
     a.n_iter = p->n_iter;
@@ -236,17 +240,17 @@ int bwl_nested_sampling_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_ns_para
     // walkers from the run's MT19937 stream: initial_setup, then one draw
     // for the energy jitter (nested_sampling.f90:88-96)
     if (c->mt.mti == 625) c->mt.seed(5489u);
-    std::vector<int8_t> cfg((size_t)g.m[0] * g.m[1] * g.m[2]);
+    const size_t ref = (size_t)g.m[0] * g.m[1] * g.m[2];
+    std::vector<int8_t> cfgs(ref * (size_t)p->n_walkers);
     rnde[(size_t)i].resize((size_t)p->n_walkers);
     for (int w = 0; w < p->n_walkers; ++w) {
-      int rc = initial_setup_host(g, species_numbers[i], concentrations[i], c->mt, cfg.data(), g_err);
+      int rc = initial_setup_host(g, species_numbers[i], concentrations[i], c->mt,
+                                  cfgs.data() + ref * (size_t)w, g_err);
       if (rc) return rc;
       rnde[(size_t)i][(size_t)w] = c->mt.genrand();
-      rc = upload_packed(g, cfg.data(), c->d_ref, a.walkers + (size_t)g.compact_bytes * w, c->stream);
-      if (rc) return rc;
-      // the staging buffer is reused: finish each upload before the next
-      HIP_OK(hipStreamSynchronize(c->stream));
     }
+    HIP_OK(hipMemcpyAsync(staging, cfgs.data(), cfgs.size(), hipMemcpyHostToDevice, c0->stream));
+    HIP_OK(launch_pack_many(staging, a.walkers, g, p->n_walkers, c0->stream));
     HIP_OK(hipMemcpyAsync((void *)a.rnde, rnde[(size_t)i].data(), sizeof(double) * p->n_walkers,
                           hipMemcpyHostToDevice, c0->stream));
     HIP_OK(hipMemcpyAsync(a.mt, c->mt.mt, 624 * 4, hipMemcpyHostToDevice, c0->stream));

@@ -137,6 +137,9 @@ struct AsroArgs {
 constexpr int kMaxAsroShells = 16;
 
 hipError_t launch_pack(const int8_t *ref, int8_t *compact, const Geo &g, hipStream_t s);
+// `count` reference-layout lattices back to back -> compact lattices back to back
+hipError_t launch_pack_many(const int8_t *ref, int8_t *compact, const Geo &g, int count,
+                            hipStream_t s);
 hipError_t launch_unpack(const int8_t *compact, int8_t *ref, const Geo &g, hipStream_t s);
 size_t hist_lds_bytes(const Geo &g, int threads);
 hipError_t launch_histogram(const HistArgs *args_dev, int n, const Geo &g_max, int threads,

@@ -20,6 +20,9 @@ namespace bwl {
 // layout conversion
 // ------------------------------------------------------------------
 __global__ void k_pack(const int8_t *__restrict__ ref, int8_t *__restrict__ compact, Geo g) {
+  // blockIdx.y selects one of several lattices laid out back to back
+  ref += (size_t)blockIdx.y * g.m[0] * g.m[1] * g.m[2];
+  compact += (size_t)blockIdx.y * g.compact_bytes;
   const int cells = g.n[0] * g.n[1] * g.n[2];
   const int total = g.nsub * cells;
   for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total;
@@ -209,10 +212,20 @@ __global__ __launch_bounds__(64) void k_metro_seq(SeqArgs A) {
 // launchers
 // ------------------------------------------------------------------
 hipError_t launch_pack(const int8_t *ref, int8_t *compact, const Geo &g, hipStream_t s) {
+  return launch_pack_many(ref, compact, g, 1, s);
+}
+
+hipError_t launch_pack_many(const int8_t *ref, int8_t *compact, const Geo &g, int count,
+                            hipStream_t s) {
   int total = g.compact_bytes;
   int blocks = (total + 255) / 256;
   if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(k_pack, dim3(blocks), dim3(256), 0, s, ref, compact, g);
+  for (int first = 0; first < count; first += 65535) {
+    const int n = count - first < 65535 ? count - first : 65535;
+    hipLaunchKernelGGL(k_pack, dim3(blocks, n), dim3(256), 0, s,
+                       ref + (size_t)first * g.m[0] * g.m[1] * g.m[2],
+                       compact + (size_t)first * g.compact_bytes, g);
+  }
   return hipGetLastError();
 }
 

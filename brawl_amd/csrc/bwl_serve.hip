@@ -369,8 +369,7 @@ int launch_ns_group(bwl_server *s, std::vector<Request *> &grp, int slot) {
     a.n_at = g.n[0] * g.n[1] * g.n[2] * 1 * g.S;  // nested_sampling.f90:82
     a.max_redraws = 1LL << 24;
     HIP_OK(hipMemcpyAsync(staging, r->walkers_ref.data(), ref * W, hipMemcpyHostToDevice, st));
-    for (size_t w = 0; w < W; ++w)
-      HIP_OK(launch_pack(staging + ref * w, a.walkers + (size_t)g.compact_bytes * w, g, st));
+    HIP_OK(launch_pack_many(staging, a.walkers, g, (int)W, st));
     HIP_OK(hipMemcpyAsync((void *)a.rnde, r->rnde.data(), sizeof(double) * W, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(a.mt, r->mt, 624 * 4, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(a.mt_index, &r->mti, 4, hipMemcpyHostToDevice, st));
@@ -430,8 +429,7 @@ int launch_wl_group(bwl_server *s, std::vector<Request *> &grp, int slot) {
     q += al(sizeof(double) * W * r->hstride);
     long long *acc = (long long *)q;
     HIP_OK(hipMemcpyAsync(staging, r->walkers_ref.data(), ref * W, hipMemcpyHostToDevice, st));
-    for (size_t w = 0; w < W; ++w)
-      HIP_OK(launch_pack(staging + ref * w, walkers + (size_t)g.compact_bytes * w, g, st));
+    HIP_OK(launch_pack_many(staging, walkers, g, (int)W, st));
     HIP_OK(hipMemcpyAsync(mt, r->wl_mt.data(), W * 624 * 4, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(mti, r->wl_mti.data(), W * 4, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(edges, r->edges.data(), sizeof(double) * (r->wp.bins + 1),
