@@ -103,11 +103,9 @@ __device__ inline void init_pair_lanes(const Geo &g, PairLanes &pl) {
     pl.shell[k] = n;
     pl.end_mask[k] = __ballot(t < 2 * z && last);
   }
+  // straight-line sums for every fcc shell count and bcc up to 6 shells
   const int key = g.lattice * 16 + g.R;
-  pl.fixed = (key == 16 + 2 || key == 16 + 3 || key == 16 + 4 || key == 32 + 2 || key == 32 + 3 ||
-              key == 32 + 4)
-                 ? key
-                 : 0;
+  pl.fixed = (g.lattice == 1 && g.R <= 6) || (g.lattice == 2 && g.R <= 6) ? key : 0;
 }
 
 // Energies of sites A and B in the reference's FP64 order (pair_energy,
@@ -136,12 +134,18 @@ __device__ inline double pair_energy_lanes(const int8_t *lat, const Geo &g, cons
     }
   }
   switch (pl.fixed) {
+    case 16 + 1: return fixed_pair_sum<1, 1>(vals, eA_out);
     case 16 + 2: return fixed_pair_sum<1, 2>(vals, eA_out);
     case 16 + 3: return fixed_pair_sum<1, 3>(vals, eA_out);
     case 16 + 4: return fixed_pair_sum<1, 4>(vals, eA_out);
+    case 16 + 5: return fixed_pair_sum<1, 5>(vals, eA_out);
+    case 16 + 6: return fixed_pair_sum<1, 6>(vals, eA_out);
+    case 32 + 1: return fixed_pair_sum<2, 1>(vals, eA_out);
     case 32 + 2: return fixed_pair_sum<2, 2>(vals, eA_out);
     case 32 + 3: return fixed_pair_sum<2, 3>(vals, eA_out);
     case 32 + 4: return fixed_pair_sum<2, 4>(vals, eA_out);
+    case 32 + 5: return fixed_pair_sum<2, 5>(vals, eA_out);
+    case 32 + 6: return fixed_pair_sum<2, 6>(vals, eA_out);
     default: break;
   }
   double eA = 0.0, e = 0.0, es = 0.0;
