@@ -265,6 +265,31 @@ def served(args):
                        "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))}}
 
 
+def wang_landau_walkers(calls: int = 10):
+    """Config 4's walker set (16 windows x 8 walkers on bcc 16^3): window
+    entry (bounded), then `calls` sweeps calls of one sweep each, every
+    walker in one launch; returns the walkers' aggregate trial rate."""
+    from brawl_amd import wang_landau, workloads
+    c = workloads.config4()
+    wl = wang_landau.WangLandau(c["lattice"], c["n"], c["v_ex"], bins=c["bins"],
+                                num_windows=c["num_windows"], walkers_per_window=c["walkers_per_window"],
+                                bin_overlap=c["bin_overlap"], energy_min=c["energy_min"],
+                                energy_max=c["energy_max"], mc_sweeps=1)
+    wl.initial_setup(species_numbers=c["counts"])
+    _, entered = wl.enter_windows(20 * 8192)
+    ms = 0.0
+    for _ in range(calls):
+        _, k = wl.sweeps(c["wl_f"], average=True)
+        ms += k
+    trials = calls * wl.n_walkers * 8192
+    wl.close()
+    log(f"[bench] Wang-Landau walkers: {trials} trials in {ms:.1f} ms device")
+    return {"workload": "16 windows x 8 walkers, bcc 16^3 (8192 sites), S=4, 2 shells, 800 bins; "
+                        f"{calls} sweeps calls of 1 sweep after a bounded window entry",
+            "walkers": wl.n_walkers, "entered": int(entered.sum()), "trials": trials,
+            "value": trials / (ms / 1e3), "unit": "swap trials/s", "device_ms": ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -371,6 +396,8 @@ def main():
         out["batched"]["vs_plain"] = out["batched"]["value"] / value
     if not args.no_served:
         out["served"] = served(args)
+    if not args.no_batched:
+        out["wang_landau_walkers"] = wang_landau_walkers()
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(chain, args.cpu_budget)

@@ -200,3 +200,15 @@ def config5(n_requests: int = 240, rate_hz: float = 400.0, seed: int = MASTER_SE
                                           mc_sweeps=20, wl_f=1.0, rank_offset=int(s % 1000)),
                                      equal_counts(n_atoms_of(BCC, (L, L, L)), 4)))
     return out
+
+
+def config4(seed: int = MASTER_SEED):
+    """Wang-Landau, quaternary bcc 16^3 (8192 sites), equiatomic, 2 shells
+    with V^(n) ~ N(0, 30/n meV); 16 windows of 50 bins (800 bins over
+    [-40, 10] meV/atom, 10 % overlap), 8 walkers per window, ln f from 1,
+    flatness 0.8."""
+    s = derive_seed(seed, 4)
+    v = random_v(4, [30.0, 15.0], s)
+    return dict(lattice=BCC, n=(16, 16, 16), v_ex=v, counts=equal_counts(8192, 4), bins=800,
+                num_windows=16, walkers_per_window=8, bin_overlap=0.1, energy_min=-40.0,
+                energy_max=10.0, wl_f=1.0, flatness=0.8, tolerance=1e-6)
