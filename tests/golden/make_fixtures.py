@@ -101,3 +101,31 @@ np.savez_compressed(
     bins=512, num_windows=4, bin_overlap=0.25, energy_min=-96.0, energy_max=0.0,
     wl_f=0.05, tolerance=5e-5, flatness=0.90, mc_sweeps=100, lng=lng)
 print("fixtures written to", OUT)
+
+
+# ---------------------------------------------------------------------------
+# Output files of the reference's tests, kept as they are (data), for the
+# format tests of brawl_amd/io.py.
+# ---------------------------------------------------------------------------
+GOLDEN_FILES = [
+    ("01_serial_metropolis", "proc_0000_energy_trajectory_at_T_0300.0.dat"),
+    ("01_serial_metropolis", "proc_0000_asro_trajectory_at_T_0300.0.dat"),
+    ("01_serial_metropolis", "proc_0000_final_config_at_0300.0.nc"),
+    ("01_serial_metropolis", "proc_0000_rho_of_T.nc"),
+    ("02_parallel_metropolis", "proc_0000_energy_diagnostics.dat"),
+    ("02_parallel_metropolis", "av_energy_diagnostics.dat"),
+    ("03_serial_nested_sampling", "fcc_al_1.00_crfeconi_K100.energies"),
+]
+
+
+def copy_golden_files():
+    import shutil
+    dst = os.path.join(OUT, "files")
+    os.makedirs(dst, exist_ok=True)
+    for test, name in GOLDEN_FILES:
+        shutil.copyfile(os.path.join(REF, "tests", "99_ref", test, name),
+                        os.path.join(dst, f"{test[:2]}_{name}"))
+
+
+if __name__ == "__main__" and os.path.isdir(os.path.join(REF, "tests", "99_ref")):
+    copy_golden_files()
