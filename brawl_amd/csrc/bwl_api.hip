@@ -395,20 +395,25 @@ int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int
   int max_pairs = 0;
   for (const RoundVar &v : plan.vars) max_pairs = v.n2 > max_pairs ? v.n2 : max_pairs;
   // fewest passes over a round's pairs, then the fewest threads for that
-  const int passes = (max_pairs + 511) / 512;
-  threads = (((max_pairs + passes - 1) / passes + 63) / 64) * 64;
-  if (threads > 512) threads = 512;
-  if (threads < 64) threads = 64;
+  auto block_size = [&](int cap) {
+    const int passes = (max_pairs + cap - 1) / cap;
+    int t = (((max_pairs + passes - 1) / passes + 63) / 64) * 64;
+    if (t > cap) t = cap;
+    return t < 64 ? 64 : t;
+  };
+  // packed counts: the natural kernel uses 8-bit fields (u32 holds 4
+  // species), the per-sublattice kernel 6-bit fields (u32 holds 5)
   bool use_nat = make_nat(g, nat);
+  bool wide = use_nat ? g.S > 4 : g.S > 5;
+  threads = block_size(par_max_threads(g.lattice, g.R, g.n[0], use_nat, wide));
   lds = par_lds_bytes(g, use_nat ? &nat : nullptr, threads);
   if (use_nat && lds > (size_t)lim) {
     use_nat = false;
+    wide = g.S > 5;
+    threads = block_size(par_max_threads(g.lattice, g.R, g.n[0], false, wide));
     lds = par_lds_bytes(g, nullptr, threads);
   }
   if (lds > (size_t)lim) return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
-  // packed counts: the natural kernel uses 8-bit fields (u32 holds 4
-  // species), the per-sublattice kernel 6-bit fields (u32 holds 5)
-  const bool wide = use_nat ? g.S > 4 : g.S > 5;
   fn = par_kernel_for(g.lattice, g.R, g.n[0], use_nat, wide);
   if (!fn) return fail(BWL_ERR_UNSUPPORTED, "no kernel for this lattice");
   if (natural) *natural = use_nat;

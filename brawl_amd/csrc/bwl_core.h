@@ -84,6 +84,8 @@ constexpr int kMaxRoundVars = 6;
 struct NatGeo {
   int32_t D[3], h[3], P[3];
   int32_t lds_bytes;
+  int32_t lane_order;    // bit A set: on round axis A the first free axis
+                         // varies fastest across a wavefront's lanes
   int32_t off[kMaxNbr];  // linear neighbour offsets in shell/list order
 };
 
@@ -104,14 +106,28 @@ __host__ __device__ constexpr void nat_delta(int lattice, int t, int &da, int &d
 
 constexpr int kNatBudget = 150 * 1024;  // LDS bytes the natural lattice may take
 
-// Natural layout of a cubic supercell with n cells per axis and R shells.
-// The padded row length P0 is made 2 mod 4 and P1 odd when that fits the
-// budget, so that 2-cell steps along either free axis of a round are odd
-// numbers of LDS dwords (bank-conflict-free byte reads across a wavefront).
+// Natural layout of a cubic supercell with n cells per axis and R shells,
+// padded so that the byte gathers of a round's wavefront (lanes on a 16 x 4
+// block of the stride-s grid over the two free axes) fall in distinct LDS
+// banks as far as the lattice allows:
+//  * bcc with grid stride 4 (rcut 2 or 3) and 16 grid positions per axis
+//    (n = 32): P0 = 4 mod 8 and P1 = 8 mod 16, so the 4-steps along x, y
+//    and z move (2 mod 4, 16 mod 32, odd) dwords; the lanes then take the
+//    second free axis fastest on round axes x and y and the first on z (two
+//    of three round axes conflict-free on 32 banks per half-wavefront, all
+//    three on 64 banks; other sizes wrap the 16-lane rows and gain nothing);
+//  * otherwise P0 = 2 mod 4 and P1 odd (2-cell steps are odd numbers of
+//    dwords), lanes with the first free axis fastest.
 // Returns false when an axis is shorter than twice its halo.
 __host__ __device__ constexpr bool nat_dims(int lattice, int R, int n, NatGeo &G) {
-  int z = 0;
+  int z = 0, rcut = 0;
   for (int s = 0; s < R; ++s) z += lattice == 1 ? kFccShellSize[s] : kBccShellSize[s];
+  for (int t = 0; t < z; ++t)
+    for (int a = 0; a < 3; ++a) {
+      const int v = lattice == 1 ? kFccOffsets[t][a] : kBccOffsets[t][a];
+      rcut = v > rcut ? v : (-v > rcut ? -v : rcut);
+    }
+  const int need = rcut + 1, stride = need % 2 ? need + 1 : need;
   G.D[0] = 2 * n;
   G.D[1] = lattice == 1 ? 2 * n : n;
   G.D[2] = n;
@@ -130,11 +146,19 @@ __host__ __device__ constexpr bool nat_dims(int lattice, int R, int n, NatGeo &G
     G.P[a] = G.D[a] + 2 * h[a];
   }
   int p0 = G.P[0], p1 = G.P[1];
-  if (p0 % 4 == 0) p0 += 2;
-  if (p1 % 2 == 0) p1 += 1;
+  const bool bcc4 = lattice == 2 && stride == 4 && 2 * n == 16 * stride;
+  if (bcc4) {
+    while (p0 % 8 != 4) ++p0;
+    while (p1 % 16 != 8) ++p1;
+  } else {
+    if (p0 % 4 == 0) p0 += 2;
+    if (p1 % 2 == 0) p1 += 1;
+  }
+  G.lane_order = 7;
   if (p0 * p1 * G.P[2] <= kNatBudget) {
     G.P[0] = p0;
     G.P[1] = p1;
+    if (bcc4) G.lane_order = 4;
   }
   G.lds_bytes = G.P[0] * G.P[1] * G.P[2];
   for (int t = 0; t < kMaxNbr; ++t) G.off[t] = 0;

@@ -149,6 +149,11 @@ hipError_t launch_site_energy(const SiteArgs &a, hipStream_t s);
 size_t seq_lds_bytes(const Geo &g);
 hipError_t launch_metro_seq(const SeqArgs &a, hipStream_t s);
 size_t par_lds_bytes(const Geo &g, const NatGeo *nat, int threads);
+// Block size limit of the parallel-sampler kernels: the natural-layout
+// kernels specialised for one lattice shape keep few enough registers for
+// 12 wavefronts per CU, the others are limited to 8.
+constexpr int kMaxNatThreads = 768;
+int par_max_threads(int lattice, int R, int n, bool nat, bool wide);
 const void *par_kernel_for(int lattice, int R, int n, bool nat, bool wide);
 hipError_t launch_metro_par_fn(const void *fn, const ParArgs *args_dev, int n_chains, size_t lds,
                                int threads, hipStream_t s);

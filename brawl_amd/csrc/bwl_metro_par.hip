@@ -458,7 +458,7 @@ __device__ __forceinline__ void asro_sample(const int8_t *lat, const ND N, const
 }
 
 template <int LAT, int R, typename CW, int NC>
-__global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ chains) {
+__global__ __launch_bounds__(NC ? kMaxNatThreads : 512) void k_metro_nat(const ParArgs *__restrict__ chains) {
   extern __shared__ __align__(16) uint8_t smem[];
   constexpr int FIELD = 8;
   const ParArgs &C = chains[blockIdx.x];
@@ -473,13 +473,19 @@ __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ c
   const int nthr = C.threads;
   const bool active = tid < nthr;
   const int S = g.S;
-  const int S1 = S - 1 > 0 ? S - 1 : 1;
+  // dtab[alpha][beta][n][s<S1P]: (V(beta,s,n)-V(alpha,s,n)) - (same for
+  // s=S-1) for s < S-1, zero beyond; the S-1 independent count differences
+  // suffice since each shell is full.  Rows are padded to S1P (the most
+  // species the count word holds, less one) so the contraction below is
+  // unrolled at compile time (the zero terms leave every sum unchanged), and
+  // the row stride is odd in doubles so the lanes' row gathers spread over
+  // the banks.
+  constexpr int S1P = sizeof(CW) == 4 ? 3 : 7;
+  constexpr int RSTR = (R * S1P) | 1;
   int8_t *lat = (int8_t *)smem;
   size_t o = ((N.P0 * N.P1 * N.P2) + 15) & ~15;
-  // dtab[alpha][beta][n][s<S-1]: (V(beta,s,n)-V(alpha,s,n)) - (same for s=S-1);
-  // the S-1 independent count differences suffice since each shell is full
   double *dtab = (double *)(smem + o);
-  o += sizeof(double) * S * S * R * S1;
+  o += sizeof(double) * S * S * RSTR;
   double *vrow = (double *)(smem + o);  // [x][n][s]: V(x,s,n)
   o += sizeof(double) * S * R * S;
   RoundP *rps = (RoundP *)(smem + o);
@@ -495,12 +501,17 @@ __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ c
 
   for (int i = tid; i < Z; i += blockDim.x) offl[i] = C.nat.off[i];
   for (int i = tid; i < S * S * R; i += blockDim.x) hacc[i] = 0ull;
-  for (int i = tid; i < S * S * R * S1; i += blockDim.x) {
-    int s = i % S1, n = (i / S1) % R, be = (i / (S1 * R)) % S, al = i / (S1 * R * S);
+  for (int i = tid; i < S * S * RSTR; i += blockDim.x) {
+    const int row = i / RSTR, k = i - row * RSTR;
+    const int s = k % S1P, n = k / S1P, be = row % S, al = row / S;
     const double *V = C.vex;
-    double d_s = V[be + S * (s + S * n)] - V[al + S * (s + S * n)];
-    double d_l = V[be + S * ((S - 1) + S * n)] - V[al + S * ((S - 1) + S * n)];
-    dtab[i] = S > 1 ? d_s - d_l : 0.0;
+    double v = 0.0;
+    if (n < R && s < S - 1) {
+      double d_s = V[be + S * (s + S * n)] - V[al + S * (s + S * n)];
+      double d_l = V[be + S * ((S - 1) + S * n)] - V[al + S * ((S - 1) + S * n)];
+      v = d_s - d_l;
+    }
+    dtab[i] = v;
   }
   for (int i = tid; i < S * R * S; i += blockDim.x) {
     int s = i % S, n = (i / S) % R, x = i / (S * R);
@@ -514,6 +525,7 @@ __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ c
 
   const int n_cells = g.n[0];
   const int L = C.L, stride = C.stride;
+  const int lane_order = NC ? KG.lane_order : C.nat.lane_order;
   unsigned long long phase = C.phase0;
   long long acc_total = 0, trials_total = 0;
 
@@ -564,13 +576,17 @@ __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ c
           const int KB = uni(P.KB), KC = uni(P.KC), nL1 = uni(P.nL1), n2 = uni(P.n2);
           const int al = uni(P.al), ab = uni(P.ab), ac = uni(P.ac);
           const unsigned long long ph = phase + (unsigned long long)r;
-          // grid-2 coordinates of this thread's first pair, and the stride
-          uint32_t jl, rem, kb, kc, djl, drem, dkb, dkc;
-          const float invKBC = 1.0f / (float)(KB * KC), invKB = 1.0f / (float)KB;
-          divmod_small((uint32_t)tid, (uint32_t)(KB * KC), invKBC, jl, rem);
-          divmod_small(rem, (uint32_t)KB, invKB, kc, kb);
-          divmod_small((uint32_t)nthr, (uint32_t)(KB * KC), invKBC, djl, drem);
-          divmod_small(drem, (uint32_t)KB, invKB, dkc, dkb);
+          // grid-2 coordinates of this thread's first pair, and the stride;
+          // pair q is (layer, slow, fast) in row-major order, the fast free
+          // axis chosen per round axis by the layout (bank spread)
+          const bool bfast = (lane_order >> ax) & 1;
+          const int KF = bfast ? KB : KC, KS = bfast ? KC : KB;
+          uint32_t jl, rem, kf, ks, djl, drem, dkf, dks;
+          const float invKFS = 1.0f / (float)(KF * KS), invKF = 1.0f / (float)KF;
+          divmod_small((uint32_t)tid, (uint32_t)(KF * KS), invKFS, jl, rem);
+          divmod_small(rem, (uint32_t)KF, invKF, ks, kf);
+          divmod_small((uint32_t)nthr, (uint32_t)(KF * KS), invKFS, djl, drem);
+          divmod_small(drem, (uint32_t)KF, invKF, dks, dkf);
           U4 rnd = {0, 0, 0, 0};
           int k = 0;
           for (int q = active ? tid : n2; q < n2; q += nthr, ++k) {
@@ -580,6 +596,7 @@ __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ c
                                C.seed_lo, C.seed_hi);
             const uint32_t ru = (k & 3) == 0 ? rnd.x : (k & 3) == 1 ? rnd.y : (k & 3) == 2 ? rnd.z : rnd.w;
             // grid-2 site
+            const uint32_t kb = bfast ? kf : ks, kc = bfast ? ks : kf;
             int pa = oA2 + (int)jl * L, pb = oB2 + (int)kb * stride, pc = oC2 + (int)kc * stride;
             if (pa >= mA) pa -= mA;
             if (pb >= mB) pb -= mB;
@@ -600,20 +617,20 @@ __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ c
             int aA, bA, cA;
             site_of<LAT>(ax, pa, pb, pc, n_cells, aA, bA, cA);
             // advance to this thread's next pair
-            kb += dkb;
-            uint32_t cy = kb >= (uint32_t)KB;
-            kb -= cy * KB;
-            kc += dkc + cy;
-            cy = kc >= (uint32_t)KC;
-            kc -= cy * KC;
+            kf += dkf;
+            uint32_t cy = kf >= (uint32_t)KF;
+            kf -= cy * KF;
+            ks += dks + cy;
+            cy = ks >= (uint32_t)KS;
+            ks -= cy * KS;
             jl += djl + cy;
 
+            // Both neighbourhoods are read before the species are compared,
+            // so all loads of the pair are in flight together; for equal
+            // species every dtab entry is zero, dE is zero and the swap is
+            // a counted no-op.
             const int addrA = nat_addr(N, aA, bA, cA), addrB = nat_addr(N, aB, bB, cB);
             const int codeA = lat[addrA], codeB = lat[addrB];
-            if (codeA == codeB) {
-              if (sampled) acc_local++;
-              continue;
-            }
             CW ca[R], cb[R];
             if constexpr (NC > 0) {
               count_c<LAT, R, CW, NC>(lat, addrA, ca);
@@ -623,26 +640,31 @@ __global__ __launch_bounds__(512) void k_metro_nat(const ParArgs *__restrict__ c
               count_nat<LAT, R, CW>(lat, addrB, off, cb);
             }
             const int alpha = codeA >> 3, betasp = codeB >> 3;
-            const double *dt = dtab + ((alpha * S + betasp) * R) * S1;
+            const double *dt = dtab + (alpha * S + betasp) * RSTR;
             double dE = 0.0;
 #pragma unroll
             for (int n = 0; n < R; ++n) {
               const uint32_t d0 = biased_diff((uint32_t)ca[n], (uint32_t)cb[n]);
-              for (int s = 0; s < S - 1 && s < 4; ++s)
-                dE += (double)((float)((d0 >> (8 * s)) & 0xffu) - 128.0f) * dt[n * S1 + s];
-              if (sizeof(CW) > 4) {
+#pragma unroll
+              for (int s = 0; s < (S1P < 4 ? S1P : 4); ++s)
+                dE += (double)((int)((d0 >> (8 * s)) & 0xffu) - 128) * dt[n * S1P + s];
+              if constexpr (sizeof(CW) > 4) {
                 const uint32_t d1 =
                     biased_diff((uint32_t)((uint64_t)ca[n] >> 32), (uint32_t)((uint64_t)cb[n] >> 32));
-                for (int s = 4; s < S - 1; ++s)
-                  dE += (double)((float)((d1 >> (8 * (s - 4))) & 0xffu) - 128.0f) * dt[n * S1 + s];
+#pragma unroll
+                for (int s = 4; s < S1P; ++s)
+                  dE += (double)((int)((d1 >> (8 * (s - 4))) & 0xffu) - 128) * dt[n * S1P + s];
               }
             }
-            bool acc = dE < 0.0;
+            const bool same = codeA == codeB;
+            bool acc = same || dE < 0.0;
             if (!acc) acc = metropolis_accept((double)ru * (1.0 / 4294967296.0), -(beta * dE));
             if (acc) {
-              nat_put(lat, N, aA, bA, cA, (int8_t)codeB);
-              nat_put(lat, N, aB, bB, cB, (int8_t)codeA);
-              dE_sum += dE;
+              if (!same) {
+                nat_put(lat, N, aA, bA, cA, (int8_t)codeB);
+                nat_put(lat, N, aB, bB, cB, (int8_t)codeA);
+                dE_sum += dE;
+              }
               if (sampled) acc_local++;
             }
           }
@@ -930,13 +952,19 @@ const void *par_kernel(int lattice, int R, int n, bool nat, bool wide) {
 
 size_t par_lds_bytes(const Geo &g, const NatGeo *nat, int threads) {
   const size_t red = sizeof(double) * (threads / 64 + 1);
-  const size_t S1 = g.S > 1 ? g.S - 1 : 1;
+  // dtab rows as in k_metro_nat: R * S1P doubles made odd
+  const size_t S1P = g.S > 4 ? 7 : 3;
+  const size_t rstr = (g.R * S1P) | 1;
   if (nat)
-    return (size_t)((nat->lds_bytes + 15) & ~15) + sizeof(double) * g.S * g.S * g.R * S1 +
+    return (size_t)((nat->lds_bytes + 15) & ~15) + sizeof(double) * g.S * g.S * rstr +
            sizeof(double) * g.S * g.R * g.S + sizeof(RoundP) * kRoundBatch +
            sizeof(int32_t) * ((g.z + 3) & ~3) + red + sizeof(unsigned long long) * g.S * g.S * g.R;
   return (size_t)((g.lds_bytes + 15) & ~15) + sizeof(double) * g.S * g.S * g.R + red +
          sizeof(int32_t) * g.nsub * g.z;
+}
+
+int par_max_threads(int lattice, int R, int n, bool nat, bool wide) {
+  return nat && specialised(lattice, R, n, wide) ? kMaxNatThreads : 512;
 }
 
 const void *par_kernel_for(int lattice, int R, int n, bool nat, bool wide) {
