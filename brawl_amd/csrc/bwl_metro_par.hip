@@ -200,21 +200,32 @@ __device__ __forceinline__ int nat_addr(const ND &N, int a, int b, int c) {
   return ((c + N.h2) * N.P1 + (b + N.h1)) * N.P0 + (a + N.h0);
 }
 
-// Store v at the interior cell and its periodic images (D >= 2h per axis).
+// Store v at the interior cell and its periodic images: with D >= 2h a cell
+// has at most one image position per axis, so the (up to 8) stores are
+// straight-line code under per-lane predicates.
 __device__ __forceinline__ void nat_put(int8_t *lat, const ND &N, int a, int b, int c, int8_t v) {
-  int pa[2], pb[2], pc[2], ka = 1, kb = 1, kc = 1;
-  pa[0] = a + N.h0;
-  if (a < N.h0) pa[ka++] = a + N.h0 + N.D0;
-  else if (a >= N.D0 - N.h0) pa[ka++] = a + N.h0 - N.D0;
-  pb[0] = b + N.h1;
-  if (b < N.h1) pb[kb++] = b + N.h1 + N.D1;
-  else if (b >= N.D1 - N.h1) pb[kb++] = b + N.h1 - N.D1;
-  pc[0] = c + N.h2;
-  if (c < N.h2) pc[kc++] = c + N.h2 + N.D2;
-  else if (c >= N.D2 - N.h2) pc[kc++] = c + N.h2 - N.D2;
-  for (int i = 0; i < kc; ++i)
-    for (int j = 0; j < kb; ++j)
-      for (int k = 0; k < ka; ++k) lat[(pc[i] * N.P1 + pb[j]) * N.P0 + pa[k]] = v;
+  const int pa = a + N.h0, pb = b + N.h1, pc = c + N.h2;
+  const int qa = a < N.h0 ? pa + N.D0 : (a >= N.D0 - N.h0 ? pa - N.D0 : -1);
+  const int qb = b < N.h1 ? pb + N.D1 : (b >= N.D1 - N.h1 ? pb - N.D1 : -1);
+  const int qc = c < N.h2 ? pc + N.D2 : (c >= N.D2 - N.h2 ? pc - N.D2 : -1);
+  const int r00 = (pc * N.P1 + pb) * N.P0;
+  lat[r00 + pa] = v;
+  if (qa >= 0) lat[r00 + qa] = v;
+  if (qb >= 0) {
+    const int r01 = (pc * N.P1 + qb) * N.P0;
+    lat[r01 + pa] = v;
+    if (qa >= 0) lat[r01 + qa] = v;
+  }
+  if (qc >= 0) {
+    const int r10 = (qc * N.P1 + pb) * N.P0;
+    lat[r10 + pa] = v;
+    if (qa >= 0) lat[r10 + qa] = v;
+    if (qb >= 0) {
+      const int r11 = (qc * N.P1 + qb) * N.P0;
+      lat[r11 + pa] = v;
+      if (qa >= 0) lat[r11 + qa] = v;
+    }
+  }
 }
 
 template <int LAT>
@@ -639,6 +650,10 @@ __global__ __launch_bounds__(NC ? kMaxNatThreads : 512) void k_metro_nat(const P
               count_nat<LAT, R, CW>(lat, addrA, off, ca);
               count_nat<LAT, R, CW>(lat, addrB, off, cb);
             }
+            // keep the contraction below out of the gather: interleaving it
+            // makes the scheduler trade the gather's loads in flight for
+            // registers (one load at a time for the last shells)
+            __builtin_amdgcn_sched_barrier(0);
             const int alpha = codeA >> 3, betasp = codeB >> 3;
             const double *dt = dtab + (alpha * S + betasp) * RSTR;
             double dE = 0.0;
