@@ -345,14 +345,73 @@ __device__ inline RoundP round_params(const ParArgs &C, unsigned long long phase
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Natural-layout address of the grid site (pa, pb, pc) along (round axis,
-// free axes); coordinates already reduced into the box.
+// Natural coordinates of the grid sites of a round as affine functions of
+// the grid indices (j along the round axis, u and v along its free axes):
+//   coordinate = base + j*cj + u*cu + v*cv,  then reduced into [0, D).
+// The doubled coordinates are x, y, z = origin + j*L | u*s | v*s (one index
+// each, by round axis), and the natural ones linear in those (bcc: a = z,
+// b = (x-z)/2, c = (y-z)/2; fcc: a = x, b = y, c = (z-x-y)/2), so the
+// coefficients are uniform per round.  a has one non-negative term and needs
+// one conditional subtract; the b and c bases carry a bias of 8D so the sums
+// stay positive and are reduced with a float reciprocal.
+struct NatAffine {
+  int a1, b1, c1, a2, b2, c2;  // biased bases of grids 1 and 2
+  int aj, au, av, bj, bu, bv, cj, cu, cv;
+};
+
 template <int LAT>
-__device__ __forceinline__ void site_of(int ax, int pa, int pb, int pc, int n,
-                                        int &a, int &b, int &c) {
-  int x, y, z;
-  place(ax, pa, pb, pc, x, y, z);
-  to_natural<LAT>(x, y, z, n, a, b, c);
+__device__ __forceinline__ NatAffine nat_affine(int ax, int L, int s, int n, int oA1, int oB1,
+                                                int oC1, int oA2, int oB2, int oC2) {
+  // (origin grid 1, origin grid 2, j, u, v) of x, y and z
+  int X[5], Y[5], Z[5];
+  auto set = [](int *t, int o1, int o2, int j, int u, int v) {
+    t[0] = o1; t[1] = o2; t[2] = j; t[3] = u; t[4] = v;
+  };
+  if (ax == 0) {
+    set(X, oA1, oA2, L, 0, 0); set(Y, oB1, oB2, 0, s, 0); set(Z, oC1, oC2, 0, 0, s);
+  } else if (ax == 1) {
+    set(X, oC1, oC2, 0, 0, s); set(Y, oA1, oA2, L, 0, 0); set(Z, oB1, oB2, 0, s, 0);
+  } else {
+    set(X, oB1, oB2, 0, s, 0); set(Y, oC1, oC2, 0, 0, s); set(Z, oA1, oA2, L, 0, 0);
+  }
+  int A[5], B[5], Cc[5];
+  for (int i = 0; i < 5; ++i) {
+    if (LAT == 1) {
+      A[i] = X[i];
+      B[i] = Y[i];
+      Cc[i] = (Z[i] - X[i] - Y[i]) / 2;  // even differences
+    } else {
+      A[i] = Z[i];
+      B[i] = (X[i] - Z[i]) / 2;
+      Cc[i] = (Y[i] - Z[i]) / 2;
+    }
+  }
+  const int D1 = LAT == 1 ? 2 * n : n;
+  NatAffine F;
+  F.a1 = A[0]; F.a2 = A[1];
+  F.b1 = B[0] + 8 * D1; F.b2 = B[1] + 8 * D1;
+  F.c1 = Cc[0] + 8 * n; F.c2 = Cc[1] + 8 * n;
+  F.aj = A[2]; F.au = A[3]; F.av = A[4];
+  F.bj = B[2]; F.bu = B[3]; F.bv = B[4];
+  F.cj = Cc[2]; F.cu = Cc[3]; F.cv = Cc[4];
+  return F;
+}
+
+// x mod d for 0 <= x < 2^20, d <= 2^12: float reciprocal (the quotient is
+// exact or one low), then one conditional subtract.
+__device__ __forceinline__ int mod_pos(int x, int d, float inv_d) {
+  const uint32_t q = (uint32_t)((float)x * inv_d);
+  const uint32_t r = (uint32_t)x - __umul24(q, (uint32_t)d);
+  return (int)min(r, r - (uint32_t)d);
+}
+
+__device__ __forceinline__ void nat_site(const NatAffine &F, int ba, int bb, int bc, int j, int u,
+                                         int v, const ND &N, float inv1, float inv2, int &a,
+                                         int &b, int &c) {
+  const uint32_t va = (uint32_t)(ba + __mul24(j, F.aj) + __mul24(u, F.au) + __mul24(v, F.av));
+  a = (int)min(va, va - (uint32_t)N.D0);
+  b = mod_pos(bb + __mul24(j, F.bj) + __mul24(u, F.bu) + __mul24(v, F.bv), N.D1, inv1);
+  c = mod_pos(bc + __mul24(j, F.cj) + __mul24(u, F.cu) + __mul24(v, F.cv), N.D2, inv2);
 }
 
 // (128 + c_a - c_b) per byte lane of a packed count word
@@ -537,6 +596,20 @@ __global__ __launch_bounds__(NC ? kMaxNatThreads : 512) void k_metro_nat(const P
   const int n_cells = g.n[0];
   const int L = C.L, stride = C.stride;
   const int lane_order = NC ? KG.lane_order : C.nat.lane_order;
+  // lane -> (layer, slow, fast) decomposition of a round's pair index q in
+  // row-major order; the natural layout is cubic, so every free axis has
+  // the same K grid positions and the decomposition is the same every round
+  const int Kq = C.K[0];
+  uint32_t jl0, ks0, kf0, djl, dks, dkf;
+  {
+    uint32_t rem;
+    const float invKK = 1.0f / (float)(Kq * Kq), invK = 1.0f / (float)Kq;
+    divmod_small((uint32_t)tid, (uint32_t)(Kq * Kq), invKK, jl0, rem);
+    divmod_small(rem, (uint32_t)Kq, invK, ks0, kf0);
+    divmod_small((uint32_t)nthr, (uint32_t)(Kq * Kq), invKK, djl, rem);
+    divmod_small(rem, (uint32_t)Kq, invK, dks, dkf);
+  }
+  const float invD1 = 1.0f / (float)N.D1, invD2 = 1.0f / (float)N.D2;
   unsigned long long phase = C.phase0;
   long long acc_total = 0, trials_total = 0;
 
@@ -581,23 +654,16 @@ __global__ __launch_bounds__(NC ? kMaxNatThreads : 512) void k_metro_nat(const P
         int r = 0;
         for (; r < kRoundBatch && done < C.min_trials_per_sweep; ++r) {
           const RoundP &P = rps[r];
-          const int ax = uni(P.ax), mA = uni(P.mA), mB = uni(P.mB), mC = uni(P.mC);
-          const int oA1 = uni(P.oA1), oB1 = uni(P.oB1), oC1 = uni(P.oC1);
-          const int oA2 = uni(P.oA2), oB2 = uni(P.oB2), oC2 = uni(P.oC2);
-          const int KB = uni(P.KB), KC = uni(P.KC), nL1 = uni(P.nL1), n2 = uni(P.n2);
+          const int ax = uni(P.ax);
+          const int nL1 = uni(P.nL1), n2 = uni(P.n2);
           const int al = uni(P.al), ab = uni(P.ab), ac = uni(P.ac);
+          const NatAffine F = nat_affine<LAT>(ax, L, stride, n_cells, uni(P.oA1), uni(P.oB1),
+                                              uni(P.oC1), uni(P.oA2), uni(P.oB2), uni(P.oC2));
           const unsigned long long ph = phase + (unsigned long long)r;
-          // grid-2 coordinates of this thread's first pair, and the stride;
-          // pair q is (layer, slow, fast) in row-major order, the fast free
-          // axis chosen per round axis by the layout (bank spread)
+          // the fast free axis of the lanes is chosen per round axis by the
+          // layout (bank spread)
           const bool bfast = (lane_order >> ax) & 1;
-          const int KF = bfast ? KB : KC, KS = bfast ? KC : KB;
-          uint32_t jl, rem, kf, ks, djl, drem, dkf, dks;
-          const float invKFS = 1.0f / (float)(KF * KS), invKF = 1.0f / (float)KF;
-          divmod_small((uint32_t)tid, (uint32_t)(KF * KS), invKFS, jl, rem);
-          divmod_small(rem, (uint32_t)KF, invKF, ks, kf);
-          divmod_small((uint32_t)nthr, (uint32_t)(KF * KS), invKFS, djl, drem);
-          divmod_small(drem, (uint32_t)KF, invKF, dks, dkf);
+          uint32_t jl = jl0, ks = ks0, kf = kf0;
           U4 rnd = {0, 0, 0, 0};
           int k = 0;
           for (int q = active ? tid : n2; q < n2; q += nthr, ++k) {
@@ -606,34 +672,23 @@ __global__ __launch_bounds__(NC ? kMaxNatThreads : 512) void k_metro_nat(const P
                                   (uint32_t)tid | ((uint32_t)(k >> 2) << 12), C.chain_id},
                                C.seed_lo, C.seed_hi);
             const uint32_t ru = (k & 3) == 0 ? rnd.x : (k & 3) == 1 ? rnd.y : (k & 3) == 2 ? rnd.z : rnd.w;
-            // grid-2 site
-            const uint32_t kb = bfast ? kf : ks, kc = bfast ? ks : kf;
-            int pa = oA2 + (int)jl * L, pb = oB2 + (int)kb * stride, pc = oC2 + (int)kc * stride;
-            if (pa >= mA) pa -= mA;
-            if (pb >= mB) pb -= mB;
-            if (pc >= mC) pc -= mC;
-            int aB, bB, cB;
-            site_of<LAT>(ax, pa, pb, pc, n_cells, aB, bB, cB);
-            // its grid-1 partner (translation by (al, ab, ac) in grid indices)
-            int j1 = (int)jl + al, k1 = (int)kb + ab, c1 = (int)kc + ac;
+            // grid-2 site, and its grid-1 partner (translation by (al, ab,
+            // ac) in grid indices)
+            const int kb = (int)(bfast ? kf : ks), kc = (int)(bfast ? ks : kf);
+            int aB, bB, cB, aA, bA, cA;
+            nat_site(F, F.a2, F.b2, F.c2, (int)jl, kb, kc, N, invD1, invD2, aB, bB, cB);
+            int j1 = (int)jl + al, k1 = kb + ab, c1 = kc + ac;
             if (j1 >= nL1) j1 -= nL1;
-            if (k1 >= KB) k1 -= KB;
-            if (c1 >= KC) c1 -= KC;
-            pa = oA1 + j1 * L;
-            pb = oB1 + k1 * stride;
-            pc = oC1 + c1 * stride;
-            if (pa >= mA) pa -= mA;
-            if (pb >= mB) pb -= mB;
-            if (pc >= mC) pc -= mC;
-            int aA, bA, cA;
-            site_of<LAT>(ax, pa, pb, pc, n_cells, aA, bA, cA);
+            if (k1 >= Kq) k1 -= Kq;
+            if (c1 >= Kq) c1 -= Kq;
+            nat_site(F, F.a1, F.b1, F.c1, j1, k1, c1, N, invD1, invD2, aA, bA, cA);
             // advance to this thread's next pair
             kf += dkf;
-            uint32_t cy = kf >= (uint32_t)KF;
-            kf -= cy * KF;
+            uint32_t cy = kf >= (uint32_t)Kq;
+            kf -= cy * Kq;
             ks += dks + cy;
-            cy = ks >= (uint32_t)KS;
-            ks -= cy * KS;
+            cy = ks >= (uint32_t)Kq;
+            ks -= cy * Kq;
             jl += djl + cy;
 
             // Both neighbourhoods are read before the species are compared,
