@@ -197,33 +197,33 @@ __host__ __device__ constexpr NatGeo nat_geo_c() {
 }
 
 __device__ __forceinline__ int nat_addr(const ND &N, int a, int b, int c) {
-  return ((c + N.h2) * N.P1 + (b + N.h1)) * N.P0 + (a + N.h0);
+  // 24-bit multiplies (full rate); LDS addresses are far below 2^24
+  return (int)__umul24(__umul24((uint32_t)(c + N.h2), (uint32_t)N.P1) + (uint32_t)(b + N.h1),
+                       (uint32_t)N.P0) + (a + N.h0);
 }
 
-// Store v at the interior cell and its periodic images: with D >= 2h a cell
-// has at most one image position per axis, so the (up to 8) stores are
+// Store v at the interior cell (LDS address addr, natural coordinates a, b,
+// c) and its periodic images: with D >= 2h a cell has at most one image
+// position per axis, at a fixed address step, so the (up to 8) stores are
 // straight-line code under per-lane predicates.
-__device__ __forceinline__ void nat_put(int8_t *lat, const ND &N, int a, int b, int c, int8_t v) {
-  const int pa = a + N.h0, pb = b + N.h1, pc = c + N.h2;
-  const int qa = a < N.h0 ? pa + N.D0 : (a >= N.D0 - N.h0 ? pa - N.D0 : -1);
-  const int qb = b < N.h1 ? pb + N.D1 : (b >= N.D1 - N.h1 ? pb - N.D1 : -1);
-  const int qc = c < N.h2 ? pc + N.D2 : (c >= N.D2 - N.h2 ? pc - N.D2 : -1);
-  const int r00 = (pc * N.P1 + pb) * N.P0;
-  lat[r00 + pa] = v;
-  if (qa >= 0) lat[r00 + qa] = v;
-  if (qb >= 0) {
-    const int r01 = (pc * N.P1 + qb) * N.P0;
-    lat[r01 + pa] = v;
-    if (qa >= 0) lat[r01 + qa] = v;
+__device__ __forceinline__ void nat_put(int8_t *lat, const ND &N, int addr, int a, int b, int c,
+                                        int8_t v) {
+  const int row = N.P0, plane = (int)__umul24((uint32_t)N.P0, (uint32_t)N.P1);
+  const int da = a < N.h0 ? N.D0 : (a >= N.D0 - N.h0 ? -N.D0 : 0);
+  const int db = b < N.h1 ? N.D1 * row : (b >= N.D1 - N.h1 ? -N.D1 * row : 0);
+  const int dc = c < N.h2 ? N.D2 * plane : (c >= N.D2 - N.h2 ? -N.D2 * plane : 0);
+  lat[addr] = v;
+  if (da) lat[addr + da] = v;
+  if (db) {
+    lat[addr + db] = v;
+    if (da) lat[addr + db + da] = v;
   }
-  if (qc >= 0) {
-    const int r10 = (qc * N.P1 + pb) * N.P0;
-    lat[r10 + pa] = v;
-    if (qa >= 0) lat[r10 + qa] = v;
-    if (qb >= 0) {
-      const int r11 = (qc * N.P1 + qb) * N.P0;
-      lat[r11 + pa] = v;
-      if (qa >= 0) lat[r11 + qa] = v;
+  if (dc) {
+    lat[addr + dc] = v;
+    if (da) lat[addr + dc + da] = v;
+    if (db) {
+      lat[addr + dc + db] = v;
+      if (da) lat[addr + dc + db + da] = v;
     }
   }
 }
@@ -397,9 +397,16 @@ __device__ __forceinline__ NatAffine nat_affine(int ax, int L, int s, int n, int
   return F;
 }
 
-// x mod d for 0 <= x < 2^20, d <= 2^12: float reciprocal (the quotient is
-// exact or one low), then one conditional subtract.
+// x mod d for 0 <= x < 2^20, d <= 2^12: a mask for powers of two, else a
+// float reciprocal (the quotient is exact or one low), then one conditional
+// subtract.
 __device__ __forceinline__ int mod_pos(int x, int d, float inv_d) {
+  if ((d & (d - 1)) == 0) {  // folds when d is a constant
+    // opaque to the demanded-bits rewrite, which would otherwise turn the
+    // 24-bit multiplies feeding x into quarter-rate 32-bit ones
+    asm volatile("" : "+v"(x));
+    return x & (d - 1);
+  }
   const uint32_t q = (uint32_t)((float)x * inv_d);
   const uint32_t r = (uint32_t)x - __umul24(q, (uint32_t)d);
   return (int)min(r, r - (uint32_t)d);
@@ -710,20 +717,30 @@ __global__ __launch_bounds__(NC ? kMaxNatThreads : 512) void k_metro_nat(const P
             // registers (one load at a time for the last shells)
             __builtin_amdgcn_sched_barrier(0);
             const int alpha = codeA >> 3, betasp = codeB >> 3;
-            const double *dt = dtab + (alpha * S + betasp) * RSTR;
+            const double *dt = dtab + __umul24((uint32_t)(alpha * S + betasp), (uint32_t)RSTR);
+            // a short table row is read into registers in one burst (the
+            // loads then overlap instead of one round trip per term)
+            constexpr bool kRowRegs = R * S1P <= 16;
+            double dv[kRowRegs ? R * S1P : 1];
+            if constexpr (kRowRegs) {
+#pragma unroll
+              for (int i = 0; i < R * S1P; ++i) dv[i] = dt[i];
+              __builtin_amdgcn_sched_barrier(0);
+            }
+            auto dtv = [&](int i) { return kRowRegs ? dv[kRowRegs ? i : 0] : dt[i]; };
             double dE = 0.0;
 #pragma unroll
             for (int n = 0; n < R; ++n) {
               const uint32_t d0 = biased_diff((uint32_t)ca[n], (uint32_t)cb[n]);
 #pragma unroll
               for (int s = 0; s < (S1P < 4 ? S1P : 4); ++s)
-                dE += (double)((int)((d0 >> (8 * s)) & 0xffu) - 128) * dt[n * S1P + s];
+                dE += (double)((int)((d0 >> (8 * s)) & 0xffu) - 128) * dtv(n * S1P + s);
               if constexpr (sizeof(CW) > 4) {
                 const uint32_t d1 =
                     biased_diff((uint32_t)((uint64_t)ca[n] >> 32), (uint32_t)((uint64_t)cb[n] >> 32));
 #pragma unroll
                 for (int s = 4; s < S1P; ++s)
-                  dE += (double)((int)((d1 >> (8 * (s - 4))) & 0xffu) - 128) * dt[n * S1P + s];
+                  dE += (double)((int)((d1 >> (8 * (s - 4))) & 0xffu) - 128) * dtv(n * S1P + s);
               }
             }
             const bool same = codeA == codeB;
@@ -731,8 +748,8 @@ __global__ __launch_bounds__(NC ? kMaxNatThreads : 512) void k_metro_nat(const P
             if (!acc) acc = metropolis_accept((double)ru * (1.0 / 4294967296.0), -(beta * dE));
             if (acc) {
               if (!same) {
-                nat_put(lat, N, aA, bA, cA, (int8_t)codeB);
-                nat_put(lat, N, aB, bB, cB, (int8_t)codeA);
+                nat_put(lat, N, addrA, aA, bA, cA, (int8_t)codeB);
+                nat_put(lat, N, addrB, aB, bB, cB, (int8_t)codeA);
                 dE_sum += dE;
               }
               if (sampled) acc_local++;
