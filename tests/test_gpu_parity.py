@@ -239,6 +239,34 @@ def test_parallel_sampler_statistics_match_reference_sampler(hip, oracle):
         assert abs(diff) <= 2.0 * sigma + 1e-9, name
 
 
+def test_config2_sampler_statistics_match_reference_sampler(hip, oracle):
+    """The full-size bcc 32^3 config, which has a kernel of its own (compile-
+    time neighbour offsets, bank-spread layout and lane order, 768 threads),
+    at 1500 K from the same start: after 100 burn-in sweeps, the block means
+    of <E>/N and C over 10 consecutive blocks of 20 sweeps agree with the
+    reference's sequential sampler (the oracle) run the same way, within 3
+    sigma of the block means."""
+    from brawl_amd import workloads
+    c = workloads.config2()
+    N = c.n_atoms
+    T = 1500.0
+    ref = oracle.OracleLattice(c.lattice, c.n, c.v_ex)
+    gen = oracle.Generator(110179)
+    start = ref.initial_setup(gen, species_numbers=c.counts)
+    r = ref.anneal(start.copy(order="F"), gen, T=T, T_steps=10, n_mc_steps=20 * N, n_sample_steps=N,
+                   burn_in_start=True, n_burn_in_steps=100 * N)
+    with hip.Lattice(c.lattice, c.n, c.v_ex) as L:
+        L.set_config(start.copy(order="F"))
+        L.metropolis_simulated_annealing(T=T, n_sweeps=100, seed=11)
+        g = L.metropolis_simulated_annealing(T=T, T_steps=10, n_sweeps=20, seed=12)
+    for name, g_, r_ in (("<E>/N", g.energies_of_T, r["energies"]), ("C", g.C_of_T, r["C"])):
+        g_, r_ = np.asarray(g_), np.asarray(r_)
+        diff = g_.mean() - r_.mean()
+        sigma = np.sqrt(g_.var(ddof=1) / len(g_) + r_.var(ddof=1) / len(r_))
+        _say(name, "gpu", g_.mean(), "ref", r_.mean(), "diff/sigma", diff / sigma)
+        assert abs(diff) <= 3.0 * sigma + 1e-12, name
+
+
 @pytest.mark.parametrize("lat,n,S,R", [(2, (8, 8, 8), 4, 4), (1, (6, 6, 6), 3, 3), (2, (5, 5, 5), 5, 2)])
 def test_anneal_radial_densities_equal_recount(hip, oracle, lat, n, S, R):
     """One sweep, one sample: the anneal's rho_of_T equals the radial
