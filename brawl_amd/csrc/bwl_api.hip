@@ -401,20 +401,18 @@ int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int
     if (t > cap) t = cap;
     return t < 64 ? 64 : t;
   };
-  // packed counts: the natural kernel uses 8-bit fields (u32 holds 4
-  // species), the per-sublattice kernel 6-bit fields (u32 holds 5)
+  // packed counts (par_count_words): 32-bit words with 8-bit fields up to
+  // 4 species, 6-bit fields for 5, else 64-bit words
   bool use_nat = make_nat(g, nat);
-  bool wide = use_nat ? g.S > 4 : g.S > 5;
-  threads = block_size(par_max_threads(g.lattice, g.R, g.n[0], use_nat, wide));
+  threads = block_size(par_max_threads(g, use_nat));
   lds = par_lds_bytes(g, use_nat ? &nat : nullptr, threads);
   if (use_nat && lds > (size_t)lim) {
     use_nat = false;
-    wide = g.S > 5;
-    threads = block_size(par_max_threads(g.lattice, g.R, g.n[0], false, wide));
+    threads = block_size(par_max_threads(g, false));
     lds = par_lds_bytes(g, nullptr, threads);
   }
   if (lds > (size_t)lim) return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
-  fn = par_kernel_for(g.lattice, g.R, g.n[0], use_nat, wide);
+  fn = par_kernel_for(g, use_nat);
   if (!fn) return fail(BWL_ERR_UNSUPPORTED, "no kernel for this lattice");
   if (natural) *natural = use_nat;
   return BWL_OK;

@@ -153,12 +153,13 @@ size_t par_lds_bytes(const Geo &g, const NatGeo *nat, int threads);
 // kernels specialised for one lattice shape keep few enough registers for
 // 12 wavefronts per CU, the others are limited to 8.
 constexpr int kMaxNatThreads = 768;
-int par_max_threads(int lattice, int R, int n, bool nat, bool wide);
-const void *par_kernel_for(int lattice, int R, int n, bool nat, bool wide);
+int par_max_threads(const Geo &g, bool nat);
+// packed count words of the sampler for a lattice (see par_count_words in
+// bwl_metro_par.hip): 0 = 32-bit / 8-bit fields, 1 = 32-bit / 6-bit, 2 = 64-bit
+int par_count_words(const Geo &g, bool nat);
+const void *par_kernel_for(const Geo &g, bool nat);
 hipError_t launch_metro_par_fn(const void *fn, const ParArgs *args_dev, int n_chains, size_t lds,
                                int threads, hipStream_t s);
-hipError_t launch_metro_par(const ParArgs *args_dev, int n_chains, int lattice, int R, int n,
-                            bool nat, size_t lds, int threads, bool wide, hipStream_t s);
 
 size_t ns_lds_bytes(const Geo &g, int n_walkers);
 // Initial walker energies (all walkers at once), then the runs (one

@@ -421,25 +421,36 @@ __device__ __forceinline__ void nat_site(const NatAffine &F, int ba, int bb, int
   c = mod_pos(bc + __mul24(j, F.cj) + __mul24(u, F.cu) + __mul24(v, F.cv), N.D2, inv2);
 }
 
-// (128 + c_a - c_b) per byte lane of a packed count word
-__device__ __forceinline__ uint32_t biased_diff(uint32_t a, uint32_t b) { return a + 0x80808080u - b; }
+// (2^(F-1) + c_a - c_b) per F-bit field of a packed count word (no field
+// borrows: |c_a - c_b| <= shell size < 2^(F-1))
+template <int F>
+constexpr uint32_t field_bias() {
+  uint32_t b = 0;
+  for (int s = 0; s + F <= 32; s += F) b |= (uint32_t)1 << (s + F - 1);
+  return b;
+}
+template <int F>
+__device__ __forceinline__ uint32_t biased_diff(uint32_t a, uint32_t b) {
+  return a + field_bias<F>() - b;
+}
 
 // One radial-density sample of the lattice: H[a][b][n] += number of
 // shell-n neighbours of species b around sites of species a.  Each lane
-// widens its sites' packed 8-bit counts to 16-bit fields and sums them per
-// (species, shell); the lanes of a wavefront are then summed with shuffles
-// and lane 0 adds the totals to H (LDS, 64-bit).  With at most 4 species the
-// per-lane sums stay in registers over all of the lane's sites (flushed
-// before a 16-bit field could overflow); otherwise every 64 sites.
-template <int LAT, int R, typename CW, int NC>
+// widens its sites' packed FIELD-bit counts to 16-bit fields and sums them
+// per (species, shell); the lanes of a wavefront are then summed with
+// shuffles and lane 0 adds the totals to H (LDS, 64-bit).  With at most 4
+// species in 8-bit fields the per-lane sums stay in registers over all of
+// the lane's sites (flushed before a 16-bit field could overflow);
+// otherwise every 64 sites.
+template <int LAT, int R, typename CW, int NC, int FIELD>
 __device__ __forceinline__ void asro_sample(const int8_t *lat, const ND N, const int32_t *offl, int S,
                                          int tid, int nthr, bool active, unsigned long long *hacc) {
-  constexpr int FIELD = 8;
+  constexpr uint64_t kMask = (1ull << FIELD) - 1;
   constexpr int Z = shell_begin<LAT>(R);
   int off[NC ? 1 : Z];
 #pragma unroll
   for (int t = 0; t < (NC ? 1 : Z); ++t) off[t] = NC ? 0 : offl[t];
-  constexpr bool kRegs = sizeof(CW) == 4 && (R <= 3 || NC > 0);
+  constexpr bool kRegs = sizeof(CW) == 4 && FIELD == 8 && (R <= 3 || NC > 0);
   constexpr int XS = kRegs ? 4 : 8;
   constexpr int kMaxField = shell_begin<LAT>(R);  // largest count a site can add
   const int nsites = N.D0 * N.D1 * N.D2;
@@ -512,8 +523,9 @@ __device__ __forceinline__ void asro_sample(const int8_t *lat, const ND N, const
           uint64_t lo = 0, hi = 0;
 #pragma unroll
           for (int f = 0; f < 4; ++f) {
-            lo |= ((w >> (8 * f)) & 0xffull) << (16 * f);
-            hi |= ((w >> (8 * (f + 4))) & 0xffull) << (16 * f);
+            lo |= ((w >> (FIELD * f)) & kMask) << (16 * f);
+            if (FIELD * (f + 5) <= 8 * (int)sizeof(CW))
+              hi |= ((w >> (FIELD * (f + 4))) & kMask) << (16 * f);
           }
 #pragma unroll
           for (int m = 32; m > 0; m >>= 1) {
@@ -534,10 +546,10 @@ __device__ __forceinline__ void asro_sample(const int8_t *lat, const ND N, const
   }
 }
 
-template <int LAT, int R, typename CW, int NC>
+template <int LAT, int R, typename CW, int NC, int FIELD>
 __global__ __launch_bounds__(NC ? kMaxNatThreads : 512) void k_metro_nat(const ParArgs *__restrict__ chains) {
   extern __shared__ __align__(16) uint8_t smem[];
-  constexpr int FIELD = 8;
+  static_assert(FIELD == 8 || (FIELD == 6 && sizeof(CW) == 4), "count fields: 8 bits, or 6 in 32-bit words");
   const ParArgs &C = chains[blockIdx.x];
   const Geo &g = C.g;
   constexpr NatGeo KG = nat_geo_c<LAT, R, NC ? NC : 8>();
@@ -557,7 +569,7 @@ __global__ __launch_bounds__(NC ? kMaxNatThreads : 512) void k_metro_nat(const P
   // unrolled at compile time (the zero terms leave every sum unchanged), and
   // the row stride is odd in doubles so the lanes' row gathers spread over
   // the banks.
-  constexpr int S1P = sizeof(CW) == 4 ? 3 : 7;
+  constexpr int S1P = 8 * (int)sizeof(CW) / FIELD - 1;
   constexpr int RSTR = (R * S1P) | 1;
   int8_t *lat = (int8_t *)smem;
   size_t o = ((N.P0 * N.P1 * N.P2) + 15) & ~15;
@@ -642,7 +654,7 @@ __global__ __launch_bounds__(NC ? kMaxNatThreads : 512) void k_metro_nat(const P
 #pragma unroll
       for (int n = 0; n < R; ++n) {
         const double *vr = vrow + (sp * R + n) * S;
-        for (int s = 0; s < S; ++s) part += (double)((cnt[n] >> (FIELD * s)) & 0xff) * vr[s];
+        for (int s = 0; s < S; ++s) part += (double)((cnt[n] >> (FIELD * s)) & ((1u << FIELD) - 1)) * vr[s];
       }
     }
     double E = 0.5 * block_sum(part, red);
@@ -716,7 +728,9 @@ __global__ __launch_bounds__(NC ? kMaxNatThreads : 512) void k_metro_nat(const P
             // makes the scheduler trade the gather's loads in flight for
             // registers (one load at a time for the last shells)
             __builtin_amdgcn_sched_barrier(0);
-            const int alpha = codeA >> 3, betasp = codeB >> 3;
+            // codes are FIELD*(species-1) >= 0: unsigned division is a shift
+            // (or a multiply-shift) without a sign fix-up
+            const int alpha = (int)((uint32_t)codeA / FIELD), betasp = (int)((uint32_t)codeB / FIELD);
             const double *dt = dtab + __umul24((uint32_t)(alpha * S + betasp), (uint32_t)RSTR);
             // a short table row is read into registers in one burst (the
             // loads then overlap instead of one round trip per term)
@@ -731,16 +745,18 @@ __global__ __launch_bounds__(NC ? kMaxNatThreads : 512) void k_metro_nat(const P
             double dE = 0.0;
 #pragma unroll
             for (int n = 0; n < R; ++n) {
-              const uint32_t d0 = biased_diff((uint32_t)ca[n], (uint32_t)cb[n]);
+              constexpr uint32_t kM = (1u << FIELD) - 1;
+              constexpr int kB = 1 << (FIELD - 1), kLo = sizeof(CW) == 4 ? S1P : 4;
+              const uint32_t d0 = biased_diff<FIELD>((uint32_t)ca[n], (uint32_t)cb[n]);
 #pragma unroll
-              for (int s = 0; s < (S1P < 4 ? S1P : 4); ++s)
-                dE += (double)((int)((d0 >> (8 * s)) & 0xffu) - 128) * dtv(n * S1P + s);
+              for (int s = 0; s < kLo; ++s)
+                dE += (double)((int)((d0 >> (FIELD * s)) & kM) - kB) * dtv(n * S1P + s);
               if constexpr (sizeof(CW) > 4) {
-                const uint32_t d1 =
-                    biased_diff((uint32_t)((uint64_t)ca[n] >> 32), (uint32_t)((uint64_t)cb[n] >> 32));
+                const uint32_t d1 = biased_diff<FIELD>((uint32_t)((uint64_t)ca[n] >> 32),
+                                                       (uint32_t)((uint64_t)cb[n] >> 32));
 #pragma unroll
                 for (int s = 4; s < S1P; ++s)
-                  dE += (double)((int)((d1 >> (8 * (s - 4))) & 0xffu) - 128) * dtv(n * S1P + s);
+                  dE += (double)((int)((d1 >> (FIELD * (s - 4))) & kM) - kB) * dtv(n * S1P + s);
               }
             }
             const bool same = codeA == codeB;
@@ -770,7 +786,7 @@ __global__ __launch_bounds__(NC ? kMaxNatThreads : 512) void k_metro_nat(const P
           if (C.e_trace) C.e_trace[(long long)t * C.n_sweeps + (sw - C.burn_sweeps)] = E;
         }
         if (C.asro_every > 0 && (sw - C.burn_sweeps + 1) % C.asro_every == 0)
-          asro_sample<LAT, R, CW, NC>(lat, N, offl, S, tid, nthr, active, hacc);
+          asro_sample<LAT, R, CW, NC, FIELD>(lat, N, offl, S, tid, nthr, active, hacc);
       }
     }
     unsigned long long acc_sum = block_sum_u64(acc_local, redu);
@@ -986,50 +1002,56 @@ __global__ __launch_bounds__(512) void k_metro_cmp(const ParArgs *__restrict__ c
   }
 }
 
+// count words: 0 = 32-bit with 8-bit fields (4 species), 1 = 32-bit with
+// 6-bit fields (5 species; shells of at most 31 sites), 2 = 64-bit with
+// 8-bit fields (8 species)
 template <int LAT, int R>
-const void *pick(bool nat, bool wide) {
-  if (nat)
-    return wide ? (const void *)k_metro_nat<LAT, R, uint64_t, 0>
-                : (const void *)k_metro_nat<LAT, R, uint32_t, 0>;
-  return wide ? (const void *)k_metro_cmp<LAT, R, uint64_t, 8>
-              : (const void *)k_metro_cmp<LAT, R, uint32_t, 6>;
+const void *pick(bool nat, int words) {
+  if (nat) {
+    if (words == 0) return (const void *)k_metro_nat<LAT, R, uint32_t, 0, 8>;
+    if constexpr (R <= 6)
+      if (words == 1) return (const void *)k_metro_nat<LAT, R, uint32_t, 0, 6>;
+    return words == 2 ? (const void *)k_metro_nat<LAT, R, uint64_t, 0, 8> : nullptr;
+  }
+  return words == 2 ? (const void *)k_metro_cmp<LAT, R, uint64_t, 8>
+                    : (const void *)k_metro_cmp<LAT, R, uint32_t, 6>;
 }
 
 // Shapes with compile-time neighbour offsets: the quaternary bcc anneal
 // (bcc, 4 shells, 32^3 cells) and the binary fcc run (fcc, 2 shells, 8^3).
-const void *specialised(int lattice, int R, int n, bool wide) {
-  if (wide) return nullptr;
-  if (lattice == 2 && R == 4 && n == 32) return (const void *)k_metro_nat<2, 4, uint32_t, 32>;
-  if (lattice == 1 && R == 2 && n == 8) return (const void *)k_metro_nat<1, 2, uint32_t, 8>;
+const void *specialised(int lattice, int R, int n, int words) {
+  if (words != 0) return nullptr;
+  if (lattice == 2 && R == 4 && n == 32) return (const void *)k_metro_nat<2, 4, uint32_t, 32, 8>;
+  if (lattice == 1 && R == 2 && n == 8) return (const void *)k_metro_nat<1, 2, uint32_t, 8, 8>;
   return nullptr;
 }
 
-const void *par_kernel(int lattice, int R, int n, bool nat, bool wide) {
+const void *par_kernel(int lattice, int R, int n, bool nat, int words) {
   if (nat) {
-    const void *f = specialised(lattice, R, n, wide);
+    const void *f = specialised(lattice, R, n, words);
     if (f) return f;
   }
   if (lattice == 1) {
     switch (R) {
-      case 1: return pick<1, 1>(nat, wide);
-      case 2: return pick<1, 2>(nat, wide);
-      case 3: return pick<1, 3>(nat, wide);
-      case 4: return pick<1, 4>(nat, wide);
-      case 5: return pick<1, 5>(nat, wide);
-      case 6: return pick<1, 6>(nat, wide);
+      case 1: return pick<1, 1>(nat, words);
+      case 2: return pick<1, 2>(nat, words);
+      case 3: return pick<1, 3>(nat, words);
+      case 4: return pick<1, 4>(nat, words);
+      case 5: return pick<1, 5>(nat, words);
+      case 6: return pick<1, 6>(nat, words);
     }
   } else {
     switch (R) {
-      case 1: return pick<2, 1>(nat, wide);
-      case 2: return pick<2, 2>(nat, wide);
-      case 3: return pick<2, 3>(nat, wide);
-      case 4: return pick<2, 4>(nat, wide);
-      case 5: return pick<2, 5>(nat, wide);
-      case 6: return pick<2, 6>(nat, wide);
-      case 7: return pick<2, 7>(nat, wide);
-      case 8: return pick<2, 8>(nat, wide);
-      case 9: return pick<2, 9>(nat, wide);
-      case 10: return pick<2, 10>(nat, wide);
+      case 1: return pick<2, 1>(nat, words);
+      case 2: return pick<2, 2>(nat, words);
+      case 3: return pick<2, 3>(nat, words);
+      case 4: return pick<2, 4>(nat, words);
+      case 5: return pick<2, 5>(nat, words);
+      case 6: return pick<2, 6>(nat, words);
+      case 7: return pick<2, 7>(nat, words);
+      case 8: return pick<2, 8>(nat, words);
+      case 9: return pick<2, 9>(nat, words);
+      case 10: return pick<2, 10>(nat, words);
     }
   }
   return nullptr;
@@ -1037,10 +1059,18 @@ const void *par_kernel(int lattice, int R, int n, bool nat, bool wide) {
 
 }  // namespace
 
+int par_count_words(const Geo &g, bool nat) {
+  if (!nat) return g.S > 5 ? 2 : 1;
+  if (g.S <= 4) return 0;
+  return g.S == 5 && g.R <= 6 ? 1 : 2;  // the shells of R <= 6 hold <= 24 sites
+}
+
 size_t par_lds_bytes(const Geo &g, const NatGeo *nat, int threads) {
   const size_t red = sizeof(double) * (threads / 64 + 1);
-  // dtab rows as in k_metro_nat: R * S1P doubles made odd
-  const size_t S1P = g.S > 4 ? 7 : 3;
+  // dtab rows as in k_metro_nat: R * S1P doubles made odd, S1P the species
+  // a count word holds, less one
+  const int words = par_count_words(g, nat != nullptr);
+  const size_t S1P = words == 0 ? 3 : (words == 1 ? 4 : 7);
   const size_t rstr = (g.R * S1P) | 1;
   if (nat)
     return (size_t)((nat->lds_bytes + 15) & ~15) + sizeof(double) * g.S * g.S * rstr +
@@ -1050,12 +1080,12 @@ size_t par_lds_bytes(const Geo &g, const NatGeo *nat, int threads) {
          sizeof(int32_t) * g.nsub * g.z;
 }
 
-int par_max_threads(int lattice, int R, int n, bool nat, bool wide) {
-  return nat && specialised(lattice, R, n, wide) ? kMaxNatThreads : 512;
+int par_max_threads(const Geo &g, bool nat) {
+  return nat && specialised(g.lattice, g.R, g.n[0], par_count_words(g, true)) ? kMaxNatThreads : 512;
 }
 
-const void *par_kernel_for(int lattice, int R, int n, bool nat, bool wide) {
-  return par_kernel(lattice, R, n, nat, wide);
+const void *par_kernel_for(const Geo &g, bool nat) {
+  return par_kernel(g.lattice, g.R, g.n[0], nat, par_count_words(g, nat));
 }
 
 hipError_t launch_metro_par_fn(const void *fn, const ParArgs *args_dev, int n_chains, size_t lds,
@@ -1067,14 +1097,5 @@ hipError_t launch_metro_par_fn(const void *fn, const ParArgs *args_dev, int n_ch
   return hipLaunchKernel(fn, dim3(n_chains), dim3(threads), args, lds, s);
 }
 
-hipError_t launch_metro_par(const ParArgs *args_dev, int n_chains, int lattice, int R, int n,
-                            bool nat, size_t lds, int threads, bool wide, hipStream_t s) {
-  const void *fn = par_kernel(lattice, R, n, nat, wide);
-  if (!fn) return hipErrorInvalidValue;
-  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
-  void *args[] = {(void *)&args_dev};
-  return hipLaunchKernel(fn, dim3(n_chains), dim3(threads), args, lds, s);
-}
 
 }  // namespace bwl

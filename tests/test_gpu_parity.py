@@ -175,26 +175,39 @@ def test_sequential_mode_bit_exact_config1(hip, oracle, nbr_swap):
         assert mti == mti_ref and np.array_equal(mt, mt_ref)
 
 
-def test_parallel_sampler_conserves_composition_and_energy(hip, oracle):
-    """Full-size config 2 lattice (bcc 32^3, 65536 sites): a short anneal
-    keeps species counts, and the last recounted energy equals the oracle's
-    energy of the downloaded configuration."""
+@pytest.mark.parametrize("shape", ["config2", (1, (6, 6, 6), 5, 3), (2, (8, 8, 8), 5, 4),
+                                   (1, (5, 5, 5), 6, 2), (2, (6, 6, 6), 3, 6)])
+def test_parallel_sampler_conserves_composition_and_energy(hip, oracle, shape):
+    """A short anneal keeps species counts, and the energy it carries (the
+    recount at the start of the T step plus every accepted dE) equals the
+    oracle's energy of the downloaded configuration: the full-size config 2
+    lattice (bcc 32^3, its own kernel), and one shape per packed count word
+    (5 species in 6-bit fields, 5 and 6 species, 3 species with 6 shells)."""
     from brawl_amd import workloads
-    c = workloads.config2()
-    ref = oracle.OracleLattice(c.lattice, c.n, c.v_ex)
-    cfg = ref.initial_setup(oracle.Generator(110179), species_numbers=c.counts)
-    with hip.Lattice(c.lattice, c.n, c.v_ex) as L:
+    if shape == "config2":
+        c = workloads.config2()
+        lat, n, v, counts = c.lattice, c.n, c.v_ex, c.counts
+    else:
+        lat, n, S, R = shape
+        v = workloads.random_v(S, [25.0 / k for k in range(1, R + 1)], 41 + S)
+        counts = workloads.dirichlet_counts(workloads.n_atoms_of(lat, n), S, np.random.default_rng(S))
+    _say("case", shape)
+    ref = oracle.OracleLattice(lat, n, v)
+    N = ref.n_atoms
+    cfg = ref.initial_setup(oracle.Generator(110179), species_numbers=counts)
+    with hip.Lattice(lat, n, v) as L:
         L.set_config(cfg)
         res = L.metropolis_simulated_annealing(T=1500.0, n_sweeps=1, seed=5)
         out = L.get_config()
-        assert np.array_equal(np.bincount(out.ravel(), minlength=5), np.bincount(cfg.ravel(), minlength=5))
+        S1 = len(counts) + 1
+        assert np.array_equal(np.bincount(out.ravel(), minlength=S1), np.bincount(cfg.ravel(), minlength=S1))
         assert np.array_equal((out > 0), (cfg > 0))
-        e_gpu = res.energies_of_T[0] * c.n_atoms       # one sample: <E> = E_last
+        e_gpu = res.energies_of_T[0] * N                # one sample: <E> = E_last
         e_ref = ref.total_energy(out)
         assert abs(e_gpu - e_ref) <= 1e-11 * max(1.0, abs(e_ref))
         h = L.pair_histogram()
         assert np.array_equal(h, ref.histogram(out))
-        assert res.trials >= c.n_atoms
+        assert res.trials >= N
         assert 0 < res.accepted <= res.trials
 
 
