@@ -401,21 +401,32 @@ int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int
     if (t > cap) t = cap;
     return t < 64 ? 64 : t;
   };
-  // packed counts (par_count_words): 32-bit words with 8-bit fields up to
-  // 4 species, 6-bit fields for 5, else 64-bit words
-  bool use_nat = make_nat(g, nat);
-  threads = block_size(par_max_threads(g, use_nat));
-  lds = par_lds_bytes(g, use_nat ? &nat : nullptr, threads);
-  if (use_nat && lds > (size_t)lim) {
-    use_nat = false;
-    threads = block_size(par_max_threads(g, false));
-    lds = par_lds_bytes(g, nullptr, threads);
+  // Layouts in order of preference: the fixed-pitch natural layout (one
+  // kernel with compile-time offsets for all sizes of a lattice and shell
+  // count), unless the shape has a kernel of its own; the natural layout
+  // with this size's own pitch; the per-sublattice layout.  Packed counts
+  // (par_count_words): 32-bit words with 8-bit fields up to 4 species,
+  // 6-bit fields for 5, else 64-bit words.
+  NatGeo per{}, fix{};
+  const bool nat_ok = make_nat(g, per);
+  const bool fix_ok = nat_ok && !par_specialised(g) && make_nat_fixed(g, fix) &&
+                      par_fixed_kernel(g.lattice, g.R, par_count_words(g, true)) != nullptr;
+  const int modes[3] = {fix_ok ? 2 : -1, nat_ok ? 1 : -1, 0};
+  for (int mode : modes) {
+    if (mode < 0) continue;
+    const bool use_nat = mode != 0, fixed = mode == 2;
+    const NatGeo *ng = fixed ? &fix : (use_nat ? &per : nullptr);
+    threads = block_size(par_max_threads(g, use_nat, fixed));
+    lds = par_lds_bytes(g, ng, threads);
+    if (lds > (size_t)lim) continue;
+    if (ng) nat = *ng;
+    else memset(&nat, 0, sizeof(nat));
+    fn = par_kernel_for(g, use_nat, fixed);
+    if (!fn) return fail(BWL_ERR_UNSUPPORTED, "no kernel for this lattice");
+    if (natural) *natural = use_nat;
+    return BWL_OK;
   }
-  if (lds > (size_t)lim) return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
-  fn = par_kernel_for(g, use_nat);
-  if (!fn) return fail(BWL_ERR_UNSUPPORTED, "no kernel for this lattice");
-  if (natural) *natural = use_nat;
-  return BWL_OK;
+  return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
 }
 
 int anneal_plan(int n, bwl_ctx *const *ctxs, const bwl_anneal_params *params, AnnealPlan &P) {

@@ -119,27 +119,41 @@ constexpr int kNatBudget = 150 * 1024;  // LDS bytes the natural lattice may tak
 //  * otherwise P0 = 2 mod 4 and P1 odd (2-cell steps are odd numbers of
 //    dwords), lanes with the first free axis fastest.
 // Returns false when an axis is shorter than twice its halo.
-__host__ __device__ constexpr bool nat_dims(int lattice, int R, int n, NatGeo &G) {
-  int z = 0, rcut = 0;
+// Halo of the natural layout (largest |delta| per axis over shells 1..R),
+// the neighbour count z and the interaction range rcut (doubled units).
+__host__ __device__ constexpr void nat_halo(int lattice, int R, int (&h)[3], int &z, int &rcut) {
+  z = 0;
+  rcut = 0;
   for (int s = 0; s < R; ++s) z += lattice == 1 ? kFccShellSize[s] : kBccShellSize[s];
-  for (int t = 0; t < z; ++t)
-    for (int a = 0; a < 3; ++a) {
-      const int v = lattice == 1 ? kFccOffsets[t][a] : kBccOffsets[t][a];
-      rcut = v > rcut ? v : (-v > rcut ? -v : rcut);
-    }
-  const int need = rcut + 1, stride = need % 2 ? need + 1 : need;
-  G.D[0] = 2 * n;
-  G.D[1] = lattice == 1 ? 2 * n : n;
-  G.D[2] = n;
-  int h[3] = {0, 0, 0};
+  h[0] = h[1] = h[2] = 0;
   for (int t = 0; t < z; ++t) {
     int d[3] = {0, 0, 0};
     nat_delta(lattice, t, d[0], d[1], d[2]);
     for (int a = 0; a < 3; ++a) {
       const int v = d[a] < 0 ? -d[a] : d[a];
       if (v > h[a]) h[a] = v;
+      const int w = lattice == 1 ? kFccOffsets[t][a] : kBccOffsets[t][a];
+      rcut = w > rcut ? w : (-w > rcut ? -w : rcut);
     }
   }
+}
+
+__host__ __device__ constexpr void nat_offsets(int lattice, int z, NatGeo &G) {
+  for (int t = 0; t < kMaxNbr; ++t) G.off[t] = 0;
+  for (int t = 0; t < z; ++t) {
+    int da = 0, db = 0, dc = 0;
+    nat_delta(lattice, t, da, db, dc);
+    G.off[t] = (dc * G.P[1] + db) * G.P[0] + da;
+  }
+}
+
+__host__ __device__ constexpr bool nat_dims(int lattice, int R, int n, NatGeo &G) {
+  int z = 0, rcut = 0, h[3] = {0, 0, 0};
+  nat_halo(lattice, R, h, z, rcut);
+  const int need = rcut + 1, stride = need % 2 ? need + 1 : need;
+  G.D[0] = 2 * n;
+  G.D[1] = lattice == 1 ? 2 * n : n;
+  G.D[2] = n;
   for (int a = 0; a < 3; ++a) {
     G.h[a] = h[a];
     if (G.D[a] < 2 * h[a]) return false;
@@ -161,12 +175,62 @@ __host__ __device__ constexpr bool nat_dims(int lattice, int R, int n, NatGeo &G
     if (bcc4) G.lane_order = 4;
   }
   G.lds_bytes = G.P[0] * G.P[1] * G.P[2];
-  for (int t = 0; t < kMaxNbr; ++t) G.off[t] = 0;
-  for (int t = 0; t < z; ++t) {
-    int da = 0, db = 0, dc = 0;
-    nat_delta(lattice, t, da, db, dc);
-    G.off[t] = (dc * G.P[1] + db) * G.P[0] + da;
+  nat_offsets(lattice, z, G);
+  return true;
+}
+
+// Fixed-pitch natural layout: one row length P0 and plane count P1 for every
+// cubic supercell of a lattice and shell count up to nmax cells per axis
+// (the planes used are n + 2h), so that the neighbour offsets are the same
+// for all those sizes (compile-time constants of one kernel).  The pitches
+// come from a model of the LDS banks hit by a round's byte gathers (lanes on
+// the stride-s grid of the free axes), averaged over n = 8..28 (fcc) and
+// 8..32 (bcc): the rule of nat_dims (P0 = 2 mod 4, P1 odd) gives e.g. fcc
+// R3 (66, 65), where the 4-step along x moves a multiple of 32 dwords and
+// the gathers conflict 5-fold; (66, 69) conflicts 2-fold on average.
+// nmax is the largest n that fits the pitches and the LDS budget.
+__host__ __device__ constexpr void nat_fixed_pitch(int lattice, int R, int &p0, int &p1) {
+  constexpr int kFcc[4][2] = {{68, 63}, {66, 69}, {66, 69}, {66, 69}};
+  constexpr int kBcc[4][2] = {{84, 47}, {86, 43}, {82, 43}, {82, 41}};
+  const int r = R < 1 ? 0 : (R > 4 ? 3 : R - 1);
+  p0 = lattice == 1 ? kFcc[r][0] : kBcc[r][0];
+  p1 = lattice == 1 ? kFcc[r][1] : kBcc[r][1];
+}
+
+__host__ __device__ constexpr int nat_fixed_nmax(int lattice, int R) {
+  int h[3] = {0, 0, 0}, z = 0, rcut = 0;
+  nat_halo(lattice, R, h, z, rcut);
+  int p0 = 0, p1 = 0;
+  nat_fixed_pitch(lattice, R, p0, p1);
+  int best = 0;
+  for (int n = 1; n <= 64; ++n) {
+    const int d1 = lattice == 1 ? 2 * n : n;
+    if (2 * n + 2 * h[0] <= p0 && d1 + 2 * h[1] <= p1 && (n + 2 * h[2]) * p0 * p1 <= kNatBudget)
+      best = n;
   }
+  return best;
+}
+
+__host__ __device__ constexpr bool nat_dims_fixed(int lattice, int R, int n, NatGeo &G) {
+  int h[3] = {0, 0, 0}, z = 0, rcut = 0;
+  nat_halo(lattice, R, h, z, rcut);
+  const int nmax = nat_fixed_nmax(lattice, R);
+  G.D[0] = 2 * n;
+  G.D[1] = lattice == 1 ? 2 * n : n;
+  G.D[2] = n;
+  for (int a = 0; a < 3; ++a) {
+    G.h[a] = h[a];
+    if (G.D[a] < 2 * h[a]) return false;
+  }
+  if (n > nmax) return false;
+  int p0 = 0, p1 = 0;
+  nat_fixed_pitch(lattice, R, p0, p1);
+  G.P[0] = p0;
+  G.P[1] = p1;
+  G.P[2] = n + 2 * h[2];
+  G.lds_bytes = G.P[0] * G.P[1] * G.P[2];
+  G.lane_order = 7;
+  nat_offsets(lattice, z, G);
   return true;
 }
 

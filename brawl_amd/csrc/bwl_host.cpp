@@ -302,6 +302,12 @@ bool make_nat(const Geo &g, NatGeo &nat) {
   return nat_dims(g.lattice, g.R, g.n[0], nat);
 }
 
+bool make_nat_fixed(const Geo &g, NatGeo &nat) {
+  memset(&nat, 0, sizeof(nat));
+  if (g.n[0] != g.n[1] || g.n[1] != g.n[2]) return false;
+  return nat_dims_fixed(g.lattice, g.R, g.n[0], nat);
+}
+
 int check_config(const Geo &g, const int8_t *config, std::string &err) {
   const int m0 = g.m[0], m1 = g.m[1], m2 = g.m[2];
   for (int z = 0; z < m2; ++z)

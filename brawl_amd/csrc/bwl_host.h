@@ -45,6 +45,9 @@ int plan_parallel(const Geo &g, ParPlan &p, std::string &err);
 // Natural layout of a cubic supercell; false if the lattice is not cubic or
 // an axis is shorter than twice its halo.
 bool make_nat(const Geo &g, NatGeo &nat);
+// The fixed-pitch natural layout (nat_dims_fixed); false if the lattice is
+// not cubic, too small for the halo or larger than the pitch covers.
+bool make_nat_fixed(const Geo &g, NatGeo &nat);
 
 // lattice_shells (analytics.f90:205-275): the first `wc` distinct REAL(4)
 // distances from site (1,1,1) to the occupied sites (0 first), in doubled-

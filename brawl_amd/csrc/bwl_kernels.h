@@ -150,14 +150,19 @@ size_t seq_lds_bytes(const Geo &g);
 hipError_t launch_metro_seq(const SeqArgs &a, hipStream_t s);
 size_t par_lds_bytes(const Geo &g, const NatGeo *nat, int threads);
 // Block size limit of the parallel-sampler kernels: the natural-layout
-// kernels specialised for one lattice shape keep few enough registers for
-// 12 wavefronts per CU, the others are limited to 8.
+// kernels with compile-time neighbour offsets and 32-bit count words keep
+// few enough registers for 12 wavefronts per CU, the others are limited to 8.
 constexpr int kMaxNatThreads = 768;
-int par_max_threads(const Geo &g, bool nat);
+// fixed: the fixed-pitch natural layout (nat_dims_fixed)
+int par_max_threads(const Geo &g, bool nat, bool fixed);
 // packed count words of the sampler for a lattice (see par_count_words in
 // bwl_metro_par.hip): 0 = 32-bit / 8-bit fields, 1 = 32-bit / 6-bit, 2 = 64-bit
 int par_count_words(const Geo &g, bool nat);
-const void *par_kernel_for(const Geo &g, bool nat);
+const void *par_kernel_for(const Geo &g, bool nat, bool fixed);
+// whether a kernel with compile-time offsets exists for this exact shape
+bool par_specialised(const Geo &g);
+// the fixed-pitch kernel for (lattice, R, count words), nullptr if none
+const void *par_fixed_kernel(int lattice, int R, int words);
 hipError_t launch_metro_par_fn(const void *fn, const ParArgs *args_dev, int n_chains, size_t lds,
                                int threads, hipStream_t s);
 

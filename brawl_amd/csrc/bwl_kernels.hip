@@ -7,7 +7,7 @@
 //   k_site_energy          one site's energy in the reference's FP64 order
 //   k_metro_seq            bit-exact sequential Metropolis driven by the
 //                          reference's MT19937 stream (one wavefront)
-// (the parallel Metropolis sampler is in bwl_metro_par.hip)
+// (the parallel Metropolis sampler is in bwl_metro_kernels.h)
 #include <hip/hip_runtime.h>
 
 #include "bwl_kernels.h"
