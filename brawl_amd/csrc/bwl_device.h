@@ -62,6 +62,28 @@ __device__ inline void unstage_lattice(const int8_t *lat, int8_t *__restrict__ c
 // (lane `l`); a single thread passes l = 0, nl = 1.
 __device__ inline void write_site(int8_t *lat, const Geo &g, int s, int cx, int cy, int cz,
                                   int8_t v, int l, int nl) {
+  if (g.h <= g.n[0] && g.h <= g.n[1] && g.h <= g.n[2]) {
+    // halo no wider than the box (every lattice but the smallest): the first
+    // image is (c + h) mod n by one conditional subtract, there are at most
+    // 3 images per axis, and image q splits into (ix, iy, iz) by compares
+    // -- the same cells as the general loop below, without divisions
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    const int p0x = cx + g.h >= n0 ? cx + g.h - n0 : cx + g.h;
+    const int p0y = cy + g.h >= n1 ? cy + g.h - n1 : cy + g.h;
+    const int p0z = cz + g.h >= n2 ? cz + g.h - n2 : cz + g.h;
+    const int kx = 1 + (p0x + n0 < g.P[0]) + (p0x + 2 * n0 < g.P[0]);
+    const int ky = 1 + (p0y + n1 < g.P[1]) + (p0y + 2 * n1 < g.P[1]);
+    const int kz = 1 + (p0z + n2 < g.P[2]) + (p0z + 2 * n2 < g.P[2]);
+    const int kxy = kx * ky, total = kxy * kz;
+    for (int q = l; q < total; q += nl) {
+      const int iz = (q >= kxy) + (q >= 2 * kxy);
+      const int r = q - iz * kxy;
+      const int iy = (r >= kx) + (r >= 2 * kx);
+      const int ix = r - iy * kx;
+      lat[s * g.sub_stride + ((p0z + iz * n2) * g.P[1] + (p0y + iy * n1)) * g.P[0] + p0x + ix * n0] = v;
+    }
+    return;
+  }
   int p0x = (cx + g.h) % g.n[0], p0y = (cy + g.h) % g.n[1], p0z = (cz + g.h) % g.n[2];
   int kx = (g.P[0] - 1 - p0x) / g.n[0] + 1;
   int ky = (g.P[1] - 1 - p0y) / g.n[1] + 1;
@@ -250,6 +272,18 @@ __host__ __device__ inline bool lt_exp(double u, double x) {
   return u < rn;
 }
 
+// lt_exp on the device with a float exponential deciding first whenever u
+// is clear of it by 1e-4 (relative; the float's error is below 1e-6), which
+// leaves the FP64 and double-double paths to the rare close calls.
+__device__ inline bool lt_exp_dev(double u, double x) {
+  if (x > -80.0 && x < 80.0) {
+    const double ef = (double)__expf((float)x);
+    if (u < ef * (1.0 - 1e-4)) return true;
+    if (u > ef * (1.0 + 1e-4)) return false;
+  }
+  return lt_exp(u, x);
+}
+
 // log(u) for u in (0,1) as double-double: Newton step on exp.
 __host__ __device__ inline DD dd_log(double u) {
   double y0 = log(u);
@@ -270,6 +304,17 @@ __host__ __device__ inline bool log_lt(double u, double t) {
   DD d = dd_log(u);
   double rn = d.hi + d.lo;
   return rn < t;
+}
+
+// log_lt on the device with a float logarithm deciding first whenever it is
+// clear of t by 1e-4 (for u >= 2^-32 its error is below 1e-5).
+__device__ inline bool log_lt_dev(double u, double t) {
+  if (u >= 0x1p-32 && t > -1e30 && t < 1e30) {
+    const double lf = (double)__logf((float)u);
+    if (lf < t - 1e-4) return true;
+    if (lf > t + 1e-4) return false;
+  }
+  return log_lt(u, t);
 }
 
 }  // namespace bwl

@@ -171,26 +171,19 @@ __global__ __launch_bounds__(64) void k_metro_seq(SeqArgs A) {
     if (sp1 == sp2) {
       acc = true;  // metropolis.F90:774-777: counted as accepted, no draw
     } else {
-      double e_un = pair_energy_lanes(lat, g, offl, vex, pl, addr1, sub1, addr2, sub2, nullptr);
-      __syncthreads();
-      write_site(lat, g, sub1, s1.x >> 1, s1.y >> 1, s1.z >> 1, (int8_t)sp2, lane, 64);
-      write_site(lat, g, sub2, s2.x >> 1, s2.y >> 1, s2.z >> 1, (int8_t)sp1, lane, 64);
-      __syncthreads();
-      double e_sw = pair_energy_lanes(lat, g, offl, vex, pl, addr1, sub1, addr2, sub2, nullptr);
+      // energies before and after the swap from one gather (sites marked)
+      double e_un, e_sw;
+      put_pair(lat, g, s1, sub1, kMarkA, s2, sub2, kMarkB);
+      pair_energy_swap_lanes(lat, g, offl, vex, pl, addr1, sub1, addr2, sub2, sp1, sp2, e_un, e_sw);
       double de = e_sw - e_un;
       if (de < 0.0) {
         acc = true;
       } else {
         uint32_t r = mt_next(st, mti);
         double u = (double)r * (1.0 / 4294967296.0);
-        acc = lt_exp(u, -(beta * de));
+        acc = lt_exp_dev(u, -(beta * de));
       }
-      if (!acc) {
-        __syncthreads();
-        write_site(lat, g, sub1, s1.x >> 1, s1.y >> 1, s1.z >> 1, (int8_t)sp1, lane, 64);
-        write_site(lat, g, sub2, s2.x >> 1, s2.y >> 1, s2.z >> 1, (int8_t)sp2, lane, 64);
-      }
-      __syncthreads();
+      put_pair(lat, g, s1, sub1, (int8_t)(acc ? sp2 : sp1), s2, sub2, (int8_t)(acc ? sp1 : sp2));
     }
     if (acc) accepted++;
     if (A.energies && A.sample_every > 0 && trial % A.sample_every == 0) {

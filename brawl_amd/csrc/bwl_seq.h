@@ -82,6 +82,39 @@ __device__ __forceinline__ double fixed_pair_sum(const double (&vals)[6], double
   return eA + eB;
 }
 
+// Two ordered sums of fixed_pair_sum at once (independent chains, so the
+// adds of one fill the latency of the other), bit-identical to two calls.
+template <int LAT, int R>
+__device__ __forceinline__ void fixed_pair_sum2(const double (&v1)[6], const double (&v2)[6],
+                                                double &e1, double &e2) {
+  constexpr int z = shell_first<LAT>(R);
+  double a1 = 0.0, b1 = 0.0, a2 = 0.0, b2 = 0.0;
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int t = shell_first<LAT>(n); t < shell_first<LAT>(n + 1); ++t) {
+      s1 = s1 + readlane_d(v1[t >> 6], t & 63);
+      s2 = s2 + readlane_d(v2[t >> 6], t & 63);
+    }
+    a1 = n == 0 ? s1 : a1 + s1;
+    a2 = n == 0 ? s2 : a2 + s2;
+  }
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int t = z + shell_first<LAT>(n); t < z + shell_first<LAT>(n + 1); ++t) {
+      s1 = s1 + readlane_d(v1[t >> 6], t & 63);
+      s2 = s2 + readlane_d(v2[t >> 6], t & 63);
+    }
+    b1 = n == 0 ? s1 : b1 + s1;
+    b2 = n == 0 ? s2 : b2 + s2;
+  }
+  e1 = a1 + b1;
+  e2 = a2 + b2;
+}
+
 __device__ inline void init_pair_lanes(const Geo &g, PairLanes &pl) {
   const int lane = threadIdx.x & 63;
   const int z = g.z;
@@ -108,31 +141,12 @@ __device__ inline void init_pair_lanes(const Geo &g, PairLanes &pl) {
   pl.fixed = (g.lattice == 1 && g.R <= 6) || (g.lattice == 2 && g.R <= 6) ? key : 0;
 }
 
-// Energies of sites A and B in the reference's FP64 order (pair_energy,
-// bw_hamiltonian.f90:99-114): every lane gathers neighbour V entries (one
-// per 64-entry pass), then the entries are added in list order from 0.0
-// per shell, shells left to right per site, and e_A + e_B -- on every lane,
-// from the lanes' values.  `off` is the offset table and `vex` V_ex, both
-// in LDS.
-__device__ inline double pair_energy_lanes(const int8_t *lat, const Geo &g, const int32_t *off,
-                                           const double *vex, const PairLanes &pl, int addrA,
-                                           int subA, int addrB, int subB, double *eA_out) {
-  const int lane = threadIdx.x & 63;
-  const int z = g.z, S = g.S;
-  const int spA = lat[addrA], spB = lat[addrB];
-  double vals[6];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    vals[k] = 0.0;
-    const int t = 64 * k + lane;
-    if (k < pl.passes && t < 2 * z) {
-      const bool isA = t < z;
-      const int tt = isA ? t : t - z;
-      const int addr = isA ? addrA : addrB, sub = isA ? subA : subB, sp = isA ? spA : spB;
-      const int nb = lat[addr + off[sub * z + tt]];
-      vals[k] = vex[(sp - 1) + S * ((nb - 1) + S * pl.shell[k])];
-    }
-  }
+// The ordered sum of pair_energy_lanes from the lanes' values (entries
+// 0..z-1 site A's, z..2z-1 site B's): e_A + e_B, each site's shells left to
+// right, each shell from 0.0 in list order.
+__device__ inline double pair_sum(const Geo &g, const PairLanes &pl, const double (&vals)[6],
+                                  double *eA_out) {
+  const int z = g.z;
   switch (pl.fixed) {
     case 16 + 1: return fixed_pair_sum<1, 1>(vals, eA_out);
     case 16 + 2: return fixed_pair_sum<1, 2>(vals, eA_out);
@@ -173,9 +187,107 @@ __device__ inline double pair_energy_lanes(const int8_t *lat, const Geo &g, cons
   return eA + e;
 }
 
+// Both ordered sums of pair_energy_swap_lanes.
+__device__ inline void pair_sum2(const Geo &g, const PairLanes &pl, const double (&v1)[6],
+                                 const double (&v2)[6], double &e1, double &e2) {
+  switch (pl.fixed) {
+    case 16 + 1: fixed_pair_sum2<1, 1>(v1, v2, e1, e2); return;
+    case 16 + 2: fixed_pair_sum2<1, 2>(v1, v2, e1, e2); return;
+    case 16 + 3: fixed_pair_sum2<1, 3>(v1, v2, e1, e2); return;
+    case 16 + 4: fixed_pair_sum2<1, 4>(v1, v2, e1, e2); return;
+    case 16 + 5: fixed_pair_sum2<1, 5>(v1, v2, e1, e2); return;
+    case 16 + 6: fixed_pair_sum2<1, 6>(v1, v2, e1, e2); return;
+    case 32 + 1: fixed_pair_sum2<2, 1>(v1, v2, e1, e2); return;
+    case 32 + 2: fixed_pair_sum2<2, 2>(v1, v2, e1, e2); return;
+    case 32 + 3: fixed_pair_sum2<2, 3>(v1, v2, e1, e2); return;
+    case 32 + 4: fixed_pair_sum2<2, 4>(v1, v2, e1, e2); return;
+    case 32 + 5: fixed_pair_sum2<2, 5>(v1, v2, e1, e2); return;
+    case 32 + 6: fixed_pair_sum2<2, 6>(v1, v2, e1, e2); return;
+    default: break;
+  }
+  e1 = pair_sum(g, pl, v1, nullptr);
+  e2 = pair_sum(g, pl, v2, nullptr);
+}
+
+// Energies of sites A and B in the reference's FP64 order (pair_energy,
+// bw_hamiltonian.f90:99-114): every lane gathers neighbour V entries (one
+// per 64-entry pass), then the entries are added in list order from 0.0
+// per shell, shells left to right per site, and e_A + e_B -- on every lane,
+// from the lanes' values.  `off` is the offset table and `vex` V_ex, both
+// in LDS.
+__device__ inline double pair_energy_lanes(const int8_t *lat, const Geo &g, const int32_t *off,
+                                           const double *vex, const PairLanes &pl, int addrA,
+                                           int subA, int addrB, int subB, double *eA_out) {
+  const int lane = threadIdx.x & 63;
+  const int z = g.z, S = g.S;
+  const int spA = lat[addrA], spB = lat[addrB];
+  double vals[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    vals[k] = 0.0;
+    const int t = 64 * k + lane;
+    if (k < pl.passes && t < 2 * z) {
+      const bool isA = t < z;
+      const int tt = isA ? t : t - z;
+      const int addr = isA ? addrA : addrB, sub = isA ? subA : subB, sp = isA ? spA : spB;
+      const int nb = lat[addr + off[sub * z + tt]];
+      vals[k] = vex[(sp - 1) + S * ((nb - 1) + S * pl.shell[k])];
+    }
+  }
+  return pair_sum(g, pl, vals, eA_out);
+}
+
+// Marks of the two sites of a pair trial: while their energies before and
+// after the swap are gathered, the LDS lattice holds these codes at both
+// sites (all images) instead of their species.
+constexpr int8_t kMarkA = 0x41, kMarkB = 0x42;
+
+// pair_energy of sites A and B (species spA, spB) before and after the
+// exchange of their species, in the reference's order, from one gather: a
+// neighbour entry that is A or B itself (marked) takes that site's species
+// before or after the swap, every other entry its neighbour's species, so
+// both sums are bit-identical to gathers before and after a real swap.
+__device__ inline void pair_energy_swap_lanes(const int8_t *lat, const Geo &g, const int32_t *off,
+                                              const double *vex, const PairLanes &pl, int addrA,
+                                              int subA, int addrB, int subB, int spA, int spB,
+                                              double &e_before, double &e_after) {
+  const int lane = threadIdx.x & 63;
+  const int z = g.z, S = g.S;
+  double vb[6], va[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    vb[k] = 0.0;
+    va[k] = 0.0;
+    const int t = 64 * k + lane;
+    if (k < pl.passes && t < 2 * z) {
+      const bool isA = t < z;
+      const int tt = isA ? t : t - z;
+      const int addr = isA ? addrA : addrB, sub = isA ? subA : subB;
+      const int nb = lat[addr + off[sub * z + tt]];
+      const int nbb = nb == kMarkA ? spA : (nb == kMarkB ? spB : nb);
+      const int nba = nb == kMarkA ? spB : (nb == kMarkB ? spA : nb);
+      const int sb = isA ? spA : spB, sa = isA ? spB : spA;
+      vb[k] = vex[(sb - 1) + S * ((nbb - 1) + S * pl.shell[k])];
+      va[k] = vex[(sa - 1) + S * ((nba - 1) + S * pl.shell[k])];
+    }
+  }
+  pair_sum2(g, pl, vb, va, e_before, e_after);
+}
+
 struct SiteXYZ {
   int x, y, z;  // 0-based doubled-grid coordinates
 };
+
+// Write species va at site a and vb at site b (every image of each; the
+// whole wavefront helps), between barriers.
+__device__ inline void put_pair(int8_t *lat, const Geo &g, SiteXYZ a, int suba, int8_t va, SiteXYZ b,
+                                int subb, int8_t vb) {
+  const int lane = threadIdx.x & 63;
+  __syncthreads();
+  write_site(lat, g, suba, a.x >> 1, a.y >> 1, a.z >> 1, va, lane, 64);
+  write_site(lat, g, subb, b.x >> 1, b.y >> 1, b.z >> 1, vb, lane, 64);
+  __syncthreads();
+}
 
 // bcc_random_site / fcc_random_site (random_site.f90:127-139, 182-193):
 // floor(c*genrand()*n) equals (r*c*n) >> 32 exactly since r*c*n < 2^53.

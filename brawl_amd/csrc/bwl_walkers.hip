@@ -17,16 +17,6 @@ namespace bwl {
 
 namespace {
 
-// Swap the species of two sites (both images of each, all lanes help).
-__device__ inline void swap_sites(int8_t *lat, const Geo &g, SiteXYZ a, int suba, int8_t va,
-                                  SiteXYZ b, int subb, int8_t vb) {
-  const int lane = threadIdx.x & 63;
-  __syncthreads();
-  write_site(lat, g, suba, a.x >> 1, a.y >> 1, a.z >> 1, vb, lane, 64);
-  write_site(lat, g, subb, b.x >> 1, b.y >> 1, b.z >> 1, va, lane, 64);
-  __syncthreads();
-}
-
 // bin_index (wang-landau.F90:515-523): truncation toward zero, +1.
 __device__ inline int wl_bin_index(double e, const double *edges, int bins) {
   const double range = edges[bins] - edges[0];
@@ -155,18 +145,19 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
       }
       const int sub1 = sub_of(g.lattice, r1.x & 1, r1.y & 1);
       const int sub2 = sub_of(g.lattice, r2.x & 1, r2.y & 1);
-      const double eu = pair_energy_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, nullptr);
-      swap_sites(lat, g, r1, sub1, (int8_t)s1, r2, sub2, (int8_t)s2);
-      const double es = pair_energy_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, nullptr);
+      // energies before and after the swap from one gather (sites marked)
+      double eu, es;
+      put_pair(lat, g, r1, sub1, kMarkA, r2, sub2, kMarkB);
+      pair_energy_swap_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, eu, es);
       const double de = es - eu;
       ++trials;
-      if (ecur + de < limit) {
+      const bool acc = ecur + de < limit;
+      if (acc) {
         ecur = ecur + de;
         ++n_acc;
         ++accepted;
-      } else {
-        swap_sites(lat, g, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
       }
+      put_pair(lat, g, r1, sub1, (int8_t)(acc ? s2 : s1), r2, sub2, (int8_t)(acc ? s1 : s2));
     }
     __syncthreads();
     if (lane == 0) E[imax] = ecur;
@@ -222,6 +213,7 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
   const int bins = A.bins;
   const long long cadence = (long long)(int)(0.02 * (double)(float)g.n_atoms);
   double e_un = ref_total_energy(lat, g, offl, vex);
+  int ibin = wl_bin_index(e_un, edges, bins);  // bin of e_un, kept with it
   long long accepted = 0;
   const long long total = (long long)A.mc_sweeps * g.n_atoms;
   for (long long i = 1; i <= total; ++i) {
@@ -236,28 +228,31 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
     // are equal; neither changes the state, so both are skipped here.
     double e_sw = e_un;
     if (s1 != s2) {
-      const double pu = pair_energy_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, nullptr);
-      swap_sites(lat, g, r1, sub1, (int8_t)s1, r2, sub2, (int8_t)s2);
-      const double ps = pair_energy_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, nullptr);
+      // energies before and after the swap from one gather (sites marked)
+      double pu, ps;
+      put_pair(lat, g, r1, sub1, kMarkA, r2, sub2, kMarkB);
+      pair_energy_swap_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, pu, ps);
       e_sw = e_un - pu + ps;
     }
-    const int ibin = wl_bin_index(e_un, edges, bins);
-    int jbin = wl_bin_index(e_sw, edges, bins);
+    bool acc = false;
+    // e_sw == e_un when the species are equal (no swap): same bin
+    int jbin = s1 != s2 ? wl_bin_index(e_sw, edges, bins) : ibin;
     const bool hit = cadence > 0 && (i % cadence) == 0;
     if (jbin > A.start_idx - 1 && jbin < A.end_idx + 1) {
       const double li = (ibin >= 1 && ibin <= bins) ? lng[ibin - 1] : 0.0;
       const double u = genrand_dev(st, mti);
-      if (log_lt(u, li - lng[jbin - 1])) {
+      if (log_lt_dev(u, li - lng[jbin - 1])) {
+        acc = true;
         ++accepted;
         e_un = e_sw;
+        ibin = jbin;
       } else {
-        if (s1 != s2) swap_sites(lat, g, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
         jbin = ibin;
       }
     } else {
       jbin = ibin;
-      if (s1 != s2) swap_sites(lat, g, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
     }
+    if (s1 != s2) put_pair(lat, g, r1, sub1, (int8_t)(acc ? s2 : s1), r2, sub2, (int8_t)(acc ? s1 : s2));
     // a walker outside its window would index outside the reference's
     // arrays; such updates are dropped
     if (lane == 0) {
@@ -367,17 +362,17 @@ __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__
     if (s1 != s2) {
       const int sub1 = sub_of(g.lattice, r1.x & 1, r1.y & 1);
       const int sub2 = sub_of(g.lattice, r2.x & 1, r2.y & 1);
-      const double pu = pair_energy_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, nullptr);
-      swap_sites(lat, g, r1, sub1, (int8_t)s1, r2, sub2, (int8_t)s2);
-      const double ps = pair_energy_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, nullptr);
+      // energies before and after the swap from one gather (sites marked)
+      double pu, ps;
+      put_pair(lat, g, r1, sub1, kMarkA, r2, sub2, kMarkB);
+      pair_energy_swap_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, pu, ps);
       const double e_sw = e_un - pu + ps;
       const double de = ((e_sw - target) * (e_sw - target) - (e_un - target) * (e_un - target)) /
                         A.denom;
       const double u = genrand_dev(st, mti);
-      if (log_lt(u, -de))
-        e_un = e_sw;
-      else
-        swap_sites(lat, g, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
+      const bool acc = log_lt_dev(u, -de);
+      if (acc) e_un = e_sw;
+      put_pair(lat, g, r1, sub1, (int8_t)(acc ? s2 : s1), r2, sub2, (int8_t)(acc ? s1 : s2));
     }
   }
   __syncthreads();
