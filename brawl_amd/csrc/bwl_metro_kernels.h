@@ -1,5 +1,8 @@
 // bwl_metro_kernels.h -- parallel Metropolis sampler on an LDS-resident lattice
-// (one workgroup per chain; many chains per launch).
+// (one workgroup per chain; many chains per launch).  Kernel templates with
+// internal linkage: bwl_metro_par.hip instantiates the per-size and
+// per-sublattice variants, bwl_metro_fixed.hip the fixed-pitch ones (two
+// translation units that compile in parallel).
 //
 // Rounds.  A round picks a kind (RoundVar: axis A and layer offset u) and
 // two random origins; its active sites are two interleaved grids (grid 1 on
@@ -21,11 +24,13 @@
 //   fcc: (a,b,c) = (x, y, (z-x-y)/2 mod n)     box 2n x 2n x n
 //   bcc: (a,b,c) = (z, (x-z)/2 mod n, (y-z)/2 mod n)   box 2n x n x n
 // For n1 = n2 = n3 these boxes are plainly periodic, so with a halo every
-// neighbour of every site is at one fixed linear offset: the z offsets are
-// kernel constants held in registers and the hot loop is one LDS byte load
-// and one shift-add per neighbour.  Other shapes (or lattices too large for
-// LDS with this halo) use the per-sublattice layout with per-sublattice
-// offset tables (k_metro_cmp).
+// neighbour of every site is at one fixed linear offset: compile-time
+// immediates of the loads for single-shape kernels and for the fixed-pitch
+// layout (one pitch for every size up to a bound, see nat_dims_fixed),
+// kernel constants held in registers otherwise; the hot loop is one LDS
+// byte load and one shift-add per neighbour.  Other shapes (or lattices too
+// large for LDS with this halo) use the per-sublattice layout with
+// per-sublattice offset tables (k_metro_cmp).
 //
 // Energy change.  LDS holds bit positions FIELD*(species-1); a site's
 // shell-n neighbour counts are packed FIELD-bit fields.  dE is contracted
