@@ -154,18 +154,41 @@ __device__ inline void mt_twist(uint32_t *st) {
   __syncthreads();
 }
 
-__device__ inline uint32_t mt_next(uint32_t *st, int &mti) {
-  if (mti >= 624) {
-    mt_twist(st);
-    mti = 0;
-  }
-  uint32_t y = st[mti];
-  mti++;
+__device__ inline uint32_t mt_temper(uint32_t y) {
   y ^= y >> 11;
   y ^= (y << 7) & 0x9d2c5680u;
   y ^= (y << 15) & 0xefc60000u;
   y ^= y >> 18;
   return y;
+}
+
+// Three consecutive draws: when no twist falls between them the three state
+// words are read together (one round trip instead of three); the values are
+// those of three mt_next calls.
+__device__ inline void mt_next3(uint32_t *st, int &mti, uint32_t &r1, uint32_t &r2, uint32_t &r3);
+
+__device__ inline uint32_t mt_next(uint32_t *st, int &mti) {
+  if (mti >= 624) {
+    mt_twist(st);
+    mti = 0;
+  }
+  const uint32_t y = st[mti];
+  mti++;
+  return mt_temper(y);
+}
+
+__device__ inline void mt_next3(uint32_t *st, int &mti, uint32_t &r1, uint32_t &r2, uint32_t &r3) {
+  if (mti <= 621) {
+    const uint32_t y1 = st[mti], y2 = st[mti + 1], y3 = st[mti + 2];
+    mti += 3;
+    r1 = mt_temper(y1);
+    r2 = mt_temper(y2);
+    r3 = mt_temper(y3);
+    return;
+  }
+  r1 = mt_next(st, mti);
+  r2 = mt_next(st, mti);
+  r3 = mt_next(st, mti);
 }
 
 // ------------------------------------------------------------------

@@ -292,18 +292,15 @@ __device__ inline void put_pair(int8_t *lat, const Geo &g, SiteXYZ a, int suba, 
 // bcc_random_site / fcc_random_site (random_site.f90:127-139, 182-193):
 // floor(c*genrand()*n) equals (r*c*n) >> 32 exactly since r*c*n < 2^53.
 __device__ inline SiteXYZ random_site_dev(const Geo &g, uint32_t *st, int &mti) {
-  uint32_t r1 = mt_next(st, mti);
+  uint32_t r1, r2, r3;
+  mt_next3(st, mti, r1, r2, r3);
   int k = (int)(((uint64_t)r1 * (uint64_t)(2 * g.n[2])) >> 32) + 1;
   SiteXYZ s;
   if (g.lattice == 2) {
-    uint32_t r2 = mt_next(st, mti);
-    uint32_t r3 = mt_next(st, mti);
     int i = 2 * (int)(((uint64_t)r2 * (uint64_t)g.n[0]) >> 32) + 2 - (k & 1);
     int j = 2 * (int)(((uint64_t)r3 * (uint64_t)g.n[1]) >> 32) + 2 - (k & 1);
     s.x = i - 1; s.y = j - 1; s.z = k - 1;
   } else {
-    uint32_t r2 = mt_next(st, mti);
-    uint32_t r3 = mt_next(st, mti);
     int i = (int)(((uint64_t)r2 * (uint64_t)(2 * g.n[0])) >> 32) + 1;
     int j = 2 * (int)(((uint64_t)r3 * (uint64_t)g.n[1]) >> 32) + 1 + pmod(i - (k & 1), 2);
     s.x = i - 1; s.y = j - 1; s.z = k - 1;
