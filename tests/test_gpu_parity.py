@@ -176,13 +176,16 @@ def test_sequential_mode_bit_exact_config1(hip, oracle, nbr_swap):
 
 
 @pytest.mark.parametrize("shape", ["config2", (1, (6, 6, 6), 5, 3), (2, (8, 8, 8), 5, 4),
-                                   (1, (5, 5, 5), 6, 2), (2, (6, 6, 6), 3, 6)])
+                                   (1, (5, 5, 5), 6, 2), (2, (6, 6, 6), 3, 6),
+                                   (1, (29, 29, 29), 4, 3), (2, (39, 39, 39), 5, 3)])
 def test_parallel_sampler_conserves_composition_and_energy(hip, oracle, shape):
     """A short anneal keeps species counts, and the energy it carries (the
     recount at the start of the T step plus every accepted dE) equals the
     oracle's energy of the downloaded configuration: the full-size config 2
-    lattice (bcc 32^3, its own kernel), and one shape per packed count word
-    (5 species in 6-bit fields, 5 and 6 species, 3 species with 6 shells)."""
+    lattice (bcc 32^3, its own kernel), one shape per packed count word
+    (5 species in 6-bit fields, 5 and 6 species, 3 species with 6 shells),
+    and the largest boxes of the fixed-pitch layout (fcc 29^3 and bcc 39^3
+    with 3 shells)."""
     from brawl_amd import workloads
     if shape == "config2":
         c = workloads.config2()
