@@ -84,28 +84,74 @@ def calibrate_copy():
 
 
 def cpu_baseline(chain, budget_s: float):
-    """The oracle (single-core C restatement of the reference's serial
-    path: per-trial pair_energy, exp acceptance, full energy after every
-    sweep) on a bounded sample of the same anneal: the first temperature
-    step, as many sweeps as fit the time budget."""
+    """The reference's serial path on a bounded sample of the same anneal
+    (its first temperature step, as many sweeps as fit the time budget):
+    when the reference build is present (oracle/_ref, compiled from the
+    reference's sources: its own rdm_site, pair_energy, pair_swap, genrand
+    in the order of monte_carlo_step_lattice, full_energy after every sweep,
+    timed around the trial loops), kind "reference"; the oracle (the C
+    restatement, "port") is timed beside it.  Also the same sample as
+    independent chains, one per available core (the reference's MPI replica
+    mode), for the aggregate rate.  Runs before the GPU is touched: the
+    reference runs in forked processes."""
+    from concurrent.futures import ThreadPoolExecutor
+    from brawl_amd.lattice import beta_of
     from oracle import oracle as orc
+    from oracle import ref as oref
     orc.build()
-    ref = orc.OracleLattice(chain.lattice, chain.n, chain.v_ex)
+    lat = orc.OracleLattice(chain.lattice, chain.n, chain.v_ex)
     gen = orc.Generator(110179)
-    cfg = ref.initial_setup(gen, species_numbers=chain.counts)
-    N = ref.n_atoms
-    # calibrate on one sweep, then run a sample of ~budget_s seconds
+    cfg = lat.initial_setup(gen, species_numbers=chain.counts)
+    N = lat.n_atoms
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    # the oracle: calibrate on one sweep, then a sample of ~budget/2 seconds
+    c = np.array(cfg, order="F", copy=True)
     t0 = time.perf_counter()
-    ref.anneal(cfg, gen, T=chain.T, n_mc_steps=N, n_sample_steps=N)
+    lat.anneal(c, gen, T=chain.T, n_mc_steps=N, n_sample_steps=N)
     one = time.perf_counter() - t0
-    sweeps = max(1, int(budget_s / max(one, 1e-6)))
+    sweeps = max(1, int(0.5 * budget_s / max(one, 1e-6)))
     t0 = time.perf_counter()
-    ref.anneal(cfg, gen, T=chain.T, n_mc_steps=sweeps * N, n_sample_steps=N)
+    lat.anneal(c, gen, T=chain.T, n_mc_steps=sweeps * N, n_sample_steps=N)
     dt = time.perf_counter() - t0
-    trials = sweeps * N
-    return {"value": trials / dt, "unit": "swap trials/s", "cores": 1, "kind": "port",
-            "sample": f"first T step (3000 K) of the same anneal, {sweeps} sweeps = {trials} "
+    port = {"value": sweeps * N / dt, "unit": "swap trials/s", "cores": 1, "kind": "port",
+            "sample": f"first T step (3000 K) of the same anneal, {sweeps} sweeps = {sweeps * N} "
                       f"trials with the reference's per-sweep energy recount, {dt:.1f} s"}
+    if not oref.available():
+        rsweeps = max(1, sweeps // 2)
+
+        def replica(k):
+            lk = orc.OracleLattice(chain.lattice, chain.n, chain.v_ex)
+            g = orc.Generator(110179 + 11 * k)
+            ck = lk.initial_setup(g, species_numbers=chain.counts)
+            lk.anneal(ck, g, T=chain.T, n_mc_steps=rsweeps * N, n_sample_steps=N)
+            return rsweeps * N
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=cores) as ex:
+            rtrials = sum(ex.map(replica, range(cores)))
+        rdt = time.perf_counter() - t0
+        port["replicas"] = {"value": rtrials / rdt, "cores": cores,
+                            "sample": f"{cores} oracle chains (one per core), {rsweeps} sweeps "
+                                      f"each, {rdt:.1f} s"}
+        return port
+    beta = beta_of(chain.T)
+    _, _, one, _ = oref.metropolis_sweeps(chain.lattice, chain.n, chain.v_ex, cfg, beta, 1)
+    rs = max(1, int(budget_s / max(one, 1e-6)))
+    _, _, secs, _ = oref.metropolis_sweeps(chain.lattice, chain.n, chain.v_ex, cfg, beta, rs)
+    rsweeps = max(1, rs // 2)
+    t0 = time.perf_counter()
+    reps = oref.metropolis_sweeps_replicas(chain.lattice, chain.n, chain.v_ex, [cfg] * cores, beta,
+                                           rsweeps)
+    rdt = time.perf_counter() - t0
+    return {"value": rs * N / secs, "unit": "swap trials/s", "cores": 1, "kind": "reference",
+            "sample": f"first T step (3000 K) of the same anneal through the reference's own "
+                      f"compiled routines, {rs} sweeps = {rs * N} trials with the per-sweep "
+                      f"energy recount, {secs:.1f} s in the trial loops",
+            "replicas": {"value": cores * rsweeps * N / rdt, "cores": cores,
+                         "sample": f"{cores} reference chains in parallel processes (ranks "
+                                   f"0..{cores - 1}), {rsweeps} sweeps each, {rdt:.1f} s wall "
+                                   f"(longest chain {max(r[2] for r in reps):.1f} s in its loops)"},
+            "port": port}
 
 
 def batched(steps: int, warmup: int):
@@ -130,18 +176,41 @@ def batched(steps: int, warmup: int):
         res, mk = anneal_batch(lats, params)
         dt = time.perf_counter() - t0
         tr = sum(r.trials for r in res)
+        # algorithmic bytes: B_trial of each chain's shells (3: fcc z=42, bcc z=26)
+        alg = sum(r.trials * bytes_per_trial(42 if ch.lattice == 1 else 26)
+                  for r, ch in zip(res, chains))
         log(f"[bench] batched {'warmup' if k < warmup else 'step'}: {tr} trials, "
             f"makespan {mk:.1f} ms, wall {dt * 1e3:.1f} ms")
         if k >= warmup:
             rates.append(tr / dt)
             mks.append(mk)
             trials = tr
+            alg_bytes = alg
     for L in lats:
         L.close()
     return {"workload": "256 chains, L~U{8..24} fcc/bcc, S~U{2..5}, 3 shells, T~U[200,2000] K, "
                         "500 sweeps", "chains": len(chains), "trials": trials,
             "value": float(np.mean(rates)), "unit": "swap trials/s",
-            "device_rate": trials / (np.mean(mks) / 1e3), "makespan_ms": float(np.mean(mks))}
+            "device_rate": trials / (np.mean(mks) / 1e3), "makespan_ms": float(np.mean(mks)),
+            "bytes_per_step": int(alg_bytes)}
+
+
+def lds_roofline(plain_gbs: float, batched_out=None):
+    """The samplers' lattices live in LDS: their algorithmic bytes against
+    the LDS read bandwidth measured by the library's probe (all CUs, and one
+    CU's share for the one-chain plain form)."""
+    from brawl_amd import _lib
+    bps, cus = _lib.lds_bandwidth()
+    peak = bps / 1e9
+    out = {"bound": "lds", "unit": "GB/s", "peak": peak, "cus": cus, "peak_per_cu": peak / cus,
+           "probe": "conflict-free 16-byte LDS reads, 8 blocks x 1024 threads per CU",
+           "plain_achieved": plain_gbs, "plain_frac": plain_gbs / peak,
+           "plain_frac_of_one_cu": plain_gbs / (peak / cus)}
+    if batched_out:
+        a = batched_out["bytes_per_step"] / (batched_out["makespan_ms"] / 1e3) / 1e9
+        out["batched_achieved"] = a
+        out["batched_frac"] = a / peak
+    return out
 
 
 def _served_submit(srv, r):
@@ -315,8 +384,7 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(the multi-GPU layout is described in DESIGN.md)"}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
 
     from brawl_amd import Lattice, workloads
@@ -327,6 +395,13 @@ def main():
         return
     chain = workloads.config2()
     log(f"[bench] workload {chain.name}: N={chain.n_atoms} S={chain.S} R={chain.R}")
+    cpu = None
+    if not args.no_cpu:
+        # before the GPU is touched (the reference baseline forks)
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(chain, args.cpu_budget)
+        log(f"[bench] cpu baseline: {cpu['value']:.3e} trials/s ({cpu['kind']}), "
+            f"{cpu['replicas']['value']:.3e} with {cpu['replicas']['cores']} chains")
     # start configuration: equiatomic random occupation (reference initial_setup)
     lat = Lattice(chain.lattice, chain.n, chain.v_ex)
     lat.seed(110179)
@@ -398,11 +473,8 @@ def main():
         out["served"] = served(args)
     if not args.no_batched:
         out["wang_landau_walkers"] = wang_landau_walkers()
-    if not args.no_cpu:
-        log("[bench] cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(chain, args.cpu_budget)
-    else:
-        out["cpu_baseline"] = None
+    out["roofline_lds"] = lds_roofline(achieved, out.get("batched"))
+    out["cpu_baseline"] = cpu
     print(json.dumps(out))
 
 

@@ -33,7 +33,7 @@ EXPORTS = (
     "bwl_wl_genrand_set_state", "bwl_wl_get_lng", "bwl_wl_set_lng", "bwl_wl_enter_windows",
     "bwl_wl_sweeps", "bwl_server_create", "bwl_server_destroy", "bwl_server_submit_anneal",
     "bwl_server_submit_ns", "bwl_server_submit_wl", "bwl_server_progress", "bwl_server_poll",
-    "bwl_server_info", "bwl_server_drain", "bwl_server_memory",
+    "bwl_server_info", "bwl_server_drain", "bwl_server_memory", "bwl_lds_bandwidth",
 )
 
 
@@ -150,6 +150,7 @@ def load():
         "bwl_server_info": (C.c_int, [vp, C.c_int64, P(RequestInfo)]),
         "bwl_server_drain": (C.c_int, [vp]),
         "bwl_server_memory": (C.c_int, [vp, i64p, i64p]),
+        "bwl_lds_bandwidth": (C.c_int, [C.c_int32, dp, i32p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -169,3 +170,11 @@ def device_count() -> int:
     n = C.c_int32(0)
     load().bwl_device_count(C.byref(n))
     return int(n.value)
+
+
+def lds_bandwidth(iters: int = 20000):
+    """(aggregate LDS read bytes/s over all CUs, number of CUs) measured by
+    the library's probe kernel."""
+    bps, cus = C.c_double(0.0), C.c_int32(0)
+    check(load().bwl_lds_bandwidth(int(iters), C.byref(bps), C.byref(cus)))
+    return float(bps.value), int(cus.value)

@@ -26,6 +26,33 @@ int bwl_device_count(int32_t *count) {
   return e == hipSuccess ? BWL_OK : fail(BWL_ERR_DEVICE, hipGetErrorString(e));
 }
 
+int bwl_lds_bandwidth(int32_t iters, double *bytes_per_s, int32_t *cus) {
+  if (iters < 1 || !bytes_per_s || !cus) return fail(BWL_ERR_ARG, "bad argument");
+  int dev = 0, n_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      n_cu < 1)
+    return fail(BWL_ERR_DEVICE, "no HIP device");
+  // two 64 KiB, 1024-thread blocks per CU run at once; 8 blocks per CU in all
+  const int blocks = 8 * n_cu;
+  DevBuf sink;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool ok = hipMalloc(&sink.p, sizeof(uint32_t) * blocks) == hipSuccess &&
+            hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+            launch_lds_bandwidth(blocks, 16, (uint32_t *)sink.p, 0) == hipSuccess &&  // warm-up
+            hipEventRecord(e0, 0) == hipSuccess &&
+            launch_lds_bandwidth(blocks, iters, (uint32_t *)sink.p, 0) == hipSuccess &&
+            hipEventRecord(e1, 0) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+  float ms = 0.0f;
+  if (ok) ok = hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0.0f;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (!ok) return fail(BWL_ERR_DEVICE, "LDS bandwidth probe failed");
+  *bytes_per_s = (double)blocks * 1024.0 * (double)iters * 16.0 / ((double)ms * 1e-3);
+  *cus = n_cu;
+  return BWL_OK;
+}
+
 int bwl_create(const bwl_lattice_desc *desc, const double *v_ex, bwl_ctx **out) {
   if (!desc || !v_ex || !out) return fail(BWL_ERR_ARG, "null argument");
   *out = nullptr;

@@ -141,6 +141,8 @@ hipError_t launch_pack(const int8_t *ref, int8_t *compact, const Geo &g, hipStre
 hipError_t launch_pack_many(const int8_t *ref, int8_t *compact, const Geo &g, int count,
                             hipStream_t s);
 hipError_t launch_unpack(const int8_t *compact, int8_t *ref, const Geo &g, hipStream_t s);
+// LDS read-bandwidth probe: blocks x 1024 threads, iters 16-byte reads each
+hipError_t launch_lds_bandwidth(int blocks, int iters, uint32_t *sink, hipStream_t s);
 size_t hist_lds_bytes(const Geo &g, int threads);
 hipError_t launch_histogram(const HistArgs *args_dev, int n, const Geo &g_max, int threads,
                             hipStream_t s);

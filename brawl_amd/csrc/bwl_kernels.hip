@@ -273,4 +273,32 @@ hipError_t launch_metro_seq(const SeqArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------
+// LDS read bandwidth probe (measurement only): each block fills 64 KiB of
+// LDS, then every lane streams 16-byte reads, a wavefront reading 1 KiB of
+// consecutive addresses per step (no bank conflicts).  The index is made
+// opaque to the compiler each step so no read can be merged away; the
+// folded values are written only if they hit one improbable pattern.
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_lds_bandwidth(int iters, uint32_t *sink) {
+  __shared__ uint4 buf[4096];
+  const int t = threadIdx.x;
+  for (int i = t; i < 4096; i += blockDim.x) buf[i] = make_uint4(i, 3u * i, 5u * i, 7u * i);
+  __syncthreads();
+  uint32_t acc = 0;
+  int idx = t & 4095;
+  for (int k = 0; k < iters; ++k) {
+    const uint4 v = buf[idx];
+    acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    idx = (idx + 1024) & 4095;
+    asm volatile("" : "+v"(idx));
+  }
+  if (acc == 0x9e3779b9u) sink[blockIdx.x] = acc;
+}
+
+hipError_t launch_lds_bandwidth(int blocks, int iters, uint32_t *sink, hipStream_t s) {
+  hipLaunchKernelGGL(k_lds_bandwidth, dim3(blocks), dim3(1024), 0, s, iters, sink);
+  return hipGetLastError();
+}
+
 }  // namespace bwl

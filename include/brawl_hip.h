@@ -349,6 +349,16 @@ int bwl_server_info(bwl_server *s, int64_t id, bwl_request_info *info);
 int bwl_server_drain(bwl_server *s);
 int bwl_server_memory(const bwl_server *s, int64_t *used_bytes, int64_t *total_bytes);
 
+/* ---------------------------------------------------------------- */
+/* measurement                                                        */
+/* ---------------------------------------------------------------- */
+/* No counterpart in the reference: the LDS read bandwidth of the current
+ * device, for the LDS roofline of the samplers (whose lattices live in
+ * LDS).  Every CU streams conflict-free 16-byte LDS reads (8 blocks of 1024
+ * threads per CU, `iters` reads per thread); bytes_per_s is the aggregate
+ * over all CUs, cus the number of CUs. */
+int bwl_lds_bandwidth(int32_t iters, double *bytes_per_s, int32_t *cus);
+
 #ifdef __cplusplus
 }
 #endif

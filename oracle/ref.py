@@ -47,6 +47,8 @@ def lib():
             "ref_initial_setup": (None, [i32] * 6 + [i64p, i8p]),
             "ref_random_sites": (None, [i32] * 6 + [i32p]),
             "ref_radial_densities": (None, [i32] * 6 + [i8p, dp, dp]),
+            "ref_metropolis_sweeps": (C.c_int64, [i32] * 6 + [dp, i8p, C.c_double, i32, i32,
+                                                             dp, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -139,3 +141,53 @@ def radial_densities(lattice, n, S, wc, cfg):
     rho = np.zeros((S, S, wc), order="F")
     lib().ref_radial_densities(lattice, *n, S, wc, cp, _p(shells, C.c_double), _p(rho, C.c_double))
     return shells, rho
+
+
+def _metropolis_sweeps(lattice, n, v_ex, cfg, beta, n_sweeps, rank=0):
+    """The reference's serial Metropolis path (ref_metropolis_sweeps):
+    n_sweeps sweeps of n_atoms trials at beta after seeding for `rank`.
+    Returns (accepted, last full energy, seconds of the trial loops, final
+    configuration)."""
+    v = np.asfortranarray(v_ex, dtype=np.float64)
+    S, _, R = v.shape
+    c, cp = _cfg(np.array(cfg, dtype=np.int8, order="F", copy=True))
+    e, secs = C.c_double(0.0), C.c_double(0.0)
+    acc = lib().ref_metropolis_sweeps(lattice, *n, S, R, _p(v, C.c_double), cp, float(beta),
+                                      int(n_sweeps), int(rank), C.byref(e), C.byref(secs))
+    return int(acc), float(e.value), float(secs.value), np.array(c)
+
+
+metropolis_sweeps = _isolated(_metropolis_sweeps)
+
+
+def metropolis_sweeps_replicas(lattice, n, v_ex, cfgs, beta, n_sweeps):
+    """One reference chain per configuration, each in its own forked process
+    (rank k for cfgs[k]), all running at once; returns their results as
+    metropolis_sweeps does, in order."""
+    ctx = mp.get_context("fork")
+    procs = []
+    for k, cfg in enumerate(cfgs):
+        rd, wr = ctx.Pipe(duplex=False)
+
+        def child(wr=wr, k=k, cfg=cfg):
+            try:
+                wr.send(("ok", _metropolis_sweeps(lattice, n, v_ex, cfg, beta, n_sweeps, rank=k)))
+            except BaseException as e:  # report, never hang the parent
+                wr.send(("err", repr(e)))
+            wr.close()
+
+        p = ctx.Process(target=child)
+        p.start()
+        wr.close()
+        procs.append((p, rd))
+    out = []
+    for p, rd in procs:
+        try:
+            status, val = rd.recv()
+        except EOFError:
+            status, val = "err", f"exit code {p.exitcode}"
+        p.join()
+        if status != "ok":
+            raise RuntimeError(f"reference replica failed: {val}")
+        out.append(val)
+    return out

@@ -16,6 +16,8 @@ module ref_shim
   use c_functions, only : f90_init_genrand, genrand
   use initialise, only : initialise_function_pointers, initial_setup
   use analytics, only : lattice_shells, radial_densities
+  use bw_hamiltonian, only : pair_energy
+  use random_site, only : pair_swap
   implicit none
 
 contains
@@ -153,5 +155,64 @@ contains
     shells_out(1:wc) = shells
     rho(1:ns*ns*wc) = reshape(r, [ns*ns*wc])
   end subroutine ref_radial_densities
+
+  !> The reference's serial Metropolis path for the CPU baseline: after
+  !> f90_init_genrand for `rank`, n_sweeps sweeps of n_atoms Kawasaki trials
+  !> at inverse temperature beta, each trial made of the reference's own
+  !> routines in the order of monte_carlo_step_lattice
+  !> (metropolis.F90:751-813: setup%rdm_site twice, equal species accepted,
+  !> pair_energy, pair_swap, pair_energy, genrand against exp(-beta dE),
+  !> pair_swap back on rejection), and setup%full_energy after every sweep
+  !> as the reference's sampling does (metropolis.F90:361).  cfg in/out;
+  !> returns the accepted count, the last energy in e_last and the seconds
+  !> of the trial loops in seconds.
+  function ref_metropolis_sweeps(lattice, n1, n2, n3, ns, nr, vex, cfg, beta, n_sweeps, rank, &
+                                 e_last, seconds) result(acc) bind(C, name="ref_metropolis_sweeps")
+    integer(c_int32_t), value :: lattice, n1, n2, n3, ns, nr, n_sweeps, rank
+    real(c_double), value :: beta
+    real(c_double), intent(in) :: vex(*)
+    integer(c_int8_t), intent(inout) :: cfg(*)
+    real(c_double), intent(out) :: e_last, seconds
+    integer(c_int64_t) :: acc
+    type(run_params) :: setup
+    integer(c_int) :: seed
+    integer :: sweep, t, nc, n_atoms
+    integer, dimension(4) :: r1, r2
+    real(real64) :: e_u, e_s, de
+    integer(int64) :: c0, c1, rate
+    call make_setup(setup, lattice, n1, n2, n3, ns, nr)
+    call load(setup, vex, cfg)
+    seed = f90_init_genrand(0_c_int, int(rank, c_int), 0_c_int)
+    n_atoms = count(config /= 0_array_int)
+    acc = 0_c_int64_t
+    e_last = 0.0_c_double
+    call system_clock(c0, rate)
+    do sweep = 1, n_sweeps
+      do t = 1, n_atoms
+        r1 = setup%rdm_site()
+        r2 = setup%rdm_site()
+        if (config(r1(1), r1(2), r1(3), r1(4)) == config(r2(1), r2(2), r2(3), r2(4))) then
+          acc = acc + 1
+          cycle
+        end if
+        e_u = pair_energy(setup, config, r1, r2)
+        call pair_swap(config, r1, r2)
+        e_s = pair_energy(setup, config, r1, r2)
+        de = e_s - e_u
+        if (de < 0.0_real64) then
+          acc = acc + 1
+        else if (genrand() < exp(-beta * de)) then
+          acc = acc + 1
+        else
+          call pair_swap(config, r1, r2)
+        end if
+      end do
+      e_last = setup%full_energy(config)
+    end do
+    call system_clock(c1)
+    seconds = real(c1 - c0, c_double) / real(rate, c_double)
+    nc = size(config)
+    cfg(1:nc) = int(reshape(config, [nc]), c_int8_t)
+  end function ref_metropolis_sweeps
 
 end module ref_shim

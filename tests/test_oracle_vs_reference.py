@@ -88,3 +88,28 @@ def test_radial_densities_bit_exact_against_reference(oracle, lat, n, S, wc):
     fin = np.isfinite(rho_ref)
     assert np.array_equal(np.isfinite(rho), fin)
     assert np.array_equal(rho[fin], rho_ref[fin])
+
+
+@pytest.mark.parametrize("lat,n,S,Rs,T,rank", [(1, (8, 8, 8), 2, 2, 600.0, 0), (2, (4, 4, 4), 4, 4, 900.0, 1),
+                                               (1, (3, 4, 5), 5, 3, 2000.0, 2)])
+def test_metropolis_path_bit_exact_against_reference(oracle, lat, n, S, Rs, T, rank):
+    """The reference's own trial sequence (its rdm_site, pair_energy,
+    pair_swap and genrand in the order of monte_carlo_step_lattice, full
+    energy per sweep) against the oracle's mc_step for 3 sweeps from the
+    same configuration and stream: accepted count, final configuration and
+    final energy identical."""
+    from brawl_amd import workloads
+    from brawl_amd.lattice import beta_of
+    v = workloads.random_v(S, [25.0 / k for k in range(1, Rs + 1)], 5 * Rs + S)
+    o = oracle.OracleLattice(lat, n, v)
+    N = o.n_atoms
+    cfg0 = o.initial_setup(oracle.Generator(7),
+                           species_numbers=workloads.dirichlet_counts(N, S, np.random.default_rng(S)))
+    beta = beta_of(T)
+    acc_r, e_r, _, cfg_r = R.metropolis_sweeps(lat, n, v, cfg0, beta, 3, rank=rank)
+    cfg = np.array(cfg0, order="F", copy=True)
+    gen = oracle.Generator(110179 + 11 * rank)
+    acc = sum(o.mc_step(cfg, beta, gen, False) for _ in range(3 * N))
+    assert acc == acc_r
+    assert np.array_equal(cfg, cfg_r)
+    assert o.total_energy(cfg) == e_r
