@@ -31,7 +31,8 @@ EXPORTS = (
     "bwl_wl_n_walkers", "bwl_wl_windows", "bwl_wl_hist_stride", "bwl_wl_initial_setup",
     "bwl_wl_set_walker", "bwl_wl_get_walker", "bwl_wl_genrand_get_state",
     "bwl_wl_genrand_set_state", "bwl_wl_get_lng", "bwl_wl_set_lng", "bwl_wl_enter_windows",
-    "bwl_wl_sweeps", "bwl_server_create", "bwl_server_destroy", "bwl_server_submit_anneal",
+    "bwl_wl_sweeps", "bwl_wl_replica_exchange", "bwl_server_create", "bwl_server_destroy",
+    "bwl_server_submit_anneal",
     "bwl_server_submit_ns", "bwl_server_submit_wl", "bwl_server_progress", "bwl_server_poll",
     "bwl_server_info", "bwl_server_drain", "bwl_server_memory", "bwl_lds_bandwidth",
 )
@@ -137,6 +138,7 @@ def load():
         "bwl_wl_set_lng": (C.c_int, [vp, dp, dp]),
         "bwl_wl_enter_windows": (C.c_int, [vp, C.c_int64, i64p, i32p]),
         "bwl_wl_sweeps": (C.c_int, [vp, C.c_double, C.c_int32, i64p, dp]),
+        "bwl_wl_replica_exchange": (C.c_int, [vp, i32p, i32p, dp]),
         "bwl_server_create": (C.c_int, [P(ServerParams), P(vp)]),
         "bwl_server_destroy": (C.c_int, [vp]),
         "bwl_server_submit_anneal": (C.c_int, [vp, P(LatticeDesc), dp, i8p, P(AnnealParams), dp, dp,

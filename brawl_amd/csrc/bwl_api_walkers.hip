@@ -59,6 +59,9 @@ struct bwl_wl {
   double *d_edges = nullptr, *d_lng = nullptr, *d_hist = nullptr, *d_e = nullptr;
   long long *d_isteps = nullptr, *d_trials = nullptr, *d_acc = nullptr;
   int32_t *d_resume = nullptr, *d_status = nullptr;
+  // replica exchange: windows, walker energies, decisions
+  int32_t *d_windows = nullptr, *d_partner = nullptr, *d_xacc = nullptr, *d_swaps = nullptr;
+  double *d_xe = nullptr;
   WlArgs *d_args = nullptr;
   WlEnterArgs *d_eargs = nullptr;
   bool has_config = false;
@@ -95,6 +98,34 @@ int wl_push_mt(bwl_wl *w) {
   HIP_OK(hipMemcpyAsync(w->d_mti, mti.data(), mti.size() * 4, hipMemcpyHostToDevice, s));
   HIP_OK(hipStreamSynchronize(s));
   return BWL_OK;
+}
+
+// The per-walker arguments of k_wl_sweeps (also read by k_wl_energies).
+void wl_fill_args(bwl_wl *w, double wl_f, std::vector<WlArgs> &args) {
+  const Geo &g = w->ctx->g;
+  const int W = w->W;
+  args.resize((size_t)W);
+  for (int k = 0; k < W; ++k) {
+    const int win = k / w->p.walkers_per_window;
+    WlArgs &a = args[(size_t)k];
+    memset(&a, 0, sizeof(a));
+    a.g = g;
+    a.off = w->ctx->d_off;
+    a.vex = w->ctx->d_vex;
+    a.compact = w->d_walkers + (size_t)g.compact_bytes * k;
+    a.mt = w->d_mt + (size_t)624 * k;
+    a.mt_index = w->d_mti + k;
+    a.nbr_swap = w->p.nbr_swap ? 1 : 0;
+    a.mc_sweeps = w->p.mc_sweeps;
+    a.bins = w->p.bins;
+    a.start_idx = w->windows[(size_t)2 * win];
+    a.end_idx = w->windows[(size_t)2 * win + 1];
+    a.wl_f = wl_f;
+    a.bin_edges = w->d_edges;
+    a.lng = w->d_lng + (size_t)w->p.bins * k;
+    a.hist = w->d_hist + (size_t)w->hstride * k;
+    a.accepted = w->d_acc + k;
+  }
 }
 
 int wl_check(bwl_wl *w, bool need_config) {
@@ -344,7 +375,9 @@ int bwl_wl_create(const bwl_lattice_desc *desc, const double *v_ex, const bwl_wl
                o_lng = take(sizeof(double) * W * p->bins),
                o_hist = take(sizeof(double) * W * w->hstride), o_e = take(sizeof(double) * W),
                o_is = take(8 * W), o_tr = take(8 * W), o_acc = take(8 * W), o_res = take(4 * W),
-               o_st = take(4 * W), o_args = take(sizeof(WlArgs) * W),
+               o_st = take(4 * W), o_win = take(8 * (size_t)p->num_windows),
+               o_part = take(4 * W), o_xacc = take(4 * W), o_swaps = take(4 * (W + 3)),
+               o_xe = take(sizeof(double) * W), o_args = take(sizeof(WlArgs) * W),
                o_eargs = take(sizeof(WlEnterArgs) * W);
   if (hipMalloc(&w->d, o) != hipSuccess) {
     bwl_wl_destroy(w);
@@ -363,10 +396,17 @@ int bwl_wl_create(const bwl_lattice_desc *desc, const double *v_ex, const bwl_wl
   w->d_acc = (long long *)(b + o_acc);
   w->d_resume = (int32_t *)(b + o_res);
   w->d_status = (int32_t *)(b + o_st);
+  w->d_windows = (int32_t *)(b + o_win);
+  w->d_partner = (int32_t *)(b + o_part);
+  w->d_xacc = (int32_t *)(b + o_xacc);
+  w->d_swaps = (int32_t *)(b + o_swaps);
+  w->d_xe = (double *)(b + o_xe);
   w->d_args = (WlArgs *)(b + o_args);
   w->d_eargs = (WlEnterArgs *)(b + o_eargs);
   bool ok = hipMemsetAsync(b, 0, o, c->stream) == hipSuccess &&
             hipMemcpyAsync(w->d_edges, w->edges.data(), sizeof(double) * (p->bins + 1),
+                           hipMemcpyHostToDevice, c->stream) == hipSuccess &&
+            hipMemcpyAsync(w->d_windows, w->windows.data(), 8 * (size_t)p->num_windows,
                            hipMemcpyHostToDevice, c->stream) == hipSuccess;
   if (!ok || wl_push_mt(w) != BWL_OK) {
     bwl_wl_destroy(w);
@@ -605,28 +645,8 @@ int bwl_wl_sweeps(bwl_wl *w, double wl_f, int32_t average, int64_t *accepted, do
   const Geo &g = w->ctx->g;
   hipStream_t s = w->ctx->stream;
   const int W = w->W;
-  std::vector<WlArgs> args((size_t)W);
-  for (int k = 0; k < W; ++k) {
-    const int win = k / w->p.walkers_per_window;
-    WlArgs &a = args[(size_t)k];
-    memset(&a, 0, sizeof(a));
-    a.g = g;
-    a.off = w->ctx->d_off;
-    a.vex = w->ctx->d_vex;
-    a.compact = w->d_walkers + (size_t)g.compact_bytes * k;
-    a.mt = w->d_mt + (size_t)624 * k;
-    a.mt_index = w->d_mti + k;
-    a.nbr_swap = w->p.nbr_swap ? 1 : 0;
-    a.mc_sweeps = w->p.mc_sweeps;
-    a.bins = w->p.bins;
-    a.start_idx = w->windows[(size_t)2 * win];
-    a.end_idx = w->windows[(size_t)2 * win + 1];
-    a.wl_f = wl_f;
-    a.bin_edges = w->d_edges;
-    a.lng = w->d_lng + (size_t)w->p.bins * k;
-    a.hist = w->d_hist + (size_t)w->hstride * k;
-    a.accepted = w->d_acc + k;
-  }
+  std::vector<WlArgs> args;
+  wl_fill_args(w, wl_f, args);
   HIP_OK(hipMemcpyAsync(w->d_args, args.data(), sizeof(WlArgs) * W, hipMemcpyHostToDevice, s));
   HIP_OK(hipEventRecord(w->ctx->ev0, s));
   HIP_OK(launch_wl_sweeps(w->d_args, W, wl_lds_bytes(g, w->p.bins, w->hstride), s));
@@ -642,6 +662,53 @@ int bwl_wl_sweeps(bwl_wl *w, double wl_f, int32_t average, int64_t *accepted, do
   if (kernel_ms) *kernel_ms = ms;
   if (accepted)
     for (int k = 0; k < W; ++k) accepted[k] = acc[(size_t)k];
+  return wl_pull_mt(w);
+}
+
+int bwl_wl_replica_exchange(bwl_wl *w, int32_t *partner, int32_t *exchanged, double *kernel_ms) {
+  int rc = wl_check(w, true);
+  if (rc) return rc;
+  const Geo &g = w->ctx->g;
+  hipStream_t s = w->ctx->stream;
+  const int W = w->W, wpw = w->p.walkers_per_window;
+  const size_t xlds = wl_exchange_lds_bytes(W, wpw);
+  if (xlds > (size_t)lds_limit())
+    return fail(BWL_ERR_UNSUPPORTED, "too many walkers for the replica-exchange kernel");
+  std::vector<WlArgs> args;
+  wl_fill_args(w, 0.0, args);
+  WlExchangeArgs x;
+  x.E = w->d_xe;
+  x.bin_edges = w->d_edges;
+  x.windows = w->d_windows;
+  x.lng = w->d_lng;
+  x.mt = w->d_mt;
+  x.mt_index = w->d_mti;
+  x.partner = w->d_partner;
+  x.accepted = w->d_xacc;
+  x.swaps = w->d_swaps;
+  x.n_swaps = w->d_swaps + W + 1;
+  x.W = W;
+  x.wpw = wpw;
+  x.num_windows = w->p.num_windows;
+  x.bins = w->p.bins;
+  HIP_OK(hipMemcpyAsync(w->d_args, args.data(), sizeof(WlArgs) * W, hipMemcpyHostToDevice, s));
+  HIP_OK(hipEventRecord(w->ctx->ev0, s));
+  HIP_OK(launch_wl_energies(w->d_args, W, w->d_xe, ns_energies_lds_bytes(g), s));
+  HIP_OK(launch_wl_exchange(x, xlds, s));
+  HIP_OK(launch_wl_swap_configs(w->d_walkers, (long long)g.compact_bytes, x.swaps, x.n_swaps,
+                                W / 2, s));
+  HIP_OK(hipEventRecord(w->ctx->ev1, s));
+  std::vector<int32_t> part((size_t)W), acc((size_t)W);
+  HIP_OK(hipMemcpyAsync(part.data(), w->d_partner, 4 * (size_t)W, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(acc.data(), w->d_xacc, 4 * (size_t)W, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, w->ctx->ev0, w->ctx->ev1);
+  if (kernel_ms) *kernel_ms = ms;
+  for (int k = 0; k < W; ++k) {
+    if (partner) partner[k] = part[(size_t)k];
+    if (exchanged) exchanged[k] = acc[(size_t)k];
+  }
   return wl_pull_mt(w);
 }
 

@@ -103,6 +103,22 @@ struct WlArgs {
   long long *accepted;      // out
 };
 
+// replica_exchange (wang-landau.F90:1392-1501) over a whole Wang-Landau
+// walker set: walker k stands for MPI rank k.
+struct WlExchangeArgs {
+  const double *E;          // walker energies in the reference's order, W
+  const double *bin_edges;  // bins + 1
+  const int32_t *windows;   // num_windows (start, end) pairs, 1-based bins
+  const double *lng;        // W rows of bins
+  uint32_t *mt;             // W x 624, in/out
+  int32_t *mt_index;        // W, in/out
+  int32_t *partner;         // W out: walker matched with, -1 if none
+  int32_t *accepted;        // W out: 1 if the matched pair exchanged
+  int32_t *swaps;           // out: exchanged pairs (a, b), at most W/2
+  int32_t *n_swaps;         // out
+  int32_t W, wpw, num_windows, bins;
+};
+
 // enter_energy_window (wang-landau.F90:643-741) of one walker, resumable:
 // the kernel stops when the walker is inside its window (status 0), when
 // the reference would re-run initial_setup (status 1; the host does that
@@ -178,6 +194,13 @@ size_t wl_lds_bytes(const Geo &g, int bins, int window_bins);
 hipError_t launch_wl_sweeps(const WlArgs *args_dev, int n, size_t lds, hipStream_t s);
 hipError_t launch_wl_average(double *lng, double *hist, int n_windows, int walkers_per_window,
                              int bins, int hist_stride, hipStream_t s);
+// Walker energies (ns_energies_lds_bytes of LDS), the exchange decisions
+// (one wavefront) and the exchange of configurations.
+hipError_t launch_wl_energies(const WlArgs *args_dev, int n, double *E, size_t lds, hipStream_t s);
+size_t wl_exchange_lds_bytes(int W, int wpw);
+hipError_t launch_wl_exchange(const WlExchangeArgs &a, size_t lds, hipStream_t s);
+hipError_t launch_wl_swap_configs(int8_t *walkers, long long bytes, const int32_t *swaps,
+                                  const int32_t *n_swaps, int max_pairs, hipStream_t s);
 size_t wl_enter_lds_bytes(const Geo &g);
 hipError_t launch_wl_enter(const WlEnterArgs *args_dev, int n, size_t lds, hipStream_t s);
 hipError_t launch_asro(const AsroArgs &a, hipStream_t s);

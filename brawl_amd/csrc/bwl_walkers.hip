@@ -298,6 +298,159 @@ __global__ void k_wl_average(double *lng, double *hist, int n_windows, int wpw, 
 }
 
 // ------------------------------------------------------------------
+// replica_exchange (wang-landau.F90:1392-1501).
+// k_wl_energies: e_unswapped = full_energy(config) of every walker
+// (bw_hamiltonian.f90:58-81 order), one wavefront per walker.
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_wl_energies(const WlArgs *__restrict__ args, double *E) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const WlArgs &A = args[blockIdx.x];
+  const Geo &g = A.g;
+  int8_t *lat = (int8_t *)smem;
+  size_t o = (g.lds_bytes + 15) & ~15;
+  double *vex = (double *)(smem + o);
+  o += sizeof(double) * g.S * g.S * g.R;
+  int32_t *offl = (int32_t *)(smem + o);
+  for (int i = threadIdx.x; i < g.S * g.S * g.R; i += 64) vex[i] = A.vex[i];
+  stage_offsets(offl, A.off, g);
+  stage_lattice(lat, A.compact, g, 0);
+  __syncthreads();
+  const double e = ref_total_energy(lat, g, offl, vex);
+  if (threadIdx.x == 0) E[blockIdx.x] = e;
+}
+
+// The exchange decisions, one wavefront for the whole walker set (a few
+// hundred random numbers at most).  Each walker's overlap location comes
+// from the bin of its energy (:1406-1431).  For each pair of neighbouring
+// windows (i, i+1), rank 0's stream (walker 0) shuffles the candidate rows
+// of both windows (shuffle_rows, :1513-1524), the rows are matched greedily
+// in shuffled order (:1447-1468), and each matched pair (a in window i, b in
+// window i+1), in order, draws genrand() from a's stream and exchanges iff
+// it is below exp(lng_a(bin_a) - lng_a(bin_b)) (:1476-1497).  All lanes
+// follow the same control flow (the MT19937 helpers are whole-wavefront);
+// lane 0 writes the shared rows and the outputs.
+__global__ __launch_bounds__(64) void k_wl_exchange(WlExchangeArgs A) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const int lane = threadIdx.x;
+  const int W = A.W, wpw = A.wpw, nwin = A.num_windows;
+  uint32_t *st0 = (uint32_t *)smem;       // walker 0's stream
+  uint32_t *st1 = st0 + 624;              // the deciding walker's stream
+  int32_t *ibin = (int32_t *)(st1 + 624); // W
+  int32_t *loc = ibin + W;                // W
+  int32_t *lo = loc + W;                  // rows (walker, location) of window i
+  int32_t *up = lo + 2 * wpw;             // rows of window i+1
+  int32_t *pr = up + 2 * wpw;             // matched pairs, then their count
+  for (int k = lane; k < W; k += 64) {
+    const int b = wl_bin_index(A.E[k], A.bin_edges, A.bins);
+    const int m = k / wpw + 1;  // the rank's window, 1-based
+    bool lower = false, upper = false;
+    if (m > 1) lower = b < A.windows[2 * (m - 2) + 1] + 1 && b > A.windows[2 * (m - 1)] - 1;
+    if (m < nwin) upper = b > A.windows[2 * m] - 1 && b < A.windows[2 * (m - 1) + 1] + 1;
+    ibin[k] = b;
+    loc[k] = upper ? m : (lower ? m - 1 : 0);
+    A.partner[k] = -1;
+    A.accepted[k] = 0;
+  }
+  for (int i = lane; i < 624; i += 64) st0[i] = A.mt[i];
+  int mti0 = A.mt_index[0];
+  int nsw = 0;
+  __syncthreads();
+  for (int i = 1; i < nwin; ++i) {
+    for (int j = lane; j < wpw; j += 64) {
+      lo[2 * j] = (i - 1) * wpw + j;
+      lo[2 * j + 1] = loc[(i - 1) * wpw + j];
+      up[2 * j] = i * wpw + j;
+      up[2 * j + 1] = loc[i * wpw + j];
+    }
+    __syncthreads();
+    for (int side = 0; side < 2; ++side) {
+      int32_t *rows = side ? up : lo;
+      for (int t = wpw; t >= 2; --t) {
+        const double u = genrand_dev(st0, mti0);
+        int r = 1 + (int)(u * (double)t);
+        if (r > wpw) r = wpw;
+        if (lane == 0) {
+          const int32_t a0 = rows[2 * (t - 1)], a1 = rows[2 * (t - 1) + 1];
+          rows[2 * (t - 1)] = rows[2 * (r - 1)];
+          rows[2 * (t - 1) + 1] = rows[2 * (r - 1) + 1];
+          rows[2 * (r - 1)] = a0;
+          rows[2 * (r - 1) + 1] = a1;
+        }
+        __syncthreads();
+      }
+    }
+    if (lane == 0) {
+      int np = 0;
+      for (int j = 0; j < wpw; ++j) {
+        if (lo[2 * j + 1] == 0) continue;
+        for (int k = 0; k < wpw; ++k) {
+          if (up[2 * k + 1] == 0) continue;
+          if (lo[2 * j + 1] == up[2 * k + 1]) {
+            pr[2 * np] = lo[2 * j];
+            pr[2 * np + 1] = up[2 * k];
+            ++np;
+            lo[2 * j] = lo[2 * j + 1] = 0;
+            up[2 * k] = up[2 * k + 1] = 0;
+          }
+        }
+      }
+      pr[2 * wpw] = np;
+    }
+    __syncthreads();
+    const int np = pr[2 * wpw];
+    for (int j = 0; j < np; ++j) {
+      const int a = pr[2 * j], b = pr[2 * j + 1];
+      double u;
+      if (a == 0) {
+        u = genrand_dev(st0, mti0);
+      } else {
+        for (int q = lane; q < 624; q += 64) st1[q] = A.mt[(size_t)624 * a + q];
+        int m1 = A.mt_index[a];
+        __syncthreads();
+        u = genrand_dev(st1, m1);
+        __syncthreads();
+        for (int q = lane; q < 624; q += 64) A.mt[(size_t)624 * a + q] = st1[q];
+        if (lane == 0) A.mt_index[a] = m1;
+        __syncthreads();
+      }
+      // ibin of a matched walker lies inside both windows' bins
+      const double x = A.lng[(size_t)A.bins * a + ibin[a] - 1] - A.lng[(size_t)A.bins * a + ibin[b] - 1];
+      const bool acc = x >= 0.0 || lt_exp_dev(u, x);  // u < 1 <= exp(x) when x >= 0
+      if (lane == 0) {
+        A.partner[a] = b;
+        A.partner[b] = a;
+        A.accepted[a] = A.accepted[b] = acc ? 1 : 0;
+        if (acc) {
+          A.swaps[2 * nsw] = a;
+          A.swaps[2 * nsw + 1] = b;
+        }
+      }
+      nsw += acc ? 1 : 0;
+    }
+    __syncthreads();
+  }
+  for (int i = lane; i < 624; i += 64) A.mt[i] = st0[i];
+  if (lane == 0) {
+    A.mt_index[0] = mti0;
+    A.n_swaps[0] = nsw;
+  }
+}
+
+// Exchange the configurations of the accepted pairs: block j swaps pair j.
+__global__ void k_wl_swap_configs(int8_t *walkers, long long bytes, const int32_t *swaps,
+                                  const int32_t *n_swaps) {
+  const int j = blockIdx.x;
+  if (j >= n_swaps[0]) return;
+  int8_t *a = walkers + bytes * swaps[2 * j];
+  int8_t *b = walkers + bytes * swaps[2 * j + 1];
+  for (long long i = threadIdx.x; i < bytes; i += blockDim.x) {
+    const int8_t t = a[i];
+    a[i] = b[i];
+    b[i] = t;
+  }
+}
+
+// ------------------------------------------------------------------
 // enter_energy_window (wang-landau.F90:643-741) for one walker.
 // ------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__ args) {
@@ -491,6 +644,33 @@ hipError_t launch_wl_average(double *lng, double *hist, int n_windows, int wpw, 
   const int total = n_windows * (bins + hist_stride);
   hipLaunchKernelGGL(k_wl_average, dim3((total + 255) / 256), dim3(256), 0, s, lng, hist,
                      n_windows, wpw, bins, hist_stride);
+  return hipGetLastError();
+}
+
+hipError_t launch_wl_energies(const WlArgs *args_dev, int n, double *E, size_t lds, hipStream_t s) {
+  hipError_t e = hipFuncSetAttribute((const void *)k_wl_energies,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_wl_energies, dim3(n), dim3(64), lds, s, args_dev, E);
+  return hipGetLastError();
+}
+
+size_t wl_exchange_lds_bytes(int W, int wpw) {
+  return 2 * 624 * 4 + sizeof(int32_t) * (size_t)(2 * W + 6 * wpw + 1);
+}
+
+hipError_t launch_wl_exchange(const WlExchangeArgs &a, size_t lds, hipStream_t s) {
+  hipError_t e = hipFuncSetAttribute((const void *)k_wl_exchange,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_wl_exchange, dim3(1), dim3(64), lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_wl_swap_configs(int8_t *walkers, long long bytes, const int32_t *swaps,
+                                  const int32_t *n_swaps, int max_pairs, hipStream_t s) {
+  hipLaunchKernelGGL(k_wl_swap_configs, dim3(max_pairs > 0 ? max_pairs : 1), dim3(256), 0, s,
+                     walkers, bytes, swaps, n_swaps);
   return hipGetLastError();
 }
 

@@ -5,18 +5,21 @@
 drawing from MT19937 seed 110179 + 11*k).  Its methods are the reference's
 per-rank routines run for every walker at once:
 
-    enter_windows   enter_energy_window      (wang-landau.F90:643-741)
-    sweeps          sweeps + window average  (wang-landau.F90:539-632)
-    dos_average     dos_average              (wang-landau.F90:1127-1133)
-    dos_combine     dos_combine              (wang-landau.F90:1147-1194)
+    enter_windows     enter_energy_window      (wang-landau.F90:643-741)
+    sweeps            sweeps + window average  (wang-landau.F90:539-632)
+    replica_exchange  replica_exchange         (wang-landau.F90:1392-1501)
+    pre_sampling      pre_sampling's loop      (wang-landau.F90:757-838)
+    dos_average       dos_average              (wang-landau.F90:1127-1133)
+    dos_combine       dos_combine              (wang-landau.F90:1147-1194)
 
-`run` is the main loop of wl_main (wang-landau.F90:198-306): sweeps until
-every walker's window histogram is flat, then halve ln f, average and
-stitch ln g, until ln f falls below the tolerance.  Not carried over:
-pre_sampling (:757-838), replica_exchange (:1392-1501), window load
-balancing (mpi_window_optimise, :1211-1328), rho_of_E sampling and per-bin
-configuration stores -- the parts that exist to balance MPI ranks or write
-auxiliary output.
+`run` is wl_main (wang-landau.F90:160-306): window entry, pre-sampling,
+then the main loop -- sweeps and a replica exchange until every walker's
+window histogram is flat, then halve ln f, average and stitch ln g, until
+ln f falls below the tolerance.  Not carried over: window load balancing
+(mpi_window_optimise, :1211-1328, driven by MPI timings), rho_of_E
+sampling, and the per-bin configuration stores with the reloads they feed
+(merge_configs, load_window_config: Fortran's random_number picks them) --
+the parts that exist to balance MPI ranks or write auxiliary output.
 """
 from __future__ import annotations
 
@@ -39,9 +42,12 @@ class WlRunResult:
     lng: np.ndarray          # combined ln g over all bins (min subtracted)
     bin_edges: np.ndarray
     iterations: int          # ln f halvings done
-    sweeps_calls: int
+    sweeps_calls: int        # main loop and pre-sampling
     trials: int
     kernel_ms: float
+    pre_sampling_calls: int = 0
+    exchanges: int = 0       # configurations exchanged (pairs)
+    exchange_pairs: int = 0  # pairs that drew for an exchange
 
 
 class WangLandau:
@@ -157,6 +163,56 @@ class WangLandau:
                                     C.byref(ms)))
         return acc, float(ms.value)
 
+    def replica_exchange(self):
+        """One replica_exchange over all walkers (wang-landau.F90:1392-1501);
+        returns (partner, exchanged, kernel_ms): partner[k] is the walker
+        matched with k (-1 if none), exchanged[k] whether that pair swapped
+        configurations."""
+        partner = np.zeros(self.n_walkers, dtype=np.int32)
+        exch = np.zeros(self.n_walkers, dtype=np.int32)
+        ms = C.c_double()
+        check(self._L.bwl_wl_replica_exchange(self._h, _p(partner, C.c_int32), _p(exch, C.c_int32),
+                                              C.byref(ms)))
+        return partner, exch.astype(bool), float(ms.value)
+
+    def pre_sampling(self, wl_f: float, flatness: float, *, exchange: bool = True,
+                     max_sweeps_calls=None):
+        """pre_sampling's loop (wang-landau.F90:757-785): sweeps with window
+        averages, a replica exchange after every 10th call (the counter then
+        restarts at 1, so every 9th after the first), until every window has
+        been flat once -- histogram minimum above 1000/walkers_per_window and
+        flatness above the threshold, judged per window (the MPI_MAX of its
+        walkers).  Returns (sweeps calls, device ms, accepted trials,
+        exchanged pairs, pairs that drew, whether every window got flat)."""
+        done = np.zeros(self.num_windows, dtype=bool)
+        wpw = self.walkers_per_window
+        i_sweeps = calls = acc_total = exch = pairs = 0
+        ms = 0.0
+        while not done.all():
+            if max_sweeps_calls is not None and calls >= max_sweeps_calls:
+                break
+            acc, k_ms = self.sweeps(wl_f, average=True)
+            calls += 1
+            ms += k_ms
+            acc_total += int(acc.sum())
+            i_sweeps += 1
+            if exchange and i_sweeps % 10 == 0:
+                i_sweeps = 1
+                partner, ex, x_ms = self.replica_exchange()
+                ms += x_ms
+                exch += int(ex.sum()) // 2
+                pairs += int((partner >= 0).sum()) // 2
+            _, hist = self.lng_hist()
+            flat = self.flatness(hist)
+            for w in range(self.num_windows):
+                if done[w]:
+                    continue
+                nb = int(self.windows[w, 1] - self.windows[w, 0] + 1)
+                for k in range(w * wpw, (w + 1) * wpw):
+                    if hist[k, :nb].min() > 1000.0 / wpw and flat[k] > flatness:
+                        done[w] = True
+        return calls, ms, acc_total, exch, pairs, bool(done.all())
+
     def dos_average(self, lng: np.ndarray) -> np.ndarray:
         """Mean of each window's ln g over its walkers, summed in walker order."""
         out = lng.copy()
@@ -200,19 +256,39 @@ class WangLandau:
             out[k] = h.min() / (s / nb) if s > 0 else 0.0
         return out
 
+    def _halve_setup(self, lng, hist):
+        """dos_average then dos_combine, the stitched ln g written to every
+        walker and the histograms cleared."""
+        lng = self.dos_average(lng)
+        comb = self.dos_combine(lng)
+        self.set_lng_hist(np.tile(comb, (self.n_walkers, 1)), np.zeros_like(hist))
+
     def run(self, *, wl_f=1.0, tolerance=1e-6, flatness=0.8, max_sweeps_calls=None,
-            enter_max_trials=10**9):
-        """wl_main's loop (see module docstring for what is left out)."""
+            enter_max_trials=10**9, pre_sampling=True, replica_exchange=True,
+            max_pre_sampling_calls=None):
+        """wl_main (see module docstring for what is left out).  With
+        pre_sampling, ln f stays at wl_f through the pre-sampling loop, whose
+        ln g (averaged and stitched) starts the main loop; with
+        replica_exchange, the main loop exchanges after every sweeps call
+        (both as the reference runs by default, performance = 0)."""
         _, entered = self.enter_windows(enter_max_trials)
         if not entered.all():
             raise RuntimeError("some walkers did not reach their energy window")
         lng, hist = self.lng_hist()
         self.set_lng_hist(np.zeros_like(lng), np.zeros_like(hist))
-        converged = np.zeros(self.n_walkers, dtype=bool)
-        calls = iters = 0
-        ms = 0.0
-        acc_total = 0
         trials_per_call = self.params.mc_sweeps * self.n_atoms * self.n_walkers
+        calls = iters = pre_calls = exch = pairs = 0
+        ms = 0.0
+        if pre_sampling:
+            pre_calls, p_ms, _, exch, pairs, flat = self.pre_sampling(
+                wl_f, flatness, exchange=replica_exchange, max_sweeps_calls=max_pre_sampling_calls)
+            if not flat:
+                raise RuntimeError("pre-sampling: not every window became flat")
+            ms += p_ms
+            lng, hist = self.lng_hist()
+            self._halve_setup(lng, hist)
+        converged = np.zeros(self.n_walkers, dtype=bool)
+        acc_total = 0
         while wl_f > tolerance:
             if max_sweeps_calls is not None and calls >= max_sweeps_calls:
                 break
@@ -220,18 +296,21 @@ class WangLandau:
             calls += 1
             ms += k_ms
             acc_total += int(acc.sum())
+            if replica_exchange:
+                partner, ex, x_ms = self.replica_exchange()
+                ms += x_ms
+                exch += int(ex.sum()) // 2
+                pairs += int((partner >= 0).sum()) // 2
             lng, hist = self.lng_hist()
             converged |= self.flatness(hist) > flatness
             if converged.all():
                 converged[:] = False
                 wl_f = wl_f * 0.5
-                lng = self.dos_average(lng)
-                comb = self.dos_combine(lng)
-                self.set_lng_hist(np.tile(comb, (self.n_walkers, 1)), np.zeros_like(hist))
+                self._halve_setup(lng, hist)
                 iters += 1
         lng, _ = self.lng_hist()
-        return WlRunResult(lng[0].copy(), self.bin_edges.copy(), iters, calls,
-                           calls * trials_per_call, ms)
+        return WlRunResult(lng[0].copy(), self.bin_edges.copy(), iters, calls + pre_calls,
+                           (calls + pre_calls) * trials_per_call, ms, pre_calls, exch, pairs)
 
 
 __all__ = ["WangLandau", "WlRunResult"]

@@ -280,6 +280,17 @@ int bwl_wl_enter_windows(bwl_wl *wl, int64_t max_trials, int64_t *trials, int32_
  * in walker order (the reference's MPI_Allreduce, :628-631).  accepted
  * (optional) per walker; kernel_ms the device time. */
 int bwl_wl_sweeps(bwl_wl *wl, double wl_f, int32_t average, int64_t *accepted, double *kernel_ms);
+/* replica_exchange (wang-landau.F90:1392-1501) over the whole walker set,
+ * walker k standing for MPI rank k (walker 0's stream, as rank 0's, shuffles
+ * the candidates).  Every walker's energy is recounted (full_energy); a
+ * walker whose bin lies in the overlap with the window above (below) is a
+ * candidate for that pair of windows; candidates of windows i and i+1 are
+ * shuffled, matched in order, and each matched pair exchanges
+ * configurations iff genrand() of the lower walker's stream is below
+ * exp(lng(bin_lower) - lng(bin_upper)) with the lower walker's ln g.
+ * partner[k] (optional) = walker matched with k or -1; exchanged[k]
+ * (optional) = 1 if that pair exchanged; kernel_ms the device time. */
+int bwl_wl_replica_exchange(bwl_wl *wl, int32_t *partner, int32_t *exchanged, double *kernel_ms);
 
 /* ---------------------------------------------------------------- */
 /* Request server (continuous batching on one device)                */

@@ -811,3 +811,97 @@ void bwo_wl_setup(const bwo_lattice *L, int32_t bins, int32_t num_windows, float
   for (int i = 1; i <= bins + 1; ++i)
     bin_edges[i - 1] = (double)energy_min * to_ry + (double)(i - 1) * width;
 }
+
+/* replica_exchange (:1392-1501) over num_windows*wpw walkers, walker k in
+ * window k/wpw + 1 standing for MPI rank k (walker 0 is rank 0, whose stream
+ * shuffles the candidate rows).  lng is W rows of `bins` values; windows is
+ * num_windows (start, end) pairs.  The configurations of accepted pairs are
+ * exchanged in cfgs; partner/accepted (W each) record every matched pair.
+ * Returns the number of exchanged pairs. */
+static void shuffle_rows(int32_t *rows, int n, bwo_mt *g) {
+  /* shuffle_rows (:1513-1524): Fisher-Yates from the last row down */
+  for (int i = n; i >= 2; --i) {
+    int r = 1 + (int)(bwo_genrand(g) * (double)i);
+    if (r > n) r = n;
+    int32_t t0 = rows[2 * (i - 1)], t1 = rows[2 * (i - 1) + 1];
+    rows[2 * (i - 1)] = rows[2 * (r - 1)];
+    rows[2 * (i - 1) + 1] = rows[2 * (r - 1) + 1];
+    rows[2 * (r - 1)] = t0;
+    rows[2 * (r - 1) + 1] = t1;
+  }
+}
+
+int32_t bwo_wl_replica_exchange(const bwo_lattice *L, int8_t *const *cfgs, bwo_mt *const *gens,
+                                const double *lng, int32_t bins, const double *bin_edges,
+                                const int32_t *windows, int32_t num_windows, int32_t wpw,
+                                int32_t *partner, int32_t *accepted) {
+  const int W = num_windows * wpw;
+  const size_t bytes = (size_t)8 * L->n1 * L->n2 * L->n3;
+  int32_t *ibin = malloc(sizeof(int32_t) * (size_t)W);
+  int32_t *loc = malloc(sizeof(int32_t) * (size_t)W);
+  int32_t *lo = malloc(sizeof(int32_t) * 2 * (size_t)wpw);
+  int32_t *up = malloc(sizeof(int32_t) * 2 * (size_t)wpw);
+  int32_t *pr = malloc(sizeof(int32_t) * 2 * (size_t)wpw);
+  int8_t *tmp = malloc(bytes);
+  for (int k = 0; k < W; ++k) {
+    /* e_unswapped and the overlap location of rank k (:1406-1431) */
+    const int b = bwo_wl_bin_index(bwo_total_energy(L, cfgs[k]), bin_edges, bins);
+    const int m = k / wpw + 1;
+    int lower = 0, upper = 0;
+    if (m > 1) lower = b < windows[2 * (m - 2) + 1] + 1 && b > windows[2 * (m - 1)] - 1;
+    if (m < num_windows) upper = b > windows[2 * m] - 1 && b < windows[2 * (m - 1) + 1] + 1;
+    ibin[k] = b;
+    loc[k] = upper ? m : (lower ? m - 1 : 0);
+    partner[k] = -1;
+    accepted[k] = 0;
+  }
+  int32_t exchanged = 0;
+  for (int i = 1; i < num_windows; ++i) {
+    /* rank 0 matches the rows of windows i and i+1 (:1440-1470) */
+    for (int j = 0; j < wpw; ++j) {
+      lo[2 * j] = (i - 1) * wpw + j;
+      lo[2 * j + 1] = loc[(i - 1) * wpw + j];
+      up[2 * j] = i * wpw + j;
+      up[2 * j + 1] = loc[i * wpw + j];
+    }
+    shuffle_rows(lo, wpw, gens[0]);
+    shuffle_rows(up, wpw, gens[0]);
+    int np = 0;
+    for (int j = 0; j < wpw; ++j) {
+      if (lo[2 * j + 1] == 0) continue;
+      for (int k = 0; k < wpw; ++k) {
+        if (up[2 * k + 1] == 0) continue;
+        if (lo[2 * j + 1] == up[2 * k + 1]) {
+          pr[2 * np] = lo[2 * j];
+          pr[2 * np + 1] = up[2 * k];
+          ++np;
+          lo[2 * j] = lo[2 * j + 1] = 0;
+          up[2 * k] = up[2 * k + 1] = 0;
+        }
+      }
+    }
+    /* each pair in order (:1476-1497): the lower walker decides */
+    for (int j = 0; j < np; ++j) {
+      const int a = pr[2 * j], b = pr[2 * j + 1];
+      const double u = bwo_genrand(gens[a]);
+      const double x = lng[(size_t)bins * a + ibin[a] - 1] - lng[(size_t)bins * a + ibin[b] - 1];
+      const int acc = u < exp(x);
+      partner[a] = b;
+      partner[b] = a;
+      accepted[a] = accepted[b] = acc;
+      if (acc) {
+        memcpy(tmp, cfgs[a], bytes);
+        memcpy(cfgs[a], cfgs[b], bytes);
+        memcpy(cfgs[b], tmp, bytes);
+        ++exchanged;
+      }
+    }
+  }
+  free(ibin);
+  free(loc);
+  free(lo);
+  free(up);
+  free(pr);
+  free(tmp);
+  return exchanged;
+}

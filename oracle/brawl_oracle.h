@@ -161,6 +161,16 @@ int64_t bwo_wl_enter_window(const bwo_lattice *L, int8_t *cfg, bwo_mt *g,
 void bwo_wl_setup(const bwo_lattice *L, int32_t bins, int32_t num_windows,
                   float bin_overlap, float energy_min, float energy_max,
                   int32_t *windows, double *bin_edges);
+/* replica_exchange (wang-landau.F90:1392-1501) over W = num_windows*wpw
+ * walkers (walker k = rank k; walker 0's stream shuffles the candidates).
+ * lng: W rows of `bins` values.  Exchanges the configurations of accepted
+ * pairs; partner[k] = walker matched with k or -1, accepted[k] = 1 if that
+ * pair exchanged.  Returns the number of exchanged pairs. */
+int32_t bwo_wl_replica_exchange(const bwo_lattice *L, int8_t *const *cfgs,
+                                bwo_mt *const *gens, const double *lng,
+                                int32_t bins, const double *bin_edges,
+                                const int32_t *windows, int32_t num_windows,
+                                int32_t wpw, int32_t *partner, int32_t *accepted);
 
 #ifdef __cplusplus
 }

@@ -88,6 +88,8 @@ def lib():
                                                 C.c_float, C.c_float, i64p, dp, C.c_int64]),
             "bwo_wl_setup": (None, [P(Lattice), C.c_int32, C.c_int32, C.c_float, C.c_float,
                                     C.c_float, i32p, dp]),
+            "bwo_wl_replica_exchange": (C.c_int32, [P(Lattice), P(i8p), P(P(MT)), dp, C.c_int32,
+                                                    dp, i32p, C.c_int32, C.c_int32, i32p, i32p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -263,6 +265,29 @@ class OracleLattice:
                                          energy_min, energy_max,
                                          None if sn is None else _ptr(sn, C.c_int64),
                                          None if cc is None else _ptr(cc, C.c_double), max_trials)
+
+
+def wl_replica_exchange(lat: "OracleLattice", cfgs, gens, lng, bin_edges, windows, wpw):
+    """replica_exchange (wang-landau.F90:1392-1501) over walkers whose
+    configurations (order='F' int8 grids) and Generators are listed in walker
+    order; swaps configurations in place.  Returns (exchanged, partner,
+    accepted)."""
+    W = len(cfgs)
+    windows = np.ascontiguousarray(np.asarray(windows, dtype=np.int32).reshape(-1))
+    lng = np.ascontiguousarray(lng, dtype=np.float64)
+    edges = np.ascontiguousarray(bin_edges, dtype=np.float64)
+    assert lng.shape == (W, len(edges) - 1) and W == (len(windows) // 2) * wpw
+    for c in cfgs:
+        assert c.dtype == np.int8 and c.flags.f_contiguous
+    cp = (C.POINTER(C.c_int8) * W)(*[_ptr(c, C.c_int8) for c in cfgs])
+    gp = (C.POINTER(MT) * W)(*[C.pointer(g.state) for g in gens])
+    partner = np.zeros(W, dtype=np.int32)
+    acc = np.zeros(W, dtype=np.int32)
+    n = lib().bwo_wl_replica_exchange(lat.ref, cp, gp, _ptr(lng, C.c_double), len(edges) - 1,
+                                      _ptr(edges, C.c_double), _ptr(windows, C.c_int32),
+                                      len(windows) // 2, int(wpw), _ptr(partner, C.c_int32),
+                                      _ptr(acc, C.c_int32))
+    return int(n), partner, acc
 
 
 def read_vij(path: str, n_species: int, n_shells: int) -> np.ndarray:

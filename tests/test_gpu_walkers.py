@@ -207,3 +207,103 @@ def test_wang_landau_enter_window_bit_exact(hip, oracle):
             assert mti == mti_ref and np.array_equal(mt, mt_ref)
             assert entered[k] == (t < max_trials)
         wl.close()
+
+
+def _wl_rx_case(hip, oracle, bins, wpw, emin, emax, mc_sweeps, overlap=0.5):
+    """bcc 4^3, 3 species, 3 windows: device walkers and oracle walkers from
+    the same streams, both walked into their windows."""
+    from brawl_amd import wang_landau, workloads
+    lat, n, S = 2, (4, 4, 4), 3
+    v = workloads.random_v(S, [30.0, 15.0], 7)
+    ref = oracle.OracleLattice(lat, n, v)
+    wl = wang_landau.WangLandau(lat, n, v, bins=bins, num_windows=3, walkers_per_window=wpw,
+                                bin_overlap=overlap, energy_min=emin, energy_max=emax,
+                                mc_sweeps=mc_sweeps)
+    windows, edges = ref.wl_setup(bins, 3, overlap, emin, emax)
+    assert np.array_equal(wl.windows, windows) and np.array_equal(wl.bin_edges, edges)
+    conc = np.full(S, 1.0 / S)
+    wl.initial_setup(concentrations=conc)
+    _, entered = wl.enter_windows(10**7)
+    assert entered.all()
+    gens, cfgs = [], []
+    for k in range(wl.n_walkers):
+        g = oracle.Generator(rank=k)
+        c = ref.initial_setup(g, concentrations=conc)
+        s, e = windows[k // wpw]
+        ref.wl_enter_window(c, g, edges[s - 1], edges[e], emin, emax, concentrations=conc)
+        gens.append(g)
+        cfgs.append(c)
+    return wl, ref, windows, edges, gens, cfgs
+
+
+def _wl_same_state(wl, gens, cfgs):
+    for k in range(wl.n_walkers):
+        assert np.array_equal(wl.get_walker(k), cfgs[k]), f"walker {k} configuration"
+        mt, mti = wl.genrand_state(k)
+        mt_ref, mti_ref = gens[k].state_arrays()
+        assert mti == mti_ref and np.array_equal(mt, mt_ref), f"walker {k} stream"
+
+
+def test_wang_landau_replica_exchange_bit_exact(hip, oracle):
+    """Sweeps then replica_exchange, 12 times: matches, decisions, exchanged
+    configurations and every walker's stream equal the oracle's."""
+    wpw, bins = 4, 40
+    wl, ref, windows, edges, gens, cfgs = _wl_rx_case(hip, oracle, bins, wpw, 8.0, 18.0, 1)
+    _say("wl replica exchange: walkers", wl.n_walkers)
+    _wl_same_state(wl, gens, cfgs)
+    lng0 = np.random.default_rng(11).normal(0.0, 1.0, (wl.n_walkers, bins))
+    wl.set_lng_hist(lng0, np.zeros((wl.n_walkers, wl.hist_stride)))
+    pairs = exchanged = 0
+    for it in range(12):
+        wl.sweeps(0.0, average=False)          # ln f = 0: ln g stays lng0
+        for k in range(wl.n_walkers):
+            s, e = windows[k // wpw]
+            ref.wl_sweeps(cfgs[k], gens[k], mc_sweeps=1, bin_edges=edges, start_idx=int(s),
+                          end_idx=int(e), wl_f=0.0, lng=lng0[k].copy(), hist=np.zeros(e - s + 1))
+        partner, ex, _ = wl.replica_exchange()
+        n, p_ref, a_ref = oracle.wl_replica_exchange(ref, cfgs, gens, lng0, edges, windows, wpw)
+        assert np.array_equal(partner, p_ref), f"round {it}"
+        assert np.array_equal(ex, a_ref.astype(bool)), f"round {it}"
+        assert int(ex.sum()) // 2 == n
+        _wl_same_state(wl, gens, cfgs)
+        pairs += int((partner >= 0).sum()) // 2
+        exchanged += n
+    _say("pairs", pairs, "exchanged", exchanged)
+    assert pairs > exchanged > 0          # both outcomes occurred
+    wl.close()
+
+
+def test_wang_landau_pre_sampling_bit_exact(hip, oracle):
+    """The pre-sampling loop (sweeps with window averages at ln f = 1, an
+    exchange after every 10th call and then every 9th, until every window
+    has been flat once) equals the same loop over the oracle's walkers:
+    number of calls, ln g, histograms, configurations and streams."""
+    wpw, bins = 2, 30
+    wl, ref, windows, edges, gens, cfgs = _wl_rx_case(hip, oracle, bins, wpw, 9.0, 16.0, 5)
+    calls, _, _, exch, pairs, flat = wl.pre_sampling(1.0, 0.8, max_sweeps_calls=2000)
+    _say("wl pre-sampling: calls", calls, "pairs", pairs, "exchanged", exch)
+    assert flat and pairs > 0
+    W = wl.n_walkers
+    lng = np.zeros((W, bins))
+    hist = np.zeros((W, wl.hist_stride))
+    i_sweeps = 0
+    for call in range(calls):
+        for k in range(W):
+            s, e = windows[k // wpw]
+            h = hist[k, :e - s + 1].copy()
+            lk = lng[k].copy()
+            ref.wl_sweeps(cfgs[k], gens[k], mc_sweeps=5, bin_edges=edges, start_idx=int(s),
+                          end_idx=int(e), wl_f=1.0, lng=lk, hist=h)
+            lng[k] = lk
+            hist[k, :e - s + 1] = h
+        lng = wl.dos_average(lng)
+        hist = wl.dos_average(hist)
+        i_sweeps += 1
+        if i_sweeps % 10 == 0:
+            i_sweeps = 1
+            oracle.wl_replica_exchange(ref, cfgs, gens, lng, edges, windows, wpw)
+    lng_d, hist_d = wl.lng_hist()
+    assert np.array_equal(lng_d, lng)
+    assert np.array_equal(hist_d, hist)
+    _wl_same_state(wl, gens, cfgs)
+    wl.close()

@@ -209,3 +209,120 @@ def test_wl_window_setup_matches_reference_formulae(oracle):
     to_ry = 128 / (13.605693122 * 1000)
     assert abs(edges[0] - (-96.0 * to_ry)) < 1e-15
     assert abs(edges[-1]) < 1e-12
+
+
+def _rx_pair_setup(oracle, seed=5):
+    """Two walkers (one per window) whose energies fall in bins 5 and 6 of
+    10, both inside the windows' overlap [5, 6]."""
+    v = workloads.random_v(2, [20.0], seed)
+    lat = oracle.OracleLattice(1, (2, 2, 2), v)
+    conc = np.full(2, 0.5)
+    cfgs, es = [], []
+    k = 0
+    while len(cfgs) < 2:
+        c = lat.initial_setup(oracle.Generator(rank=100 + k), concentrations=conc)
+        e = lat.total_energy(c)
+        k += 1
+        if all(abs(e - x) > 1e-9 for x in es):
+            cfgs.append(c)
+            es.append(e)
+    order = np.argsort(es)
+    cfgs = [cfgs[i] for i in order]
+    ea, eb = sorted(es)
+    d = eb - ea
+    edges = ea - 4.5 * d + np.arange(11) * d
+    windows = np.array([[1, 6], [5, 10]], dtype=np.int32)
+    return lat, cfgs, edges, windows
+
+
+def test_wl_replica_exchange_acceptance_is_exp_of_lng_difference(oracle):
+    """One pair, the lower walker in bin 5 with ln g 0 there and -ln 0.3 at
+    the upper walker's bin 6: it exchanges with probability exp(ln 0.3) =
+    0.3, drawing one number of its own stream per call; the upper walker's
+    stream is untouched."""
+    lat, cfgs0, edges, windows = _rx_pair_setup(oracle)
+    ep = oracle._ptr(edges, oracle.C.c_double)
+    assert [oracle.lib().bwo_wl_bin_index(lat.total_energy(c), ep, 10) for c in cfgs0] == [5, 6]
+    lng = np.zeros((2, 10))
+    lng[0, 5] = -np.log(0.3)
+    g0, g1 = oracle.Generator(rank=0), oracle.Generator(rank=1)
+    shadow = oracle.Generator(rank=0)
+    n_calls, hits = 4000, 0
+    for _ in range(n_calls):
+        cfgs = [c.copy(order="F") for c in cfgs0]
+        n, partner, acc = oracle.wl_replica_exchange(lat, cfgs, [g0, g1], lng, edges, windows, 1)
+        u = shadow.genrand()
+        assert list(partner) == [1, 0]
+        assert bool(acc[0]) == (u < 0.3 * (1 + 1e-15)) or abs(u - 0.3) < 1e-12
+        assert n == int(acc[0]) and acc[0] == acc[1]
+        if n:
+            assert np.array_equal(cfgs[0], cfgs0[1]) and np.array_equal(cfgs[1], cfgs0[0])
+        else:
+            assert np.array_equal(cfgs[0], cfgs0[0]) and np.array_equal(cfgs[1], cfgs0[1])
+        hits += n
+    assert g1.state_arrays()[1] == oracle.Generator(rank=1).state_arrays()[1]
+    p = hits / n_calls
+    assert abs(p - 0.3) < 4 * np.sqrt(0.3 * 0.7 / n_calls)
+    # equal ln g everywhere: always exchange; ln g far lower at bin 5 than
+    # at bin 6 of the lower walker: never
+    for row, expect in [(np.zeros(10), 1), (np.where(np.arange(10) == 5, 1e4, 0.0), 0)]:
+        lng = np.tile(row, (2, 1))
+        for _ in range(50):
+            cfgs = [c.copy(order="F") for c in cfgs0]
+            n, _, _ = oracle.wl_replica_exchange(lat, cfgs, [g0, g1], lng, edges, windows, 1)
+            assert n == expect
+
+
+def test_wl_replica_exchange_matching_is_uniform(oracle):
+    """Three walkers per window, all in the overlap: every walker is matched,
+    and the matching (a pairing of two independently shuffled lists) is a
+    uniformly random permutation -- a shuffle that only makes cyclic
+    permutations (Sattolo's, an off-by-one of Fisher-Yates) reaches 3 of 6.
+    Walker 0's stream gives exactly 2*(wpw-1) shuffle draws plus its own
+    decision draw per call."""
+    lat, cfgs0, edges, _ = _rx_pair_setup(oracle)
+    # both configurations in bin 5 or 6: windows [1,6] and [5,10] overlap there
+    windows = np.array([[1, 6], [5, 10]], dtype=np.int32)
+    pool = [cfgs0[0], cfgs0[1], cfgs0[0], cfgs0[1], cfgs0[0], cfgs0[1]]
+    lng = np.zeros((6, 10))
+    gens = [oracle.Generator(rank=k) for k in range(6)]
+    shadow = [oracle.Generator(rank=k) for k in range(6)]
+    counts = {}
+    n_calls = 3000
+    for _ in range(n_calls):
+        cfgs = [c.copy(order="F") for c in pool]
+        n, partner, acc = oracle.wl_replica_exchange(lat, cfgs, gens, lng, edges, windows, 3)
+        assert n == 3 and acc.all()
+        perm = tuple(int(partner[j]) - 3 for j in range(3))
+        assert sorted(perm) == [0, 1, 2]
+        assert all(partner[partner[j]] == j for j in range(6))
+        counts[perm] = counts.get(perm, 0) + 1
+        for _ in range(2 * 2 + 1):
+            shadow[0].genrand()
+        for j in (1, 2):
+            shadow[j].genrand()
+        for j in range(6):
+            assert gens[j].state_arrays()[1] == shadow[j].state_arrays()[1]
+    assert len(counts) == 6
+    expect = n_calls / 6
+    chi2 = sum((c - expect) ** 2 / expect for c in counts.values())
+    assert chi2 < 20.5          # chi-square, 5 degrees of freedom, p = 0.001
+
+
+def test_wl_replica_exchange_only_overlap_walkers_pair(oracle):
+    """Walkers outside the overlap never pair; a window pair with candidates
+    on one side only makes no pair; configurations are only permuted."""
+    lat, cfgs0, edges, _ = _rx_pair_setup(oracle)
+    # bin 5 is only in window 1's range [1,5]; window 2 is [6,10]: no overlap
+    windows = np.array([[1, 5], [6, 10]], dtype=np.int32)
+    gens = [oracle.Generator(rank=k) for k in range(2)]
+    cfgs = [c.copy(order="F") for c in cfgs0]
+    n, partner, _ = oracle.wl_replica_exchange(lat, cfgs, gens, np.zeros((2, 10)), edges, windows, 1)
+    assert n == 0 and (partner == -1).all()
+    # bin 6 walker in window 1 of [1,6]/[6,10]: it is a candidate, but the
+    # window-2 walker (bin 5) is below window 2: no pair
+    windows = np.array([[1, 6], [6, 10]], dtype=np.int32)
+    cfgs = [cfgs0[1].copy(order="F"), cfgs0[0].copy(order="F")]
+    n, partner, _ = oracle.wl_replica_exchange(lat, cfgs, gens, np.zeros((2, 10)), edges, windows, 1)
+    assert n == 0 and (partner == -1).all()
+    assert np.array_equal(cfgs[0], cfgs0[1]) and np.array_equal(cfgs[1], cfgs0[0])
