@@ -318,6 +318,16 @@ module brawl_hip
       real(c_double), intent(out) :: kernel_ms
     end function
 
+    ! replica_exchange (wang-landau.F90:1392-1501) over every walker
+    integer(c_int) function bwl_wl_replica_exchange(wl, partner, exchanged, kernel_ms) &
+        bind(C, name="bwl_wl_replica_exchange")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: wl
+      integer(c_int32_t), intent(out) :: partner(*)
+      integer(c_int32_t), intent(out) :: exchanged(*)
+      real(c_double), intent(out) :: kernel_ms
+    end function
+
     integer(c_int) function c_strlen(s) bind(C, name="strlen")
       import :: c_int, c_ptr
       type(c_ptr), value :: s
