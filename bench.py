@@ -346,17 +346,26 @@ def wang_landau_walkers(calls: int = 10):
                                 energy_max=c["energy_max"], mc_sweeps=1)
     wl.initial_setup(species_numbers=c["counts"])
     _, entered = wl.enter_windows(20 * 8192)
-    ms = 0.0
+    ms = x_ms = 0.0
+    pairs = exchanged = 0
     for _ in range(calls):
         _, k = wl.sweeps(c["wl_f"], average=True)
         ms += k
+        # the reference's main loop exchanges after every sweeps call
+        partner, ex, xk = wl.replica_exchange()
+        x_ms += xk
+        pairs += int((partner >= 0).sum()) // 2
+        exchanged += int(ex.sum()) // 2
     trials = calls * wl.n_walkers * 8192
     wl.close()
-    log(f"[bench] Wang-Landau walkers: {trials} trials in {ms:.1f} ms device")
+    log(f"[bench] Wang-Landau walkers: {trials} trials in {ms:.1f} ms device; "
+        f"{calls} replica exchanges in {x_ms:.2f} ms")
     return {"workload": "16 windows x 8 walkers, bcc 16^3 (8192 sites), S=4, 2 shells, 800 bins; "
                         f"{calls} sweeps calls of 1 sweep after a bounded window entry",
             "walkers": wl.n_walkers, "entered": int(entered.sum()), "trials": trials,
-            "value": trials / (ms / 1e3), "unit": "swap trials/s", "device_ms": ms}
+            "value": trials / (ms / 1e3), "unit": "swap trials/s", "device_ms": ms,
+            "replica_exchange": {"calls": calls, "ms_per_call": x_ms / calls, "pairs": pairs,
+                                 "exchanged": exchanged}}
 
 
 def main():

@@ -401,8 +401,14 @@ __global__ __launch_bounds__(64) void k_wl_exchange(WlExchangeArgs A) {
     for (int j = 0; j < np; ++j) {
       const int a = pr[2 * j], b = pr[2 * j + 1];
       double u;
+      const int mti_a = a == 0 ? 0 : A.mt_index[a];
       if (a == 0) {
         u = genrand_dev(st0, mti0);
+      } else if (mti_a < 624) {
+        // no twist due: the draw is the next stored word, tempered
+        u = (double)mt_temper(A.mt[(size_t)624 * a + mti_a]) * (1.0 / 4294967296.0);
+        __syncthreads();  // every lane has read the index before lane 0 moves it
+        if (lane == 0) A.mt_index[a] = mti_a + 1;
       } else {
         for (int q = lane; q < 624; q += 64) st1[q] = A.mt[(size_t)624 * a + q];
         int m1 = A.mt_index[a];

@@ -24,6 +24,11 @@
  *   - Wang-Landau walker move: parity unpinned by goldens (the reference
  *     test is time-seeded); checked against the compiled reference build
  *     in oracle/_ref where that builds, and by closed-form invariants.
+ *   - Wang-Landau replica exchange: no golden and no reference build (its
+ *     file needs an MPI module); pinned by what the algorithm fixes: the
+ *     exchange probability exp(d ln g), a uniformly random matching of the
+ *     shuffled candidates, the draws each stream gives, overlap-only
+ *     pairing (tests/test_oracle_invariants.py).
  *
  * Conventions
  *   config   int8, the reference's config(1, 2*n1, 2*n2, 2*n3) array in
