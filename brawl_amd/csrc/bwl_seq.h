@@ -362,31 +362,40 @@ __device__ inline double wave_total_energy(const int8_t *lat, const Geo &g, cons
 
 // total_energy (bw_hamiltonian.f90:58-81) in the reference's order: sites in
 // doubled-grid order (x fastest, then y, then z), empty cells skipped, site
-// energies added left to right from 0.0, halved at the end.  Lanes compute
-// 64 consecutive site energies, lane 0 adds them in order.  Every lane
+// energies added left to right from 0.0, halved at the end.  The lanes take
+// 64 consecutive occupied sites (a doubled-grid row holds every other x:
+// fcc in every row, from x = (y+z) mod 2; bcc in the rows with y = z mod 2,
+// from x = z mod 2), and lane 0 adds their energies in order.  Every lane
 // returns the result.
 __device__ inline double ref_total_energy(const int8_t *lat, const Geo &g, const int32_t *off,
                                           const double *vex) {
   const int lane = threadIdx.x & 63;
-  const int total = g.m[0] * g.m[1] * g.m[2];
+  const int hx = g.m[0] >> 1;  // occupied sites per occupied row
+  const int hy = g.m[1] >> 1;
+  const int total = g.n_atoms;
   double e = 0.0;
   for (int base = 0; base < total; base += 64) {
     const int idx = base + lane;
     double v = 0.0;
-    bool occ = false;
-    if (idx < total) {
-      const int x = idx % g.m[0];
-      const int y = (idx / g.m[0]) % g.m[1];
-      const int z = idx / (g.m[0] * g.m[1]);
-      occ = g.lattice == 1 ? (((x + y + z) & 1) == 0)
-                           : ((x & 1) == (y & 1) && (y & 1) == (z & 1));
-      if (occ) {
-        const int addr = padded_addr_xyz(g, x, y, z);
-        const int sub = sub_of(g.lattice, x & 1, y & 1);
-        const int sp = lat[addr];
-        v = shell_sum_lane(lat, g, off, vex, addr, sub, sp, 0);
-        for (int n = 1; n < g.R; ++n) v = v + shell_sum_lane(lat, g, off, vex, addr, sub, sp, n);
+    const bool occ = idx < total;
+    if (occ) {
+      const int row = idx / hx;
+      const int xh = idx - row * hx;
+      int x, y, z;
+      if (g.lattice == 1) {
+        z = row / g.m[1];
+        y = row - z * g.m[1];
+        x = 2 * xh + ((y + z) & 1);
+      } else {
+        z = row / hy;
+        y = 2 * (row - z * hy) + (z & 1);
+        x = 2 * xh + (z & 1);
       }
+      const int addr = padded_addr_xyz(g, x, y, z);
+      const int sub = sub_of(g.lattice, x & 1, y & 1);
+      const int sp = lat[addr];
+      v = shell_sum_lane(lat, g, off, vex, addr, sub, sp, 0);
+      for (int n = 1; n < g.R; ++n) v = v + shell_sum_lane(lat, g, off, vex, addr, sub, sp, n);
     }
     const unsigned long long mask = __ballot(occ);
     for (int l = 0; l < 64; ++l)
