@@ -154,11 +154,13 @@ def cpu_baseline(chain, budget_s: float):
             "port": port}
 
 
-def batched(steps: int, warmup: int):
+def batched(steps: int, warmup: int, tune: bool = True):
     """The batched form: 256 uneven Metropolis chains (fcc/bcc, 8^3..24^3
     cells, 2-5 species with Dirichlet(1) counts, 3 shells, T in [200,2000] K,
-    500 sweeps each) in one batched call."""
-    from brawl_amd import Lattice, anneal_batch, workloads
+    500 sweeps each) in one batched call; first with the layouts the library
+    picks by rule, then (tune) with the layouts the autotuner picks per
+    shape (tuning time reported, outside the timed steps)."""
+    from brawl_amd import Lattice, anneal_batch, autotune, workloads
     chains = workloads.config3()
     lats, params = [], []
     for ch in chains:
@@ -168,31 +170,50 @@ def batched(steps: int, warmup: int):
         lats.append(L)
         params.append(dict(T=ch.T, n_sweeps=ch.n_sweeps, seed=ch.seed))
     starts = [L.get_config() for L in lats]
-    rates, mks, trials = [], [], 0
-    for k in range(warmup + steps):
+
+    def run(tag):
+        rates, mks, trials, alg_bytes = [], [], 0, 0
+        for k in range(warmup + steps):
+            for L, s0 in zip(lats, starts):
+                L.set_config(s0)
+            t0 = time.perf_counter()
+            res, mk = anneal_batch(lats, params)
+            dt = time.perf_counter() - t0
+            tr = sum(r.trials for r in res)
+            # algorithmic bytes: B_trial of each chain's shells (3: fcc z=42, bcc z=26)
+            alg = sum(r.trials * bytes_per_trial(42 if ch.lattice == 1 else 26)
+                      for r, ch in zip(res, chains))
+            log(f"[bench] batched ({tag}) {'warmup' if k < warmup else 'step'}: {tr} trials, "
+                f"makespan {mk:.1f} ms, wall {dt * 1e3:.1f} ms")
+            if k >= warmup:
+                rates.append(tr / dt)
+                mks.append(mk)
+                trials, alg_bytes = tr, alg
+        return rates, mks, trials, alg_bytes
+
+    rates, mks, trials, alg_bytes = run("layouts by rule")
+    out = {"workload": "256 chains, L~U{8..24} fcc/bcc, S~U{2..5}, 3 shells, T~U[200,2000] K, "
+                       "500 sweeps", "chains": len(chains), "trials": trials,
+           "value": float(np.mean(rates)), "unit": "swap trials/s",
+           "device_rate": trials / (np.mean(mks) / 1e3), "makespan_ms": float(np.mean(mks)),
+           "bytes_per_step": int(alg_bytes)}
+    if tune:
         for L, s0 in zip(lats, starts):
             L.set_config(s0)
         t0 = time.perf_counter()
-        res, mk = anneal_batch(lats, params)
-        dt = time.perf_counter() - t0
-        tr = sum(r.trials for r in res)
-        # algorithmic bytes: B_trial of each chain's shells (3: fcc z=42, bcc z=26)
-        alg = sum(r.trials * bytes_per_trial(42 if ch.lattice == 1 else 26)
-                  for r, ch in zip(res, chains))
-        log(f"[bench] batched {'warmup' if k < warmup else 'step'}: {tr} trials, "
-            f"makespan {mk:.1f} ms, wall {dt * 1e3:.1f} ms")
-        if k >= warmup:
-            rates.append(tr / dt)
-            mks.append(mk)
-            trials = tr
-            alg_bytes = alg
+        table = autotune.tune(lats)
+        tune_s = time.perf_counter() - t0
+        table.apply(lats)
+        picks = list(table.choices.values())
+        t_rates, t_mks, t_trials, _ = run("autotuned layouts")
+        out["autotuned"] = {"value": float(np.mean(t_rates)),
+                            "device_rate": t_trials / (np.mean(t_mks) / 1e3),
+                            "makespan_ms": float(np.mean(t_mks)), "tuning_s": tune_s,
+                            "shapes": len(picks), "natural": picks.count("natural"),
+                            "fixed": picks.count("fixed")}
     for L in lats:
         L.close()
-    return {"workload": "256 chains, L~U{8..24} fcc/bcc, S~U{2..5}, 3 shells, T~U[200,2000] K, "
-                        "500 sweeps", "chains": len(chains), "trials": trials,
-            "value": float(np.mean(rates)), "unit": "swap trials/s",
-            "device_rate": trials / (np.mean(mks) / 1e3), "makespan_ms": float(np.mean(mks)),
-            "bytes_per_step": int(alg_bytes)}
+    return out
 
 
 def lds_roofline(plain_gbs: float, batched_out=None):

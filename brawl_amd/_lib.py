@@ -19,6 +19,8 @@ BWL_ERR_DEVICE = -3
 BWL_ERR_NOMEM = -4
 BWL_ERR_STATE = -5
 FCC, BCC = 1, 2
+# parallel-sampler layouts (bwl_set_layout)
+LAYOUTS = {"auto": 0, "compact": 1, "natural": 2, "fixed": 3}
 
 # Every symbol include/brawl_hip.h declares.
 EXPORTS = (
@@ -35,6 +37,7 @@ EXPORTS = (
     "bwl_server_submit_anneal",
     "bwl_server_submit_ns", "bwl_server_submit_wl", "bwl_server_progress", "bwl_server_poll",
     "bwl_server_info", "bwl_server_drain", "bwl_server_memory", "bwl_lds_bandwidth",
+    "bwl_set_layout", "bwl_layouts", "bwl_layout_info",
 )
 
 
@@ -153,6 +156,9 @@ def load():
         "bwl_server_drain": (C.c_int, [vp]),
         "bwl_server_memory": (C.c_int, [vp, i64p, i64p]),
         "bwl_lds_bandwidth": (C.c_int, [C.c_int32, dp, i32p]),
+        "bwl_set_layout": (C.c_int, [vp, C.c_int32]),
+        "bwl_layouts": (C.c_int, [vp, i32p]),
+        "bwl_layout_info": (C.c_int, [vp, C.c_int32, i32p, i64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -312,6 +312,37 @@ int bwl_mc_steps(bwl_ctx *c, double beta, int32_t nbr_swap, int64_t n_trials, in
   return BWL_OK;
 }
 
+int bwl_set_layout(bwl_ctx *c, int32_t layout) {
+  if (!c) return fail(BWL_ERR_ARG, "null context");
+  if (layout < BWL_LAYOUT_AUTO || layout > BWL_LAYOUT_FIXED) return fail(BWL_ERR_ARG, "unknown layout");
+  c->layout = layout;
+  return BWL_OK;
+}
+
+int bwl_layout_info(bwl_ctx *c, int32_t layout, int32_t *threads, int64_t *lds_bytes) {
+  if (!c) return fail(BWL_ERR_ARG, "null context");
+  if (layout < BWL_LAYOUT_AUTO || layout > BWL_LAYOUT_FIXED) return fail(BWL_ERR_ARG, "unknown layout");
+  ParPlan plan;
+  NatGeo nat;
+  const void *fn = nullptr;
+  int t = 0;
+  size_t lds = 0;
+  int rc = anneal_kernel(c->g, plan, nat, fn, t, lds, nullptr, layout);
+  if (rc) return rc;
+  if (threads) *threads = t;
+  if (lds_bytes) *lds_bytes = (int64_t)lds;
+  return BWL_OK;
+}
+
+int bwl_layouts(bwl_ctx *c, int32_t *mask) {
+  if (!c || !mask) return fail(BWL_ERR_ARG, "null argument");
+  *mask = 0;
+  for (int layout = BWL_LAYOUT_COMPACT; layout <= BWL_LAYOUT_FIXED; ++layout)
+    if (bwl_layout_info(c, layout, nullptr, nullptr) == BWL_OK) *mask |= 1 << (layout - 1);
+  bwl::g_err.clear();
+  return BWL_OK;
+}
+
 int bwl_metropolis_anneal(bwl_ctx *c, const bwl_anneal_params *p, double *energies_of_T,
                           double *C_of_T, double *acceptance_of_T, bwl_run_stats *stats) {
   if (!p) return fail(BWL_ERR_ARG, "null argument");
@@ -415,7 +446,7 @@ int ctx_create_borrowed(const bwl_lattice_desc &d, const double *v_ex, char *mem
 namespace bwl {
 
 int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int &threads,
-                  size_t &lds, bool *natural) {
+                  size_t &lds, bool *natural, int layout) {
   const int lim = lds_limit();
   int rc = plan_parallel(g, plan, g_err);
   if (rc) return rc;
@@ -439,8 +470,9 @@ int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int
   const bool fix_ok = nat_ok && !par_specialised(g) && make_nat_fixed(g, fix) &&
                       par_fixed_kernel(g.lattice, g.R, par_count_words(g, true)) != nullptr;
   const int modes[3] = {fix_ok ? 2 : -1, nat_ok ? 1 : -1, 0};
+  // BWL_LAYOUT_COMPACT / _NATURAL / _FIXED are modes 0 / 1 / 2
   for (int mode : modes) {
-    if (mode < 0) continue;
+    if (mode < 0 || (layout != 0 && mode != layout - 1)) continue;
     const bool use_nat = mode != 0, fixed = mode == 2;
     const NatGeo *ng = fixed ? &fix : (use_nat ? &per : nullptr);
     threads = block_size(par_max_threads(g, use_nat, fixed));
@@ -453,6 +485,7 @@ int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int
     if (natural) *natural = use_nat;
     return BWL_OK;
   }
+  if (layout != 0) return fail(BWL_ERR_UNSUPPORTED, "the requested layout is not available for this lattice");
   return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
 }
 
@@ -477,7 +510,7 @@ int anneal_plan(int n, bwl_ctx *const *ctxs, const bwl_anneal_params *params, An
     int threads = 0;
     size_t lds = 0;
     bool natural = false;
-    rc = anneal_kernel(g, it.plan, nat, it.fn, threads, lds, &natural);
+    rc = anneal_kernel(g, it.plan, nat, it.fn, threads, lds, &natural, c->layout);
     if (rc) return rc;
     if (p->asro_every < 0 || p->asro_every > p->n_sweeps)
       return fail(BWL_ERR_ARG, "asro_every must be in 0..n_sweeps");

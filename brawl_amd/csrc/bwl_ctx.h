@@ -47,6 +47,7 @@ struct bwl_ctx {
   unsigned long long phase = 0;  // parallel-sampler round counter
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool borrowed = false;  // device memory and stream belong to someone else
+  int32_t layout = 0;     // parallel-sampler layout: 0 by rule, else BWL_LAYOUT_*
 };
 
 namespace bwl {
@@ -113,9 +114,11 @@ struct AnnealPlan {
   size_t scratch_bytes = 0;
   char *base = nullptr;
 };
-// The parallel-sampler kernel variant, block size and LDS for a lattice.
+// The parallel-sampler kernel variant, block size and LDS for a lattice:
+// layout 0 takes the first that fits of fixed pitch, own pitch, per
+// sublattice; a BWL_LAYOUT_* value takes only that one.
 int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int &threads,
-                  size_t &lds, bool *natural = nullptr);
+                  size_t &lds, bool *natural = nullptr, int layout = 0);
 int anneal_plan(int n, bwl_ctx *const *ctxs, const bwl_anneal_params *params, AnnealPlan &P);
 int anneal_upload(AnnealPlan &P, char *scratch_dev, hipStream_t s);
 int anneal_launch(const AnnealPlan &P, const hipStream_t *streams);

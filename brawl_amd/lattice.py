@@ -86,6 +86,7 @@ class Lattice:
         na = C.c_int32()
         check(L.bwl_n_atoms(h, C.byref(na)))
         self.n_atoms = int(na.value)
+        self.layout = "auto"
 
     # -- lifetime ---------------------------------------------------------
     def close(self):
@@ -196,6 +197,28 @@ class Lattice:
         return rho
 
     # -- parallel sampler -------------------------------------------------
+    def set_layout(self, layout) -> None:
+        """The parallel sampler's LDS layout for later anneals: "auto" (by
+        rule), "compact", "natural" or "fixed" (see bwl_set_layout)."""
+        code = _lib.LAYOUTS[layout] if isinstance(layout, str) else int(layout)
+        check(self._L.bwl_set_layout(self._h, code))
+        self.layout = layout if isinstance(layout, str) else {v: k for k, v in _lib.LAYOUTS.items()}[code]
+
+    def layouts(self) -> list:
+        """The layouts available to this lattice, in the rule's order of
+        preference (fixed, natural, compact)."""
+        m = C.c_int32()
+        check(self._L.bwl_layouts(self._h, C.byref(m)))
+        return [name for name in ("fixed", "natural", "compact")
+                if m.value & (1 << (_lib.LAYOUTS[name] - 1))]
+
+    def layout_info(self, layout="auto"):
+        """(threads per chain, LDS bytes) of the sampler with that layout."""
+        code = _lib.LAYOUTS[layout] if isinstance(layout, str) else int(layout)
+        t, b = C.c_int32(), C.c_int64()
+        check(self._L.bwl_layout_info(self._h, code, C.byref(t), C.byref(b)))
+        return int(t.value), int(b.value)
+
     def metropolis_simulated_annealing(self, *, T, delta_T=0.0, T_steps=1, n_sweeps,
                                        burn_in_sweeps=0, seed=20261202,
                                        asro_every=0) -> AnnealResult:

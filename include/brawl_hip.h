@@ -161,6 +161,31 @@ int bwl_metropolis_anneal_rho(bwl_ctx *ctx, const bwl_anneal_params *p, double *
                               double *C_of_T, double *acceptance_of_T, double *rho_of_T,
                               bwl_run_stats *stats);
 
+/* Layout of the parallel sampler's lattice in LDS, the tunable part of the
+ * kernel choice (no counterpart in the reference).  BWL_LAYOUT_AUTO (the
+ * default) takes the first of FIXED, NATURAL, COMPACT that exists for the
+ * lattice and fits in LDS.  NATURAL and FIXED run the same sampler on the
+ * lattice held in lattice coordinates (NATURAL with this size's own row
+ * pitch, or the kernel compiled for this exact shape where there is one;
+ * FIXED with a pitch shared by all sizes of the lattice type and shell
+ * count and compile-time neighbour offsets) and give identical results;
+ * COMPACT holds the lattice per sublattice and draws its random numbers per
+ * pair, so its chain differs.  bwl_set_layout applies to the context's
+ * later anneals (an unavailable layout fails there with
+ * BWL_ERR_UNSUPPORTED); bwl_layouts sets bit (layout - 1) of *mask for
+ * every layout available to the context's lattice; bwl_layout_info gives
+ * the threads per chain and the LDS bytes of one layout (AUTO: the rule's
+ * pick), or BWL_ERR_UNSUPPORTED.  A chain's random numbers are keyed by
+ * its thread and step: layouts run with the same thread count give the
+ * same chain. */
+#define BWL_LAYOUT_AUTO 0
+#define BWL_LAYOUT_COMPACT 1
+#define BWL_LAYOUT_NATURAL 2
+#define BWL_LAYOUT_FIXED 3
+int bwl_set_layout(bwl_ctx *ctx, int32_t layout);
+int bwl_layouts(bwl_ctx *ctx, int32_t *mask);
+int bwl_layout_info(bwl_ctx *ctx, int32_t layout, int32_t *threads, int64_t *lds_bytes);
+
 /* Batched form: n independent chains (contexts on one device, uneven
  * lattice sizes, compositions and schedules) annealed in as few launches
  * as possible -- one workgroup per chain, chains with the same kernel

@@ -361,3 +361,91 @@ def test_batched_chains_match_single_runs(hip, oracle):
         assert abs(ref.total_energy(f) - r.energies_of_T[0] * ref.n_atoms) <= 1e-11 * ref.n_atoms
     for L in lats:
         L.close()
+
+
+LAYOUT_SHAPES = [(1, (6, 6, 6), 3, 3), (2, (8, 8, 8), 4, 3), (1, (10, 10, 10), 5, 3),
+                 (2, (14, 14, 14), 2, 2), (1, (12, 12, 12), 4, 4), (2, (20, 20, 20), 3, 3)]
+
+
+def test_layouts_same_chain_at_equal_threads_and_exact_otherwise(hip, oracle):
+    """Every available layout conserves composition and carries the exact
+    energy; the two lattice-coordinate layouts (own pitch, fixed pitch) give
+    the same chain whenever they run it with the same thread count, since
+    the random numbers are keyed by thread and step; a layout the lattice
+    does not have is refused."""
+    from brawl_amd import workloads
+    same = 0
+    for lat, n, S, R in LAYOUT_SHAPES:
+        v = workloads.random_v(S, [25.0 / k for k in range(1, R + 1)], 7 + S)
+        counts = workloads.dirichlet_counts(workloads.n_atoms_of(lat, n), S, np.random.default_rng(S))
+        ref = oracle.OracleLattice(lat, n, v)
+        cfg = ref.initial_setup(oracle.Generator(110179), species_numbers=counts)
+        outs = {}
+        with hip.Lattice(lat, n, v) as L0:
+            names = L0.layouts()
+            info = {c: L0.layout_info(c) for c in names}
+            assert L0.layout_info("auto") == info[names[0]]      # the rule's first choice
+            for c in ("compact", "natural", "fixed"):
+                if c not in names:
+                    L0.set_layout(c)
+                    L0.set_config(cfg)
+                    with pytest.raises(hip.BrawlError):
+                        L0.metropolis_simulated_annealing(T=1200.0, n_sweeps=1, seed=3)
+        _say("layouts", lat, n, S, R, info)
+        for c in names:
+            with hip.Lattice(lat, n, v) as L:
+                L.set_layout(c)
+                L.set_config(cfg)
+                res = L.metropolis_simulated_annealing(T=1200.0, n_sweeps=1, burn_in_sweeps=2, seed=3)
+                out = L.get_config()
+                S1 = S + 1
+                assert np.array_equal(np.bincount(out.ravel(), minlength=S1),
+                                      np.bincount(cfg.ravel(), minlength=S1))
+                e_ref = ref.total_energy(out)
+                assert abs(res.energies_of_T[0] * ref.n_atoms - e_ref) <= 1e-11 * max(1.0, abs(e_ref))
+                outs[c] = (out, res)
+        if "natural" in outs and "fixed" in outs and info["natural"][0] == info["fixed"][0]:
+            same += 1
+            (o1, r1), (o2, r2) = outs["natural"], outs["fixed"]
+            assert np.array_equal(o1, o2)
+            assert r1.energies_of_T[0] == r2.energies_of_T[0] and r1.accepted == r2.accepted
+    assert same > 0
+    with hip.Lattice(1, (4, 4, 4), workloads.random_v(2, [20.0], 1)) as L:
+        with pytest.raises(hip.BrawlError):
+            L.set_layout(9)
+
+
+def test_autotuned_batch_matches_single_runs(hip, oracle):
+    """Layouts picked by the autotuner per shape, saved and loaded as JSON:
+    the batch gives exactly what each chain gives alone with the same
+    layout, and the tracked energies equal recounts."""
+    from brawl_amd import autotune, workloads
+    chains = workloads.config3(n_chains=8, n_sweeps=1)
+    lats, params = [], []
+    for ch in chains:
+        L = hip.Lattice(ch.lattice, ch.n, ch.v_ex)
+        L.seed(ch.seed & 0xFFFFFFFF)
+        L.initial_setup(species_numbers=ch.counts)
+        lats.append(L)
+        params.append(dict(T=ch.T, n_sweeps=1, burn_in_sweeps=2, seed=ch.seed))
+    starts = [L.get_config() for L in lats]
+    table = autotune.tune(lats, sweeps=2, repeats=1)
+    table = autotune.LayoutTable.from_json(table.to_json())
+    assert table.apply(lats) == len(lats)
+    _say("tuned", table.choices)
+    for L, s0 in zip(lats, starts):
+        assert np.array_equal(L.get_config(), s0)           # tuning leaves the chains alone
+    res, _ = hip.anneal_batch(lats, params)
+    for L, s0, p, r in zip(lats, starts, params, res):
+        f = L.get_config()
+        L2 = hip.Lattice(L.lattice, L.n, L.v_ex)
+        L2.set_layout(table.get(L))
+        L2.set_config(s0)
+        r1 = L2.metropolis_simulated_annealing(**p)
+        assert np.array_equal(L2.get_config(), f)
+        assert r1.energies_of_T[0] == r.energies_of_T[0] and r1.accepted == r.accepted
+        L2.close()
+        ref = oracle.OracleLattice(L.lattice, L.n, L.v_ex)
+        assert abs(ref.total_energy(f) - r.energies_of_T[0] * ref.n_atoms) <= 1e-11 * ref.n_atoms
+    for L in lats:
+        L.close()
