@@ -318,6 +318,28 @@ module brawl_hip
       real(c_double), intent(out) :: kernel_ms
     end function
 
+    ! the parallel sampler's lattice layout in LDS (no reference counterpart)
+    integer(c_int) function bwl_set_layout(ctx, layout) bind(C, name="bwl_set_layout")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: layout
+    end function
+
+    integer(c_int) function bwl_layouts(ctx, mask) bind(C, name="bwl_layouts")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: mask
+    end function
+
+    integer(c_int) function bwl_layout_info(ctx, layout, threads, lds_bytes) &
+        bind(C, name="bwl_layout_info")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: layout
+      integer(c_int32_t), intent(out) :: threads
+      integer(c_int64_t), intent(out) :: lds_bytes
+    end function
+
     ! replica_exchange (wang-landau.F90:1392-1501) over every walker
     integer(c_int) function bwl_wl_replica_exchange(wl, partner, exchanged, kernel_ms) &
         bind(C, name="bwl_wl_replica_exchange")
