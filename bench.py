@@ -53,10 +53,11 @@ def bytes_per_trial(z: int) -> int:
 
 
 # HBM bytes per launch of the plain anneal kernel, from rocprofv3 --pmc
-# FETCH_SIZE (68.0625 KiB, x2.0 copy-kernel scale) and WRITE_SIZE (96.46875
-# KiB, x1.0): the lattice is read once into LDS and written back once, plus
-# the per-T-step outputs.
-TRAFFIC_BYTES = int(round((68.0625 * 2.0 + 96.46875) * 1024))
+# FETCH_SIZE (115.0625 KiB, x2.0 copy-kernel scale) and WRITE_SIZE
+# (232.46875 KiB, x1.0): the lattice is read once into LDS and written back
+# once, plus the per-T-step outputs and the register spills of the
+# radial-density reduction.
+TRAFFIC_BYTES = int(round((115.0625 * 2.0 + 232.46875) * 1024))
 PLAIN_KERNEL = "k_metro_nat<bcc, R=4, 32-bit counts, 32 cells>"
 # radial densities (alpha^(1..4)) sampled every 10 sweeps of each T step
 ASRO_EVERY = 10

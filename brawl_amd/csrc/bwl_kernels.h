@@ -168,9 +168,11 @@ size_t seq_lds_bytes(const Geo &g);
 hipError_t launch_metro_seq(const SeqArgs &a, hipStream_t s);
 size_t par_lds_bytes(const Geo &g, const NatGeo *nat, int threads);
 // Block size limit of the parallel-sampler kernels: the natural-layout
-// kernels with compile-time neighbour offsets and 32-bit count words keep
-// few enough registers for 12 wavefronts per CU, the others are limited to 8.
-constexpr int kMaxNatThreads = 768;
+// kernels with compile-time neighbour offsets and 32-bit count words run 16
+// wavefronts per CU (128 registers each; what the compiler then spills is
+// in the radial-density reduction and the per-round set-up, not the pair
+// loop), the others are limited to 8.
+constexpr int kMaxNatThreads = 1024;
 // fixed: the fixed-pitch natural layout (nat_dims_fixed)
 int par_max_threads(const Geo &g, bool nat, bool fixed);
 // packed count words of the sampler for a lattice (see par_count_words in
