@@ -20,10 +20,14 @@
  *     brute-force O(N^2) pair enumeration on tiny lattices.
  *   - radial densities / lattice shells: golden rho_of_T.nc and ASRO
  *     trajectories.
- *   - nested sampling: golden .energies file (tests/99_ref 03).
- *   - Wang-Landau walker move: parity unpinned by goldens (the reference
- *     test is time-seeded); checked against the compiled reference build
- *     in oracle/_ref where that builds, and by closed-form invariants.
+ *   - nested sampling: golden .energies file (tests/99_ref 03,
+ *     statistically: that golden is stale), and bit for bit against the
+ *     reference's loop (nested_sampling.f90:78-200) run over the
+ *     reference's own compiled routines in oracle/_ref.
+ *   - Wang-Landau sweeps and window entry: the reference test is
+ *     time-seeded; bit for bit against the reference's loops
+ *     (wang-landau.F90:539-741, MPI left out) run over its own compiled
+ *     routines in oracle/_ref, and closed-form invariants.
  *   - Wang-Landau replica exchange: no golden and no reference build (its
  *     file needs an MPI module); pinned by what the algorithm fixes: the
  *     exchange probability exp(d ln g), a uniformly random matching of the

@@ -7,7 +7,9 @@
 #                    pointers to the GPU adaptors (brawl_amd/fortran) and
 #                    checks them against the reference's routines
 # Modules that need NetCDF or a flang-built MPI module (metropolis.F90,
-# nested_sampling.f90, wang-landau.F90, main.F90) are not built.
+# nested_sampling.f90, wang-landau.F90, main.F90) are not built; the shim
+# restates their sampling loops (Metropolis sweeps, Wang-Landau sweeps and
+# window entry, the nested-sampling loop) over the built modules' routines.
 REF ?= /root/reference/src
 AMDFLANG ?= /opt/rocm/lib/llvm/bin/amdflang
 FC := $(AMDFLANG)

@@ -49,6 +49,11 @@ def lib():
             "ref_radial_densities": (None, [i32] * 6 + [i8p, dp, dp]),
             "ref_metropolis_sweeps": (C.c_int64, [i32] * 6 + [dp, i8p, C.c_double, i32, i32,
                                                              dp, dp]),
+            "ref_wl_sweeps": (C.c_int64, [i32] * 6 + [dp, i8p] + [i32] * 5 + [dp, i32, i32,
+                                                                         C.c_double, dp, dp]),
+            "ref_wl_enter": (C.c_int64, [i32] * 6 + [dp, i8p, i32, C.c_double, C.c_double,
+                                                     C.c_float, C.c_float, C.c_int64]),
+            "ref_nested_sampling": (C.c_int32, [i32] * 6 + [dp, i32, i64p, i32, i32, i32, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -158,6 +163,52 @@ def _metropolis_sweeps(lattice, n, v_ex, cfg, beta, n_sweeps, rank=0):
 
 
 metropolis_sweeps = _isolated(_metropolis_sweeps)
+
+
+@_isolated
+def wl_sweeps(lattice, n, v_ex, cfg, *, rank, nbr_swap, mc_sweeps, n_calls, bin_edges, start_idx,
+              end_idx, wl_f, lng, hist):
+    """n_calls `sweeps` calls of one walker from the stream of `rank`
+    (ref_wl_sweeps); returns (accepted, configuration, ln g, histogram)."""
+    v = np.asfortranarray(v_ex, dtype=np.float64)
+    S, _, R = v.shape
+    c, cp = _cfg(np.array(cfg, dtype=np.int8, order="F", copy=True))
+    e = np.ascontiguousarray(bin_edges, dtype=np.float64)
+    lg = np.array(lng, dtype=np.float64, copy=True)
+    h = np.array(hist, dtype=np.float64, copy=True)
+    acc = lib().ref_wl_sweeps(lattice, *n, S, R, _p(v, C.c_double), cp, int(rank), int(nbr_swap),
+                              int(mc_sweeps), int(n_calls), len(e) - 1, _p(e, C.c_double),
+                              int(start_idx), int(end_idx), float(wl_f), _p(lg, C.c_double),
+                              _p(h, C.c_double))
+    return int(acc), np.array(c), lg, h
+
+
+@_isolated
+def wl_enter(lattice, n, v_ex, cfg, *, rank, min_e, max_e, energy_min, energy_max, max_trials):
+    """enter_energy_window of one walker from the stream of `rank`
+    (ref_wl_enter); returns (trials, configuration)."""
+    v = np.asfortranarray(v_ex, dtype=np.float64)
+    S, _, R = v.shape
+    c, cp = _cfg(np.array(cfg, dtype=np.int8, order="F", copy=True))
+    t = lib().ref_wl_enter(lattice, *n, S, R, _p(v, C.c_double), cp, int(rank), float(min_e),
+                           float(max_e), float(energy_min), float(energy_max), int(max_trials))
+    return int(t), np.array(c)
+
+
+@_isolated
+def nested_sampling(lattice, n, v_ex, species_numbers, *, rank, n_walkers, n_steps, n_iter):
+    """The nested-sampling loop from the stream of `rank`
+    (ref_nested_sampling); returns the n_iter energy limits."""
+    v = np.asfortranarray(v_ex, dtype=np.float64)
+    S, _, R = v.shape
+    sn = np.ascontiguousarray(species_numbers, dtype=np.int64)
+    out = np.zeros(n_iter)
+    rc = lib().ref_nested_sampling(lattice, *n, S, R, _p(v, C.c_double), int(rank),
+                                   _p(sn, C.c_int64), int(n_walkers), int(n_steps), int(n_iter),
+                                   _p(out, C.c_double))
+    if rc:
+        raise RuntimeError(f"ref_nested_sampling returned {rc}")
+    return out
 
 
 def metropolis_sweeps_replicas(lattice, n, v_ex, cfgs, beta, n_sweeps):

@@ -10,7 +10,7 @@ module ref_shim
 
   use iso_c_binding
   use kinds, only : real64, int64
-  use constants, only : array_int
+  use constants, only : array_int, Ry_to_eV
   use shared_data, only : config, V_ex
   use derived_types, only : run_params
   use c_functions, only : f90_init_genrand, genrand
@@ -214,5 +214,242 @@ contains
     nc = size(config)
     cfg(1:nc) = int(reshape(config, [nc]), c_int8_t)
   end function ref_metropolis_sweeps
+
+  !> wang-landau.F90 bin_index (:515-523), restated here because the module
+  !> needs an MPI module this image does not have.
+  integer function wl_bin(energy, edges, bins)
+    real(real64), intent(in) :: energy, edges(*)
+    integer, intent(in) :: bins
+    real(real64) :: bin_range
+    bin_range = edges(bins + 1) - edges(1)
+    wl_bin = int(((energy - edges(1))/(bin_range))*real(bins, real64)) + 1
+  end function wl_bin
+
+  !> n_calls `sweeps` calls (wang-landau.F90:539-632) of one walker after
+  !> f90_init_genrand for `rank`: the loop of the reference with its own
+  !> routines (setup%full_energy, setup%rdm_site / rdm_nbr as mc_select,
+  !> pair_energy, pair_swap, genrand), without the window average over
+  !> ranks and the radial-density and configuration stores (they draw no
+  !> numbers).  The walker must start inside its window [start_idx,
+  !> end_idx] (it then never leaves it).  cfg, lng (bins) and hist
+  !> (end_idx-start_idx+1) in/out; returns the accepted count.
+  function ref_wl_sweeps(lattice, n1, n2, n3, ns, nr, vex, cfg, rank, nbr_swap, mc_sweeps, &
+                         n_calls, bins, edges, start_idx, end_idx, wl_f, lng, hist) result(acc) &
+      bind(C, name="ref_wl_sweeps")
+    integer(c_int32_t), value :: lattice, n1, n2, n3, ns, nr, rank, nbr_swap, mc_sweeps, n_calls, &
+                                 bins, start_idx, end_idx
+    real(c_double), value :: wl_f
+    real(c_double), intent(in) :: vex(*), edges(*)
+    integer(c_int8_t), intent(inout) :: cfg(*)
+    real(c_double), intent(inout) :: lng(*), hist(*)
+    integer(c_int64_t) :: acc
+    type(run_params) :: setup
+    integer(c_int) :: seed
+    integer :: call_no, i, ibin, jbin, nc
+    integer, dimension(4) :: rdm1, rdm2
+    integer(array_int) :: site1, site2
+    real(real64) :: e_swapped, e_unswapped, pair_unswapped, pair_swapped
+    call make_setup(setup, lattice, n1, n2, n3, ns, nr)
+    call load(setup, vex, cfg)
+    seed = f90_init_genrand(0_c_int, int(rank, c_int), 0_c_int)
+    acc = 0_c_int64_t
+    do call_no = 1, n_calls
+      e_unswapped = setup%full_energy(config)
+      e_swapped = e_unswapped
+      do i = 1, mc_sweeps*setup%n_atoms
+        rdm1 = setup%rdm_site()
+        if (nbr_swap /= 0) then
+          rdm2 = setup%rdm_nbr(rdm1)
+        else
+          rdm2 = setup%rdm_site()
+        end if
+        site1 = config(rdm1(1), rdm1(2), rdm1(3), rdm1(4))
+        site2 = config(rdm2(1), rdm2(2), rdm2(3), rdm2(4))
+        e_swapped = e_unswapped
+        pair_unswapped = pair_energy(setup, config, rdm1, rdm2)
+        pair_swapped = pair_unswapped
+        call pair_swap(config, rdm1, rdm2)
+        if (site1 /= site2) then
+          pair_swapped = pair_energy(setup, config, rdm1, rdm2)
+          e_swapped = e_unswapped - pair_unswapped + pair_swapped
+        end if
+        ibin = wl_bin(e_unswapped, edges, bins)
+        jbin = wl_bin(e_swapped, edges, bins)
+        if (jbin > start_idx - 1 .and. jbin < end_idx + 1) then
+          if (log(genrand()) .lt. (lng(ibin) - lng(jbin))) then
+            acc = acc + 1
+            e_unswapped = e_swapped
+          else
+            call pair_swap(config, rdm1, rdm2)
+            jbin = ibin
+          end if
+        else
+          jbin = ibin
+          call pair_swap(config, rdm1, rdm2)
+        end if
+        if (MOD(i, INT(0.02_real64*REAL(setup%n_atoms))) == 0) then
+          hist(jbin - start_idx + 1) = hist(jbin - start_idx + 1) + 1.0_real64
+        end if
+        lng(jbin) = lng(jbin) + wl_f
+      end do
+    end do
+    nc = size(config)
+    cfg(1:nc) = int(reshape(config, [nc]), c_int8_t)
+  end function ref_wl_sweeps
+
+  !> enter_energy_window (wang-landau.F90:643-741) of one walker after
+  !> f90_init_genrand for `rank`, without the hand-off between the ranks of
+  !> a window: the reference's loop with initial_setup every n_atoms*250
+  !> iterations (species concentrations 1/ns), its own routines for the
+  !> trials, stopping after max_trials trials.  min_e, max_e are the
+  !> window's outer bin edges; energy_min, energy_max the REAL(4) bounds in
+  !> meV/atom.  cfg in/out; returns the trials made.
+  function ref_wl_enter(lattice, n1, n2, n3, ns, nr, vex, cfg, rank, min_e, max_e, energy_min, &
+                        energy_max, max_trials) result(trials) bind(C, name="ref_wl_enter")
+    integer(c_int32_t), value :: lattice, n1, n2, n3, ns, nr, rank
+    real(c_double), value :: min_e, max_e
+    real(c_float), value :: energy_min, energy_max
+    integer(c_int64_t), value :: max_trials
+    real(c_double), intent(in) :: vex(*)
+    integer(c_int8_t), intent(inout) :: cfg(*)
+    integer(c_int64_t) :: trials
+    type(run_params) :: setup
+    integer(c_int) :: seed
+    integer :: i_steps, sweeps, nc
+    integer, dimension(4) :: rdm1, rdm2
+    integer(array_int) :: site1, site2
+    real :: e_min4, e_max4
+    real(real64) :: e_swapped, e_unswapped, pair_swapped, pair_unswapped, delta_e, target_energy, &
+                    condition
+    call make_setup(setup, lattice, n1, n2, n3, ns, nr)
+    call load(setup, vex, cfg)
+    seed = f90_init_genrand(0_c_int, int(rank, c_int), 0_c_int)
+    e_min4 = energy_min
+    e_max4 = energy_max
+    target_energy = (min_e + max_e)/2.0_real64
+    condition = ABS(max_e - min_e)*0.1_real64
+    e_unswapped = setup%full_energy(config)
+    trials = 0_c_int64_t
+    i_steps = 0
+    sweeps = 50
+    do while (.true.)
+      i_steps = i_steps + 1
+      if (MOD(i_steps, setup%n_atoms*sweeps*5) == 0) then
+        i_steps = 0
+        call initial_setup(setup, config)
+      end if
+      if (e_unswapped < max_e - condition .and. e_unswapped > min_e + condition) then
+        e_unswapped = setup%full_energy(config)
+        if (e_unswapped < max_e - condition .and. e_unswapped > min_e + condition) then
+        else
+          cycle
+        end if
+        exit
+      end if
+      if (trials >= max_trials) exit
+      trials = trials + 1
+      rdm1 = setup%rdm_site()
+      rdm2 = setup%rdm_site()
+      site1 = config(rdm1(1), rdm1(2), rdm1(3), rdm1(4))
+      site2 = config(rdm2(1), rdm2(2), rdm2(3), rdm2(4))
+      if (site1 /= site2) then
+        pair_unswapped = pair_energy(setup, config, rdm1, rdm2)
+        call pair_swap(config, rdm1, rdm2)
+        pair_swapped = pair_energy(setup, config, rdm1, rdm2)
+        e_swapped = e_unswapped - pair_unswapped + pair_swapped
+        delta_e = ((e_swapped-target_energy)**2-(e_unswapped-target_energy)**2)/ &
+                  (2.0_real64*(0.0025*ABS(e_max4 - e_min4)*setup%n_atoms/(Ry_to_eV*1000))**2)
+        if (log(genrand()) .lt. -delta_e) then
+          e_unswapped = e_swapped
+        else
+          call pair_swap(config, rdm1, rdm2)
+        end if
+      end if
+    end do
+    nc = size(config)
+    cfg(1:nc) = int(reshape(config, [nc]), c_int8_t)
+  end function ref_wl_enter
+
+  !> The nested-sampling loop (nested_sampling.f90:78-200) after
+  !> f90_init_genrand for `rank`, with the reference's own routines
+  !> (initial_setup with species_numbers, genrand, setup%full_energy,
+  !> setup%rdm_site, pair_energy, pair_swap); the energies file, trajectory
+  !> and screen output are left out.  n_walkers >= 4, so the first
+  !> walk-length check follows a walk.  e_limits(n_iter) out; returns 0.
+  function ref_nested_sampling(lattice, n1, n2, n3, ns, nr, vex, rank, species_numbers, n_walkers, &
+                               n_steps, n_iter, e_limits) result(status) &
+      bind(C, name="ref_nested_sampling")
+    integer(c_int32_t), value :: lattice, n1, n2, n3, ns, nr, rank, n_walkers, n_steps, n_iter
+    real(c_double), intent(in) :: vex(*)
+    integer(c_int64_t), intent(in) :: species_numbers(*)
+    real(c_double), intent(out) :: e_limits(*)
+    integer(c_int32_t) :: status
+    type(run_params) :: setup
+    integer(c_int) :: seed
+    integer(array_int), allocatable :: walkers(:,:,:,:,:)
+    real(real64), allocatable :: walker_energies(:)
+    integer :: i_walker, i_iter, i_step, extra_steps, n_acc, n_at, irnd
+    integer, dimension(1) :: i_max
+    integer, dimension(4) :: rdm1, rdm2
+    integer(array_int) :: site1, site2
+    real(real64) :: rnde, rnd, ener_limit, e_unswapped, e_swapped, delta_e
+    call make_setup(setup, lattice, n1, n2, n3, ns, nr)
+    call load_vex(vex)
+    setup%species_numbers = species_numbers(1:ns)
+    setup%species_concentrations = 0.0_real64
+    seed = f90_init_genrand(0_c_int, int(rank, c_int), 0_c_int)
+    allocate(walkers(setup%n_basis, 2*n1, 2*n2, 2*n3, n_walkers), walker_energies(n_walkers))
+    n_at = setup%n_1*setup%n_2*setup%n_3*setup%n_basis*setup%n_species
+    do i_walker = 1, n_walkers
+      call initial_setup(setup, config)
+      walkers(:,:,:,:,i_walker) = config
+      rnde = genrand()
+      walker_energies(i_walker) = setup%full_energy(walkers(:,:,:,:,i_walker)) + rnde*1e-8
+    end do
+    extra_steps = 0
+    n_acc = 0
+    do i_iter = 1, n_iter
+      i_max = maxloc(walker_energies)
+      ener_limit = walker_energies(i_max(1))
+      e_limits(i_iter) = ener_limit
+      if (mod(i_iter, int(n_walkers/2.0)) == 0) then
+        if ((n_acc < n_at*0.05) .and. (extra_steps < n_steps*100)) then
+          extra_steps = extra_steps + int(n_steps)
+        end if
+      end if
+      rnd = genrand()
+      irnd = ceiling(rnd*n_walkers)
+      walkers(:,:,:,:,i_max(1)) = walkers(:,:,:,:,irnd)
+      walker_energies(i_max(1)) = walker_energies(irnd)
+      n_acc = 0
+      do i_step = 1, n_steps + extra_steps
+        rdm1 = setup%rdm_site()
+        site1 = walkers(rdm1(1), rdm1(2), rdm1(3), rdm1(4), i_max(1))
+        do
+          rdm2 = setup%rdm_site()
+          site2 = walkers(rdm2(1), rdm2(2), rdm2(3), rdm2(4), i_max(1))
+          if (site1 /= site2) exit
+        end do
+        e_unswapped = pair_energy(setup, walkers(:,:,:,:,i_max(1)), rdm1, rdm2)
+        call pair_swap(walkers(:,:,:,:,i_max(1)), rdm1, rdm2)
+        e_swapped = pair_energy(setup, walkers(:,:,:,:,i_max(1)), rdm1, rdm2)
+        delta_e = e_swapped - e_unswapped
+        if (walker_energies(i_max(1)) + delta_e < ener_limit) then
+          walker_energies(i_max(1)) = walker_energies(i_max(1)) + delta_e
+          n_acc = n_acc + 1
+        else
+          call pair_swap(walkers(:,:,:,:,i_max(1)), rdm1, rdm2)
+        end if
+      end do
+    end do
+    status = 0
+  end function ref_nested_sampling
+
+  subroutine load_vex(vex)
+    real(c_double), intent(in) :: vex(*)
+    integer :: nv
+    nv = size(V_ex)
+    V_ex = reshape(vex(1:nv), shape(V_ex))
+  end subroutine load_vex
 
 end module ref_shim

@@ -113,3 +113,86 @@ def test_metropolis_path_bit_exact_against_reference(oracle, lat, n, S, Rs, T, r
     assert acc == acc_r
     assert np.array_equal(cfg, cfg_r)
     assert o.total_energy(cfg) == e_r
+
+
+WL_CASES = [
+    # lattice, n, S, R, nbr_swap, energy range (meV/atom), bins, windows
+    (2, (4, 4, 4), 3, 2, 0, (9.0, 16.0), 30, 3),
+    (1, (3, 3, 3), 2, 3, 1, (-6.0, 6.0), 24, 2),
+    (2, (3, 4, 3), 4, 1, 0, (-8.0, 8.0), 20, 2),
+]
+
+
+def _wl_case(oracle, lat, n, S, Rs, lo, hi, bins, nwin):
+    from brawl_amd import workloads
+    v = workloads.random_v(S, [30.0 / k for k in range(1, Rs + 1)], 5 + S + Rs)
+    o = oracle.OracleLattice(lat, n, v)
+    windows, edges = o.wl_setup(bins, nwin, 0.5, lo, hi)
+    return v, o, windows, edges
+
+
+@pytest.mark.parametrize("lat,n,S,Rs,nbr,rng_e,bins,nwin", WL_CASES)
+def test_wang_landau_walker_bit_exact_against_reference(oracle, lat, n, S, Rs, nbr, rng_e, bins, nwin):
+    """enter_energy_window then three `sweeps` calls per walker, the oracle
+    against the reference's loops over its own compiled routines
+    (wang-landau.F90:539-741; the MPI parts left out on both sides): trial
+    counts, configurations, ln g, histograms and accepted counts equal."""
+    lo, hi = rng_e
+    v, o, windows, edges = _wl_case(oracle, lat, n, S, Rs, lo, hi, bins, nwin)
+    conc = np.full(S, 1.0 / S)
+    swept = 0
+    for w in range(nwin):
+        # both sides seed the rank's stream afresh before each phase
+        rank = 3 * w + 1
+        cfg0 = o.initial_setup(oracle.Generator(rank=100 + w), concentrations=conc)
+        s, e = (int(x) for x in windows[w])
+        min_e, max_e = edges[s - 1], edges[e]
+        cfg = np.array(cfg0, order="F", copy=True)
+        t = o.wl_enter_window(cfg, oracle.Generator(rank=rank), min_e, max_e, lo, hi,
+                              concentrations=conc, max_trials=200000)
+        t_ref, cfg_ref = R.wl_enter(lat, n, v, cfg0, rank=rank, min_e=min_e, max_e=max_e,
+                                    energy_min=lo, energy_max=hi, max_trials=200000)
+        assert t == t_ref and np.array_equal(cfg, cfg_ref), (w, t, t_ref)
+        b = oracle.lib().bwo_wl_bin_index(o.total_energy(cfg), oracle._ptr(edges, oracle.C.c_double),
+                                          bins)
+        if not s <= b <= e:
+            continue            # not entered within the bound: sweeps need a walker inside
+        swept += 1
+        # sweeps from the entered walker, ln g random so that both branches occur
+        lng0 = np.random.default_rng(w).normal(0.0, 0.3, bins)
+        lng = lng0.copy()
+        h = np.zeros(e - s + 1)
+        g2 = oracle.Generator(rank=rank)
+        acc = 0
+        for _ in range(3):
+            acc += o.wl_sweeps(cfg, g2, mc_sweeps=1, bin_edges=edges, start_idx=s, end_idx=e,
+                               wl_f=0.25, lng=lng, hist=h, nbr_swap=bool(nbr))
+        acc_r, cfg_r, lng_r, h_r = R.wl_sweeps(lat, n, v, cfg_ref, rank=rank, nbr_swap=nbr,
+                                               mc_sweeps=1, n_calls=3, bin_edges=edges,
+                                               start_idx=s, end_idx=e, wl_f=0.25, lng=lng0,
+                                               hist=np.zeros(e - s + 1))
+        assert acc == acc_r and acc > 0
+        assert np.array_equal(cfg, cfg_r)
+        assert np.array_equal(lng, lng_r) and np.array_equal(h, h_r)
+    assert swept > 0
+
+
+@pytest.mark.parametrize("lat,n,S,Rs,nw,steps,iters,rank", [
+    (1, (2, 2, 2), 5, 4, 8, 30, 40, 0),
+    (2, (3, 3, 3), 3, 2, 6, 20, 60, 2),
+    (1, (3, 3, 3), 4, 3, 10, 25, 30, 5),
+    (1, (2, 2, 2), 5, 4, 4, 5, 400, 1),      # late iterations: walk-length adaptation
+])
+def test_nested_sampling_bit_exact_against_reference(oracle, lat, n, S, Rs, nw, steps, iters, rank):
+    """The nested-sampling loop (nested_sampling.f90:78-200) of the oracle
+    against the reference's loop over its own compiled routines: every
+    energy limit equal."""
+    from brawl_amd import workloads
+    v = workloads.random_v(S, [25.0 / k for k in range(1, Rs + 1)], 17 + S)
+    o = oracle.OracleLattice(lat, n, v)
+    counts = workloads.dirichlet_counts(o.n_atoms, S, np.random.default_rng(S + rank))
+    got = o.nested_sampling(oracle.Generator(rank=rank), n_walkers=nw, n_steps=steps, n_iter=iters,
+                            species_numbers=counts)
+    want = R.nested_sampling(lat, n, v, counts, rank=rank, n_walkers=nw, n_steps=steps,
+                             n_iter=iters)
+    assert np.array_equal(got, want)
