@@ -72,6 +72,11 @@ def test_null_arguments():
     assert L.bwl_create(None, None, None) == _lib.BWL_ERR_ARG
     assert L.bwl_destroy(None) == 0
     assert L.bwl_total_energy(None, None) == _lib.BWL_ERR_ARG
+    assert L.bwl_set_layout(None, 0) == _lib.BWL_ERR_ARG
+    assert L.bwl_layouts(None, None) == _lib.BWL_ERR_ARG
+    assert L.bwl_layout_info(None, 0, None, None) == _lib.BWL_ERR_ARG
+    assert L.bwl_wl_replica_exchange(None, None, None, None) == _lib.BWL_ERR_ARG
+    assert L.bwl_lds_bandwidth(0, None, None) == _lib.BWL_ERR_ARG
 
 
 def test_no_cpu_fallback_without_gpu():
