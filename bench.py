@@ -21,7 +21,7 @@ the same requests are then run one at a time through the plain entry points
 for the plain rate and to check that every served result equals its solo
 run.
 
-Prints ONE JSON line.  With --gpus N > 1 it exits: this build runs on one GPU.
+Prints ONE JSON line.  With --gpus N > 1 it exits: the library runs on one GPU.
 """
 from __future__ import annotations
 
@@ -415,7 +415,7 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "brawl_amd runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
 
     from brawl_amd import Lattice, workloads
