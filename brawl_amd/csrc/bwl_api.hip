@@ -160,6 +160,7 @@ int bwl_pair_histogram(bwl_ctx *c, int64_t *hist) {
   a.off = c->d_off;
   a.hist = (long long *)((char *)c->d_scratch + ((sizeof(HistArgs) + 63) & ~63));
   HIP_OK(hipMemcpyAsync(c->d_scratch, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemsetAsync(a.hist, 0, hbytes, c->stream));
   HIP_OK(launch_histogram((const HistArgs *)c->d_scratch, 1, g, threads, c->stream));
   HIP_OK(hipMemcpyAsync(hist, a.hist, hbytes, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));

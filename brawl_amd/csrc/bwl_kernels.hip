@@ -59,11 +59,14 @@ __global__ void k_unpack(const int8_t *__restrict__ compact, int8_t *__restrict_
 // ------------------------------------------------------------------
 // pair-count histogram: H[n][b][a] += number of shell-n neighbours of
 // species b around sites of species a (Fortran H(a,b,n) layout).
-// Per-wave private copies in LDS, summed in a fixed order at the end.
+// Block (x, y) stages lattice y whole in LDS and counts the x-th of
+// gridDim.x equal slices of its sites into per-wave private copies in LDS;
+// the block's sums are added to the (zeroed) output with 64-bit atomics.
+// Integer sums: the result does not depend on the slicing.
 // ------------------------------------------------------------------
 __global__ __launch_bounds__(512) void k_histogram(const HistArgs *__restrict__ args) {
   extern __shared__ __align__(16) uint8_t smem[];
-  const HistArgs &A = args[blockIdx.x];
+  const HistArgs &A = args[blockIdx.y];
   const Geo &g = A.g;
   int8_t *lat = (int8_t *)smem;
   const int hsize = g.S * g.S * g.R;
@@ -77,7 +80,10 @@ __global__ __launch_bounds__(512) void k_histogram(const HistArgs *__restrict__ 
   uint32_t *mine = hw + (threadIdx.x >> 6) * hsize;
   const int cells = g.n[0] * g.n[1] * g.n[2];
   const int nsites = g.nsub * cells;
-  for (int i = threadIdx.x; i < nsites; i += blockDim.x) {
+  const int per = (nsites + gridDim.x - 1) / gridDim.x;
+  const int i0 = blockIdx.x * per;
+  const int i1 = i0 + per < nsites ? i0 + per : nsites;
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     int s = i / cells;
     int r = i - s * cells;
     int cz = r / (g.n[0] * g.n[1]);
@@ -103,7 +109,7 @@ __global__ __launch_bounds__(512) void k_histogram(const HistArgs *__restrict__ 
   for (int i = threadIdx.x; i < hsize; i += blockDim.x) {
     unsigned long long t = 0;
     for (int w = 0; w < nw; ++w) t += hw[w * hsize + i];
-    A.hist[i] = (long long)t;
+    if (t) atomicAdd((unsigned long long *)&A.hist[i], t);
   }
 }
 
@@ -241,7 +247,11 @@ hipError_t launch_histogram(const HistArgs *args_dev, int n, const Geo &g_max, i
   hipError_t e = hipFuncSetAttribute((const void *)k_histogram,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_histogram, dim3(n), dim3(threads), lds, s, args_dev);
+  // slices of about 2048 sites (each block restages the lattice, from L2)
+  const int nsites = g_max.nsub * g_max.n[0] * g_max.n[1] * g_max.n[2];
+  int parts = nsites / 2048;
+  parts = parts < 1 ? 1 : (parts > 256 ? 256 : parts);
+  hipLaunchKernelGGL(k_histogram, dim3(parts, n), dim3(threads), lds, s, args_dev);
   return hipGetLastError();
 }
 
