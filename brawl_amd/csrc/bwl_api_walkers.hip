@@ -649,7 +649,8 @@ int bwl_wl_sweeps(bwl_wl *w, double wl_f, int32_t average, int64_t *accepted, do
   wl_fill_args(w, wl_f, args);
   HIP_OK(hipMemcpyAsync(w->d_args, args.data(), sizeof(WlArgs) * W, hipMemcpyHostToDevice, s));
   HIP_OK(hipEventRecord(w->ctx->ev0, s));
-  HIP_OK(launch_wl_sweeps(w->d_args, W, wl_lds_bytes(g, w->p.bins, w->hstride), s));
+  HIP_OK(launch_wl_sweeps(w->d_args, W, wl_lds_bytes(g, w->p.bins, w->hstride), s,
+                          pair_sum_key(g.lattice, g.R)));
   if (average)
     HIP_OK(launch_wl_average(w->d_lng, w->d_hist, w->p.num_windows, w->p.walkers_per_window,
                              w->p.bins, w->hstride, s));

@@ -44,6 +44,13 @@ struct Geo {
   int32_t shell_start[kMaxShells + 1];
 };
 
+// The straight-line variant of the sequential walkers' ordered pair-energy
+// sum for a lattice: lattice*16 + R for fcc and bcc up to 6 shells, 0 for
+// the mask-driven loop (bwl_seq.h).
+__host__ __device__ constexpr int pair_sum_key(int lattice, int R) {
+  return ((lattice == 1 || lattice == 2) && R >= 1 && R <= 6) ? lattice * 16 + R : 0;
+}
+
 // Parity bits of each sublattice.
 __host__ __device__ inline void sub_bits(int lattice, int s, int &bx, int &by, int &bz) {
   if (lattice == 1) {

@@ -193,7 +193,8 @@ hipError_t launch_ns(const NsArgs *args_dev, int n, size_t lds, int max_walkers,
                      hipStream_t s);
 size_t ns_energies_lds_bytes(const Geo &g);
 size_t wl_lds_bytes(const Geo &g, int bins, int window_bins);
-hipError_t launch_wl_sweeps(const WlArgs *args_dev, int n, size_t lds, hipStream_t s);
+// key: the pair-sum variant shared by every walker (pair_sum_key), -1 if mixed
+hipError_t launch_wl_sweeps(const WlArgs *args_dev, int n, size_t lds, hipStream_t s, int key);
 hipError_t launch_wl_average(double *lng, double *hist, int n_windows, int walkers_per_window,
                              int bins, int hist_stride, hipStream_t s);
 // Walker energies (ns_energies_lds_bytes of LDS), the exchange decisions

@@ -137,16 +137,17 @@ __device__ inline void init_pair_lanes(const Geo &g, PairLanes &pl) {
     pl.end_mask[k] = __ballot(t < 2 * z && last);
   }
   // straight-line sums for every fcc shell count and bcc up to 6 shells
-  const int key = g.lattice * 16 + g.R;
-  pl.fixed = (g.lattice == 1 && g.R <= 6) || (g.lattice == 2 && g.R <= 6) ? key : 0;
+  pl.fixed = pair_sum_key(g.lattice, g.R);
 }
 
 // The ordered sum of pair_energy_lanes from the lanes' values (entries
 // 0..z-1 site A's, z..2z-1 site B's): e_A + e_B, each site's shells left to
 // right, each shell from 0.0 in list order.
+__device__ inline double pair_sum_loop(const Geo &g, const PairLanes &pl,
+                                       const double (&vals)[6], double *eA_out);
+
 __device__ inline double pair_sum(const Geo &g, const PairLanes &pl, const double (&vals)[6],
                                   double *eA_out) {
-  const int z = g.z;
   switch (pl.fixed) {
     case 16 + 1: return fixed_pair_sum<1, 1>(vals, eA_out);
     case 16 + 2: return fixed_pair_sum<1, 2>(vals, eA_out);
@@ -162,6 +163,13 @@ __device__ inline double pair_sum(const Geo &g, const PairLanes &pl, const doubl
     case 32 + 6: return fixed_pair_sum<2, 6>(vals, eA_out);
     default: break;
   }
+  return pair_sum_loop(g, pl, vals, eA_out);
+}
+
+// The same sum for any shell structure, driven by the lanes' shell-end mask.
+__device__ inline double pair_sum_loop(const Geo &g, const PairLanes &pl,
+                                       const double (&vals)[6], double *eA_out) {
+  const int z = g.z;
   double eA = 0.0, e = 0.0, es = 0.0;
   bool first = true;
   int t = 0;
@@ -209,6 +217,22 @@ __device__ inline void pair_sum2(const Geo &g, const PairLanes &pl, const double
   e2 = pair_sum(g, pl, v2, nullptr);
 }
 
+// pair_sum2 for a variant fixed at compile time (KEY = pair_sum_key, which
+// must match pl.fixed), or chosen at run time (KEY < 0): a kernel compiled
+// for one variant holds only that sum's code.
+template <int KEY>
+__device__ inline void pair_sum2_t(const Geo &g, const PairLanes &pl, const double (&v1)[6],
+                                   const double (&v2)[6], double &e1, double &e2) {
+  if constexpr (KEY > 0) {
+    fixed_pair_sum2<KEY / 16, KEY % 16>(v1, v2, e1, e2);
+  } else if constexpr (KEY == 0) {
+    e1 = pair_sum_loop(g, pl, v1, nullptr);
+    e2 = pair_sum_loop(g, pl, v2, nullptr);
+  } else {
+    pair_sum2(g, pl, v1, v2, e1, e2);
+  }
+}
+
 // Energies of sites A and B in the reference's FP64 order (pair_energy,
 // bw_hamiltonian.f90:99-114): every lane gathers neighbour V entries (one
 // per 64-entry pass), then the entries are added in list order from 0.0
@@ -247,6 +271,7 @@ constexpr int8_t kMarkA = 0x41, kMarkB = 0x42;
 // neighbour entry that is A or B itself (marked) takes that site's species
 // before or after the swap, every other entry its neighbour's species, so
 // both sums are bit-identical to gathers before and after a real swap.
+template <int KEY = -1>
 __device__ inline void pair_energy_swap_lanes(const int8_t *lat, const Geo &g, const int32_t *off,
                                               const double *vex, const PairLanes &pl, int addrA,
                                               int subA, int addrB, int subB, int spA, int spB,
@@ -271,7 +296,7 @@ __device__ inline void pair_energy_swap_lanes(const int8_t *lat, const Geo &g, c
       va[k] = vex[(sa - 1) + S * ((nba - 1) + S * pl.shell[k])];
     }
   }
-  pair_sum2(g, pl, vb, va, e_before, e_after);
+  pair_sum2_t<KEY>(g, pl, vb, va, e_before, e_after);
 }
 
 struct SiteXYZ {

@@ -465,7 +465,11 @@ int launch_wl_group(bwl_server *s, std::vector<Request *> &grp, int slot) {
   HIP_OK(hipEventCreate(&L->ev0));
   HIP_OK(hipEventCreate(&L->ev1));
   HIP_OK(hipEventRecord(L->ev0, st));
-  HIP_OK(launch_wl_sweeps((const WlArgs *)L->scratch, (int)total_w, lds, st));
+  // one pair-sum variant for the launch when every walker shares it
+  int key = pair_sum_key(L->wl_args[0].g.lattice, L->wl_args[0].g.R);
+  for (const WlArgs &a : L->wl_args)
+    if (pair_sum_key(a.g.lattice, a.g.R) != key) key = -1;
+  HIP_OK(launch_wl_sweeps((const WlArgs *)L->scratch, (int)total_w, lds, st, key));
   for (size_t i = 0; i < grp.size(); ++i) {
     Request *r = grp[i];
     const WlArgs &a0 = L->wl_args[L->wl_first[i]];

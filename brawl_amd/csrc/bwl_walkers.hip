@@ -177,8 +177,12 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
 // ------------------------------------------------------------------
 // Wang-Landau `sweeps` (wang-landau.F90:539-632) for one walker, without the
 // window average (a separate kernel does that after every walker's call).
-// The walker's ln g and visit histogram are staged in LDS.
+// The walker's ln g and visit histogram are staged in LDS.  KEY is the
+// pair-sum variant of every walker of the launch (pair_sum_key), or -1 when
+// the launch mixes lattices: a kernel for one variant carries only that
+// sum's code, which leaves the registers to the rest of the trial.
 // ------------------------------------------------------------------
+template <int KEY>
 __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ args) {
   extern __shared__ __align__(16) uint8_t smem[];
   const WlArgs &A = args[blockIdx.x];
@@ -231,7 +235,7 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
       // energies before and after the swap from one gather (sites marked)
       double pu, ps;
       put_pair(lat, g, r1, sub1, kMarkA, r2, sub2, kMarkB);
-      pair_energy_swap_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, pu, ps);
+      pair_energy_swap_lanes<KEY>(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, pu, ps);
       e_sw = e_un - pu + ps;
     }
     bool acc = false;
@@ -637,12 +641,33 @@ size_t wl_lds_bytes(const Geo &g, int bins, int window_bins) {
          sizeof(double) * (size_t)(2 * bins + 1 + window_bins) + sizeof(int32_t) * g.nsub * g.z;
 }
 
-hipError_t launch_wl_sweeps(const WlArgs *args_dev, int n, size_t lds, hipStream_t s) {
-  hipError_t e = hipFuncSetAttribute((const void *)k_wl_sweeps,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+namespace {
+const void *wl_sweeps_kernel(int key) {
+  switch (key) {
+    case 0: return (const void *)k_wl_sweeps<0>;
+    case 16 + 1: return (const void *)k_wl_sweeps<16 + 1>;
+    case 16 + 2: return (const void *)k_wl_sweeps<16 + 2>;
+    case 16 + 3: return (const void *)k_wl_sweeps<16 + 3>;
+    case 16 + 4: return (const void *)k_wl_sweeps<16 + 4>;
+    case 16 + 5: return (const void *)k_wl_sweeps<16 + 5>;
+    case 16 + 6: return (const void *)k_wl_sweeps<16 + 6>;
+    case 32 + 1: return (const void *)k_wl_sweeps<32 + 1>;
+    case 32 + 2: return (const void *)k_wl_sweeps<32 + 2>;
+    case 32 + 3: return (const void *)k_wl_sweeps<32 + 3>;
+    case 32 + 4: return (const void *)k_wl_sweeps<32 + 4>;
+    case 32 + 5: return (const void *)k_wl_sweeps<32 + 5>;
+    case 32 + 6: return (const void *)k_wl_sweeps<32 + 6>;
+    default: return (const void *)k_wl_sweeps<-1>;
+  }
+}
+}  // namespace
+
+hipError_t launch_wl_sweeps(const WlArgs *args_dev, int n, size_t lds, hipStream_t s, int key) {
+  const void *fn = wl_sweeps_kernel(key);
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_wl_sweeps, dim3(n), dim3(64), lds, s, args_dev);
-  return hipGetLastError();
+  void *kargs[] = {(void *)&args_dev};
+  return hipLaunchKernel(fn, dim3(n), dim3(64), kargs, lds, s);
 }
 
 hipError_t launch_wl_average(double *lng, double *hist, int n_windows, int wpw, int bins,
