@@ -297,7 +297,11 @@ int bwl_nested_sampling_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_ns_para
     max_w = std::max(max_w, params[i].n_walkers);
     lds_e = std::max(lds_e, ns_energies_lds_bytes(ctxs[i]->g));
   }
-  HIP_OK(launch_ns((const NsArgs *)base, n, lds, max_w, lds_e, c0->stream));
+  // one pair-sum variant for the launch when every run shares it
+  int key = pair_sum_key(ctxs[0]->g.lattice, ctxs[0]->g.R);
+  for (int i = 1; i < n; ++i)
+    if (pair_sum_key(ctxs[i]->g.lattice, ctxs[i]->g.R) != key) key = -1;
+  HIP_OK(launch_ns((const NsArgs *)base, n, lds, max_w, lds_e, c0->stream, key));
   HIP_OK(hipEventRecord(c0->ev1, c0->stream));
   HIP_OK(hipStreamSynchronize(c0->stream));
   float ms = 0.f;
@@ -600,7 +604,7 @@ int bwl_wl_enter_windows(bwl_wl *w, int64_t max_trials, int64_t *trials, int32_t
     }
     HIP_OK(hipMemcpyAsync(w->d_eargs, args.data(), sizeof(WlEnterArgs) * act.size(),
                           hipMemcpyHostToDevice, s));
-    HIP_OK(launch_wl_enter(w->d_eargs, (int)act.size(), lds, s));
+    HIP_OK(launch_wl_enter(w->d_eargs, (int)act.size(), lds, s, pair_sum_key(g.lattice, g.R)));
     std::vector<int32_t> st((size_t)W);
     HIP_OK(hipMemcpyAsync(st.data(), w->d_status, 4 * W, hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(tr.data(), w->d_trials, 8 * W, hipMemcpyDeviceToHost, s));

@@ -189,8 +189,9 @@ hipError_t launch_metro_par_fn(const void *fn, const ParArgs *args_dev, int n_ch
 size_t ns_lds_bytes(const Geo &g, int n_walkers);
 // Initial walker energies (all walkers at once), then the runs (one
 // wavefront each).
+// key: the pair-sum variant shared by every run (pair_sum_key), -1 if mixed
 hipError_t launch_ns(const NsArgs *args_dev, int n, size_t lds, int max_walkers, size_t lds_energies,
-                     hipStream_t s);
+                     hipStream_t s, int key);
 size_t ns_energies_lds_bytes(const Geo &g);
 size_t wl_lds_bytes(const Geo &g, int bins, int window_bins);
 // key: the pair-sum variant shared by every walker (pair_sum_key), -1 if mixed
@@ -205,7 +206,7 @@ hipError_t launch_wl_exchange(const WlExchangeArgs &a, size_t lds, hipStream_t s
 hipError_t launch_wl_swap_configs(int8_t *walkers, long long bytes, const int32_t *swaps,
                                   const int32_t *n_swaps, int max_pairs, hipStream_t s);
 size_t wl_enter_lds_bytes(const Geo &g);
-hipError_t launch_wl_enter(const WlEnterArgs *args_dev, int n, size_t lds, hipStream_t s);
+hipError_t launch_wl_enter(const WlEnterArgs *args_dev, int n, size_t lds, hipStream_t s, int key);
 hipError_t launch_asro(const AsroArgs &a, hipStream_t s);
 
 }  // namespace bwl

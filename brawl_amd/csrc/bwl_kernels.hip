@@ -141,6 +141,7 @@ __global__ __launch_bounds__(64) void k_site_energy(SiteArgs A) {
 // Bit-exact sequential Metropolis (metropolis.F90:751-891), one wavefront.
 // All lanes run the same control flow; the MT19937 state is in LDS.
 // ------------------------------------------------------------------
+template <int KEY>
 __global__ __launch_bounds__(64) void k_metro_seq(SeqArgs A) {
   extern __shared__ __align__(16) uint8_t smem[];
   const Geo &g = A.g;
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(64) void k_metro_seq(SeqArgs A) {
       // energies before and after the swap from one gather (sites marked)
       double e_un, e_sw;
       put_pair(lat, g, s1, sub1, kMarkA, s2, sub2, kMarkB);
-      pair_energy_swap_lanes(lat, g, offl, vex, pl, addr1, sub1, addr2, sub2, sp1, sp2, e_un, e_sw);
+      pair_energy_swap_lanes<KEY>(lat, g, offl, vex, pl, addr1, sub1, addr2, sub2, sp1, sp2, e_un, e_sw);
       double de = e_sw - e_un;
       if (de < 0.0) {
         acc = true;
@@ -274,13 +275,18 @@ size_t seq_lds_bytes(const Geo &g) {
          sizeof(int32_t) * g.nsub * g.z + 64;
 }
 
+namespace {
+const void *metro_seq_kernel(int key) { BWL_KERNEL_FOR_PAIR_SUM_KEY(k_metro_seq, key) }
+}  // namespace
+
 hipError_t launch_metro_seq(const SeqArgs &a, hipStream_t s) {
   size_t lds = seq_lds_bytes(a.g);
-  hipError_t e = hipFuncSetAttribute((const void *)k_metro_seq,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const void *fn = metro_seq_kernel(pair_sum_key(a.g.lattice, a.g.R));
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_metro_seq, dim3(1), dim3(64), lds, s, a);
-  return hipGetLastError();
+  SeqArgs arg = a;
+  void *kargs[] = {(void *)&arg};
+  return hipLaunchKernel(fn, dim3(1), dim3(64), kargs, lds, s);
 }
 
 // ------------------------------------------------------------------

@@ -233,6 +233,26 @@ __device__ inline void pair_sum2_t(const Geo &g, const PairLanes &pl, const doub
   }
 }
 
+// The kernel of template family `kern` for pair-sum variant `key`
+// (pair_sum_key), the run-time-choice instance (-1) for any other key.
+#define BWL_KERNEL_FOR_PAIR_SUM_KEY(kern, key)            \
+  switch (key) {                                          \
+    case 0: return (const void *)kern<0>;                 \
+    case 16 + 1: return (const void *)kern<16 + 1>;       \
+    case 16 + 2: return (const void *)kern<16 + 2>;       \
+    case 16 + 3: return (const void *)kern<16 + 3>;       \
+    case 16 + 4: return (const void *)kern<16 + 4>;       \
+    case 16 + 5: return (const void *)kern<16 + 5>;       \
+    case 16 + 6: return (const void *)kern<16 + 6>;       \
+    case 32 + 1: return (const void *)kern<32 + 1>;       \
+    case 32 + 2: return (const void *)kern<32 + 2>;       \
+    case 32 + 3: return (const void *)kern<32 + 3>;       \
+    case 32 + 4: return (const void *)kern<32 + 4>;       \
+    case 32 + 5: return (const void *)kern<32 + 5>;       \
+    case 32 + 6: return (const void *)kern<32 + 6>;       \
+    default: return (const void *)kern<-1>;               \
+  }
+
 // Energies of sites A and B in the reference's FP64 order (pair_energy,
 // bw_hamiltonian.f90:99-114): every lane gathers neighbour V entries (one
 // per 64-entry pass), then the entries are added in list order from 0.0

@@ -386,7 +386,10 @@ int launch_ns_group(bwl_server *s, std::vector<Request *> &grp, int slot) {
     max_w = std::max(max_w, r->np.n_walkers);
     lds_e = std::max(lds_e, ns_energies_lds_bytes(r->g));
   }
-  HIP_OK(launch_ns((const NsArgs *)L->scratch, (int)grp.size(), lds, max_w, lds_e, st));
+  int key = pair_sum_key(grp[0]->g.lattice, grp[0]->g.R);
+  for (Request *r : grp)
+    if (pair_sum_key(r->g.lattice, r->g.R) != key) key = -1;
+  HIP_OK(launch_ns((const NsArgs *)L->scratch, (int)grp.size(), lds, max_w, lds_e, st, key));
   HIP_OK(hipEventRecord(L->ev1, st));
   s->slot_busy[(size_t)slot] = true;
   s->running.push_back(std::move(L));

@@ -59,6 +59,7 @@ __global__ __launch_bounds__(64) void k_ns_energies(const NsArgs *__restrict__ a
 // one run.  Walker lattices stay in HBM; the walker being walked is staged
 // in LDS.  Walker energies live in LDS.
 // ------------------------------------------------------------------
+template <int KEY>
 __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
   extern __shared__ __align__(16) uint8_t smem[];
   const NsArgs &A = args[blockIdx.x];
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
       // energies before and after the swap from one gather (sites marked)
       double eu, es;
       put_pair(lat, g, r1, sub1, kMarkA, r2, sub2, kMarkB);
-      pair_energy_swap_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, eu, es);
+      pair_energy_swap_lanes<KEY>(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, eu, es);
       const double de = es - eu;
       ++trials;
       const bool acc = ecur + de < limit;
@@ -463,6 +464,7 @@ __global__ void k_wl_swap_configs(int8_t *walkers, long long bytes, const int32_
 // ------------------------------------------------------------------
 // enter_energy_window (wang-landau.F90:643-741) for one walker.
 // ------------------------------------------------------------------
+template <int KEY>
 __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__ args) {
   extern __shared__ __align__(16) uint8_t smem[];
   const WlEnterArgs &A = args[blockIdx.x];
@@ -528,7 +530,7 @@ __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__
       // energies before and after the swap from one gather (sites marked)
       double pu, ps;
       put_pair(lat, g, r1, sub1, kMarkA, r2, sub2, kMarkB);
-      pair_energy_swap_lanes(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, pu, ps);
+      pair_energy_swap_lanes<KEY>(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, pu, ps);
       const double e_sw = e_un - pu + ps;
       const double de = ((e_sw - target) * (e_sw - target) - (e_un - target) * (e_un - target)) /
                         A.denom;
@@ -612,23 +614,30 @@ __global__ __launch_bounds__(256) void k_asro(AsroArgs A) {
 // ------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------
+namespace {
+const void *wl_sweeps_kernel(int key) { BWL_KERNEL_FOR_PAIR_SUM_KEY(k_wl_sweeps, key) }
+const void *ns_kernel(int key) { BWL_KERNEL_FOR_PAIR_SUM_KEY(k_ns, key) }
+const void *wl_enter_kernel(int key) { BWL_KERNEL_FOR_PAIR_SUM_KEY(k_wl_enter, key) }
+}  // namespace
+
 size_t ns_lds_bytes(const Geo &g, int n_walkers) {
   return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + sizeof(double) * g.S * g.S * g.R +
          sizeof(double) * (size_t)n_walkers + sizeof(int32_t) * g.nsub * g.z;
 }
 
 hipError_t launch_ns(const NsArgs *args_dev, int n, size_t lds, int max_walkers, size_t lds_energies,
-                     hipStream_t s) {
+                     hipStream_t s, int key) {
   hipError_t e = hipFuncSetAttribute((const void *)k_ns_energies,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_energies);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_ns_energies, dim3(max_walkers, n), dim3(64), lds_energies, s, args_dev);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void *)k_ns, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const void *fn = ns_kernel(key);
+  e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_ns, dim3(n), dim3(64), lds, s, args_dev);
-  return hipGetLastError();
+  void *kargs[] = {(void *)&args_dev};
+  return hipLaunchKernel(fn, dim3(n), dim3(64), kargs, lds, s);
 }
 
 size_t ns_energies_lds_bytes(const Geo &g) {
@@ -640,27 +649,6 @@ size_t wl_lds_bytes(const Geo &g, int bins, int window_bins) {
   return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + sizeof(double) * g.S * g.S * g.R +
          sizeof(double) * (size_t)(2 * bins + 1 + window_bins) + sizeof(int32_t) * g.nsub * g.z;
 }
-
-namespace {
-const void *wl_sweeps_kernel(int key) {
-  switch (key) {
-    case 0: return (const void *)k_wl_sweeps<0>;
-    case 16 + 1: return (const void *)k_wl_sweeps<16 + 1>;
-    case 16 + 2: return (const void *)k_wl_sweeps<16 + 2>;
-    case 16 + 3: return (const void *)k_wl_sweeps<16 + 3>;
-    case 16 + 4: return (const void *)k_wl_sweeps<16 + 4>;
-    case 16 + 5: return (const void *)k_wl_sweeps<16 + 5>;
-    case 16 + 6: return (const void *)k_wl_sweeps<16 + 6>;
-    case 32 + 1: return (const void *)k_wl_sweeps<32 + 1>;
-    case 32 + 2: return (const void *)k_wl_sweeps<32 + 2>;
-    case 32 + 3: return (const void *)k_wl_sweeps<32 + 3>;
-    case 32 + 4: return (const void *)k_wl_sweeps<32 + 4>;
-    case 32 + 5: return (const void *)k_wl_sweeps<32 + 5>;
-    case 32 + 6: return (const void *)k_wl_sweeps<32 + 6>;
-    default: return (const void *)k_wl_sweeps<-1>;
-  }
-}
-}  // namespace
 
 hipError_t launch_wl_sweeps(const WlArgs *args_dev, int n, size_t lds, hipStream_t s, int key) {
   const void *fn = wl_sweeps_kernel(key);
@@ -710,12 +698,12 @@ size_t wl_enter_lds_bytes(const Geo &g) {
          sizeof(int32_t) * g.nsub * g.z;
 }
 
-hipError_t launch_wl_enter(const WlEnterArgs *args_dev, int n, size_t lds, hipStream_t s) {
-  hipError_t e = hipFuncSetAttribute((const void *)k_wl_enter,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+hipError_t launch_wl_enter(const WlEnterArgs *args_dev, int n, size_t lds, hipStream_t s, int key) {
+  const void *fn = wl_enter_kernel(key);
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_wl_enter, dim3(n), dim3(64), lds, s, args_dev);
-  return hipGetLastError();
+  void *kargs[] = {(void *)&args_dev};
+  return hipLaunchKernel(fn, dim3(n), dim3(64), kargs, lds, s);
 }
 
 hipError_t launch_asro(const AsroArgs &a, hipStream_t s) {
