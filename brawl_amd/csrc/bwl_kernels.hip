@@ -150,6 +150,8 @@ __global__ __launch_bounds__(64) void k_metro_seq(SeqArgs A) {
   size_t o = (g.lds_bytes + 15) & ~15;
   uint32_t *st = (uint32_t *)(smem + o);
   o += 624 * 4;
+  double *sumbuf = (double *)(smem + o);
+  o += kSumBufBytes;
   double *vex = (double *)(smem + o);
   o += sizeof(double) * g.S * g.S * g.R;
   int32_t *offl = (int32_t *)(smem + o);
@@ -181,7 +183,7 @@ __global__ __launch_bounds__(64) void k_metro_seq(SeqArgs A) {
       // energies before and after the swap from one gather (sites marked)
       double e_un, e_sw;
       put_pair(lat, g, s1, sub1, kMarkA, s2, sub2, kMarkB);
-      pair_energy_swap_lanes<KEY>(lat, g, offl, vex, pl, addr1, sub1, addr2, sub2, sp1, sp2, e_un, e_sw);
+      pair_energy_swap_lanes<KEY>(lat, g, offl, vex, pl, addr1, sub1, addr2, sub2, sp1, sp2, sumbuf, e_un, e_sw);
       double de = e_sw - e_un;
       if (de < 0.0) {
         acc = true;
@@ -271,7 +273,7 @@ hipError_t launch_site_energy(const SiteArgs &a, hipStream_t s) {
 }
 
 size_t seq_lds_bytes(const Geo &g) {
-  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + sizeof(double) * g.S * g.S * g.R +
+  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + kSumBufBytes + sizeof(double) * g.S * g.S * g.R +
          sizeof(int32_t) * g.nsub * g.z + 64;
 }
 

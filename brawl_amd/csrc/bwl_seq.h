@@ -115,6 +115,56 @@ __device__ __forceinline__ void fixed_pair_sum2(const double (&v1)[6], const dou
   e2 = a2 + b2;
 }
 
+// Bytes of LDS for fixed_pair_sum2_lds: two sums of up to 3 passes of 64
+// entries (the straight-line variants need at most 172 entries per sum).
+constexpr int kSumBufBytes = 2 * 3 * 64 * 8;
+
+// fixed_pair_sum2 with the lanes' values passed through LDS (`buf`,
+// kSumBufBytes) instead of lane reads: every lane stores its entries, then
+// reads them back in list order, one 64-bit add per entry instead of two
+// lane reads and an add.  The same values are added in the same order, so
+// the results are bit-identical.  One wavefront per workgroup: its LDS
+// accesses complete in program order, so no barrier is needed.
+template <int LAT, int R>
+__device__ __forceinline__ void fixed_pair_sum2_lds(const double (&v1)[6], const double (&v2)[6],
+                                                    double *buf, double &e1, double &e2) {
+  constexpr int z = shell_first<LAT>(R);
+  constexpr int P = (2 * z + 63) / 64;
+  static_assert(2 * P * 64 * 8 <= kSumBufBytes, "sum buffer too small");
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    buf[k * 64 + lane] = v1[k];
+    buf[(P + k) * 64 + lane] = v2[k];
+  }
+  const double *b1 = buf, *b2 = buf + P * 64;
+  double a1 = 0.0, bb1 = 0.0, a2 = 0.0, bb2 = 0.0;
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int t = shell_first<LAT>(n); t < shell_first<LAT>(n + 1); ++t) {
+      s1 = s1 + b1[t];
+      s2 = s2 + b2[t];
+    }
+    a1 = n == 0 ? s1 : a1 + s1;
+    a2 = n == 0 ? s2 : a2 + s2;
+  }
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int t = z + shell_first<LAT>(n); t < z + shell_first<LAT>(n + 1); ++t) {
+      s1 = s1 + b1[t];
+      s2 = s2 + b2[t];
+    }
+    bb1 = n == 0 ? s1 : bb1 + s1;
+    bb2 = n == 0 ? s2 : bb2 + s2;
+  }
+  e1 = a1 + bb1;
+  e2 = a2 + bb2;
+}
+
 __device__ inline void init_pair_lanes(const Geo &g, PairLanes &pl) {
   const int lane = threadIdx.x & 63;
   const int z = g.z;
@@ -218,13 +268,14 @@ __device__ inline void pair_sum2(const Geo &g, const PairLanes &pl, const double
 }
 
 // pair_sum2 for a variant fixed at compile time (KEY = pair_sum_key, which
-// must match pl.fixed), or chosen at run time (KEY < 0): a kernel compiled
-// for one variant holds only that sum's code.
+// must match pl.fixed; the straight-line sums read the entries back from
+// `sumbuf` in LDS), or chosen at run time (KEY < 0): a kernel compiled for
+// one variant holds only that sum's code.
 template <int KEY>
 __device__ inline void pair_sum2_t(const Geo &g, const PairLanes &pl, const double (&v1)[6],
-                                   const double (&v2)[6], double &e1, double &e2) {
+                                   const double (&v2)[6], double *sumbuf, double &e1, double &e2) {
   if constexpr (KEY > 0) {
-    fixed_pair_sum2<KEY / 16, KEY % 16>(v1, v2, e1, e2);
+    fixed_pair_sum2_lds<KEY / 16, KEY % 16>(v1, v2, sumbuf, e1, e2);
   } else if constexpr (KEY == 0) {
     e1 = pair_sum_loop(g, pl, v1, nullptr);
     e2 = pair_sum_loop(g, pl, v2, nullptr);
@@ -295,7 +346,7 @@ template <int KEY = -1>
 __device__ inline void pair_energy_swap_lanes(const int8_t *lat, const Geo &g, const int32_t *off,
                                               const double *vex, const PairLanes &pl, int addrA,
                                               int subA, int addrB, int subB, int spA, int spB,
-                                              double &e_before, double &e_after) {
+                                              double *sumbuf, double &e_before, double &e_after) {
   const int lane = threadIdx.x & 63;
   const int z = g.z, S = g.S;
   double vb[6], va[6];
@@ -316,7 +367,7 @@ __device__ inline void pair_energy_swap_lanes(const int8_t *lat, const Geo &g, c
       va[k] = vex[(sa - 1) + S * ((nba - 1) + S * pl.shell[k])];
     }
   }
-  pair_sum2_t<KEY>(g, pl, vb, va, e_before, e_after);
+  pair_sum2_t<KEY>(g, pl, vb, va, sumbuf, e_before, e_after);
 }
 
 struct SiteXYZ {

@@ -69,6 +69,8 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
   size_t o = (g.lds_bytes + 15) & ~15;
   uint32_t *st = (uint32_t *)(smem + o);
   o += 624 * 4;
+  double *sumbuf = (double *)(smem + o);
+  o += kSumBufBytes;
   double *vex = (double *)(smem + o);
   o += sizeof(double) * g.S * g.S * g.R;
   double *E = (double *)(smem + o);
@@ -149,7 +151,7 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
       // energies before and after the swap from one gather (sites marked)
       double eu, es;
       put_pair(lat, g, r1, sub1, kMarkA, r2, sub2, kMarkB);
-      pair_energy_swap_lanes<KEY>(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, eu, es);
+      pair_energy_swap_lanes<KEY>(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, sumbuf, eu, es);
       const double de = es - eu;
       ++trials;
       const bool acc = ecur + de < limit;
@@ -194,6 +196,8 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
   size_t o = (g.lds_bytes + 15) & ~15;
   uint32_t *st = (uint32_t *)(smem + o);
   o += 624 * 4;
+  double *sumbuf = (double *)(smem + o);
+  o += kSumBufBytes;
   double *vex = (double *)(smem + o);
   o += sizeof(double) * g.S * g.S * g.R;
   double *lng = (double *)(smem + o);
@@ -236,7 +240,7 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
       // energies before and after the swap from one gather (sites marked)
       double pu, ps;
       put_pair(lat, g, r1, sub1, kMarkA, r2, sub2, kMarkB);
-      pair_energy_swap_lanes<KEY>(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, pu, ps);
+      pair_energy_swap_lanes<KEY>(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, sumbuf, pu, ps);
       e_sw = e_un - pu + ps;
     }
     bool acc = false;
@@ -474,6 +478,8 @@ __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__
   size_t o = (g.lds_bytes + 15) & ~15;
   uint32_t *st = (uint32_t *)(smem + o);
   o += 624 * 4;
+  double *sumbuf = (double *)(smem + o);
+  o += kSumBufBytes;
   double *vex = (double *)(smem + o);
   o += sizeof(double) * g.S * g.S * g.R;
   int32_t *offl = (int32_t *)(smem + o);
@@ -530,7 +536,7 @@ __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__
       // energies before and after the swap from one gather (sites marked)
       double pu, ps;
       put_pair(lat, g, r1, sub1, kMarkA, r2, sub2, kMarkB);
-      pair_energy_swap_lanes<KEY>(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, pu, ps);
+      pair_energy_swap_lanes<KEY>(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, sumbuf, pu, ps);
       const double e_sw = e_un - pu + ps;
       const double de = ((e_sw - target) * (e_sw - target) - (e_un - target) * (e_un - target)) /
                         A.denom;
@@ -621,7 +627,7 @@ const void *wl_enter_kernel(int key) { BWL_KERNEL_FOR_PAIR_SUM_KEY(k_wl_enter, k
 }  // namespace
 
 size_t ns_lds_bytes(const Geo &g, int n_walkers) {
-  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + sizeof(double) * g.S * g.S * g.R +
+  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + kSumBufBytes + sizeof(double) * g.S * g.S * g.R +
          sizeof(double) * (size_t)n_walkers + sizeof(int32_t) * g.nsub * g.z;
 }
 
@@ -646,7 +652,7 @@ size_t ns_energies_lds_bytes(const Geo &g) {
 }
 
 size_t wl_lds_bytes(const Geo &g, int bins, int window_bins) {
-  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + sizeof(double) * g.S * g.S * g.R +
+  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + kSumBufBytes + sizeof(double) * g.S * g.S * g.R +
          sizeof(double) * (size_t)(2 * bins + 1 + window_bins) + sizeof(int32_t) * g.nsub * g.z;
 }
 
@@ -694,7 +700,7 @@ hipError_t launch_wl_swap_configs(int8_t *walkers, long long bytes, const int32_
 }
 
 size_t wl_enter_lds_bytes(const Geo &g) {
-  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + sizeof(double) * g.S * g.S * g.R +
+  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + kSumBufBytes + sizeof(double) * g.S * g.S * g.R +
          sizeof(int32_t) * g.nsub * g.z;
 }
 
