@@ -390,6 +390,28 @@ def wang_landau_walkers(calls: int = 10):
                                  "exchanged": exchanged}}
 
 
+def sequential_bit_exact(sweeps: int = 100):
+    """Config 1 (fcc 8^3, binary, 2 shells, 600 K) through the sequential
+    mode that consumes the reference's MT19937 stream (bwl_mc_steps; its
+    trajectory is the reference's bit for bit), `sweeps` sweeps with the
+    energy recounted after each, as the reference samples it."""
+    from brawl_amd import Lattice, beta_of, workloads
+    c = workloads.config1()
+    with Lattice(c.lattice, c.n, c.v_ex) as L:
+        L.seed(110179)
+        L.initial_setup(species_numbers=c.counts)
+        N = L.n_atoms
+        L.mc_steps(beta_of(c.T), N, sample_every=N)          # warm-up sweep
+        t0 = time.perf_counter()
+        acc, _ = L.mc_steps(beta_of(c.T), sweeps * N, sample_every=N)
+        dt = time.perf_counter() - t0
+    log(f"[bench] sequential bit-exact (config 1): {sweeps * N} trials in {dt * 1e3:.1f} ms")
+    return {"workload": f"fcc 8^3 (2048 sites), S=2, 2 shells, 600 K, {sweeps} sweeps, energy "
+                        "recounted after every sweep; one wavefront, the reference's MT19937 stream",
+            "trials": sweeps * N, "value": sweeps * N / dt, "unit": "swap trials/s",
+            "us_per_trial": dt / (sweeps * N) * 1e6, "accepted": acc}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -504,6 +526,7 @@ def main():
         out["served"] = served(args)
     if not args.no_batched:
         out["wang_landau_walkers"] = wang_landau_walkers()
+        out["sequential_bit_exact"] = sequential_bit_exact()
     out["roofline_lds"] = lds_roofline(achieved, out.get("batched"))
     out["cpu_baseline"] = cpu
     print(json.dumps(out))
