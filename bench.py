@@ -406,10 +406,34 @@ def sequential_bit_exact(sweeps: int = 100):
         acc, _ = L.mc_steps(beta_of(c.T), sweeps * N, sample_every=N)
         dt = time.perf_counter() - t0
     log(f"[bench] sequential bit-exact (config 1): {sweeps * N} trials in {dt * 1e3:.1f} ms")
+    # the reference's replica ensemble, bit for bit: one chain per rank
+    # (static seed 110179 + 11*rank), all ranks in one launch
+    from brawl_amd.lattice import mc_steps_batch
+    reps, rsweeps = 256, 20
+    lats = []
+    for k in range(reps):
+        Lk = Lattice(c.lattice, c.n, c.v_ex)
+        Lk.seed(110179 + 11 * k)
+        Lk.initial_setup(species_numbers=c.counts)
+        lats.append(Lk)
+    betas = [beta_of(c.T)] * reps
+    mc_steps_batch(lats, betas, N, sample_every=N)              # warm-up sweep
+    t0 = time.perf_counter()
+    racc, _, rms = mc_steps_batch(lats, betas, rsweeps * N, sample_every=N)
+    rdt = time.perf_counter() - t0
+    for Lk in lats:
+        Lk.close()
+    rtrials = reps * rsweeps * N
+    log(f"[bench] sequential bit-exact replicas: {rtrials} trials in {rdt * 1e3:.1f} ms "
+        f"({rms:.1f} ms device)")
     return {"workload": f"fcc 8^3 (2048 sites), S=2, 2 shells, 600 K, {sweeps} sweeps, energy "
                         "recounted after every sweep; one wavefront, the reference's MT19937 stream",
             "trials": sweeps * N, "value": sweeps * N / dt, "unit": "swap trials/s",
-            "us_per_trial": dt / (sweeps * N) * 1e6, "accepted": acc}
+            "us_per_trial": dt / (sweeps * N) * 1e6, "accepted": acc,
+            "replicas": {"chains": reps, "sweeps": rsweeps, "trials": rtrials,
+                         "value": rtrials / rdt, "device_rate": rtrials / (rms / 1e3),
+                         "unit": "swap trials/s",
+                         "note": "one chain per rank, each the reference's trajectory bit for bit"}}
 
 
 def main():

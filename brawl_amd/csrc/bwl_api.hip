@@ -313,6 +313,82 @@ int bwl_mc_steps(bwl_ctx *c, double beta, int32_t nbr_swap, int64_t n_trials, in
   return BWL_OK;
 }
 
+int bwl_mc_steps_batch(int32_t n, bwl_ctx *const *ctxs, const double *beta, int32_t nbr_swap,
+                       int64_t n_trials, int64_t sample_every, double *const *energies,
+                       int64_t *accepted, double *kernel_ms) {
+  if (n < 1 || !ctxs || !beta) return fail(BWL_ERR_ARG, "null argument or empty batch");
+  if (n_trials < 0) return fail(BWL_ERR_ARG, "negative trial count");
+  if (energies && sample_every <= 0) return fail(BWL_ERR_ARG, "sample_every must be positive");
+  size_t lds = 0;
+  int key = 0;
+  for (int i = 0; i < n; ++i) {
+    bwl_ctx *c = ctxs[i];
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if (c->device != ctxs[0]->device) return fail(BWL_ERR_ARG, "all chains of a batch must be on one device");
+    if (2 * c->g.R > 64) return fail(BWL_ERR_UNSUPPORTED, "too many shells");
+    lds = std::max(lds, seq_lds_bytes(c->g));
+    const int k = pair_sum_key(c->g.lattice, c->g.R);
+    key = i == 0 ? k : (k == key ? key : -1);
+  }
+  if (lds > (size_t)lds_limit()) return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
+  HIP_OK(hipSetDevice(ctxs[0]->device));
+  // per chain: MT state, index, accepted count, sampled energies
+  const long long n_samples = energies ? n_trials / sample_every : 0;
+  const size_t stride = ((624 * 4 + 64 + sizeof(double) * (size_t)n_samples) + 255) & ~(size_t)255;
+  const size_t args_bytes = (sizeof(SeqArgs) * (size_t)n + 255) & ~(size_t)255;
+  DevBuf buf;
+  if (hipMalloc(&buf.p, args_bytes + stride * (size_t)n) != hipSuccess)
+    return fail(BWL_ERR_NOMEM, "device allocation failed");
+  char *base = (char *)buf.p;
+  std::vector<SeqArgs> args((size_t)n);
+  std::vector<char> host(stride * (size_t)n, 0);
+  for (int i = 0; i < n; ++i) {
+    bwl_ctx *c = ctxs[i];
+    if (c->mt.mti == 625) c->mt.seed(5489u);
+    HIP_OK(hipStreamSynchronize(c->stream));  // the chain's own earlier work
+    char *h = host.data() + stride * (size_t)i;
+    memcpy(h, c->mt.mt, 624 * 4);
+    memcpy(h + 624 * 4, &c->mt.mti, 4);
+    char *d = base + args_bytes + stride * (size_t)i;
+    SeqArgs &a = args[(size_t)i];
+    a.g = c->g;
+    a.compact = c->d_compact;
+    a.off = c->d_off;
+    a.vex = c->d_vex;
+    a.mt = (uint32_t *)d;
+    a.mt_index = (int32_t *)(d + 624 * 4);
+    a.accepted = (long long *)(d + 624 * 4 + 16);
+    a.energies = n_samples ? (double *)(d + 624 * 4 + 64) : nullptr;
+    a.beta = beta[i];
+    a.nbr_swap = nbr_swap ? 1 : 0;
+    a.n_trials = n_trials;
+    a.sample_every = energies ? sample_every : 0;
+  }
+  hipStream_t s = ctxs[0]->stream;
+  HIP_OK(hipMemcpyAsync(base, args.data(), sizeof(SeqArgs) * (size_t)n, hipMemcpyHostToDevice, s));
+  HIP_OK(hipMemcpyAsync(base + args_bytes, host.data(), host.size(), hipMemcpyHostToDevice, s));
+  HIP_OK(hipEventRecord(ctxs[0]->ev0, s));
+  HIP_OK(launch_metro_seq_batch((const SeqArgs *)base, n, lds, s, key));
+  HIP_OK(hipEventRecord(ctxs[0]->ev1, s));
+  HIP_OK(hipMemcpyAsync(host.data(), base + args_bytes, host.size(), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, ctxs[0]->ev0, ctxs[0]->ev1);
+  if (kernel_ms) *kernel_ms = ms;
+  for (int i = 0; i < n; ++i) {
+    bwl_ctx *c = ctxs[i];
+    const char *h = host.data() + stride * (size_t)i;
+    memcpy(c->mt.mt, h, 624 * 4);
+    memcpy(&c->mt.mti, h + 624 * 4, 4);
+    long long acc = 0;
+    memcpy(&acc, h + 624 * 4 + 16, sizeof(acc));
+    if (accepted) accepted[i] = acc;
+    if (n_samples && energies[i]) memcpy(energies[i], h + 624 * 4 + 64, sizeof(double) * n_samples);
+  }
+  return BWL_OK;
+}
+
 int bwl_set_layout(bwl_ctx *c, int32_t layout) {
   if (!c) return fail(BWL_ERR_ARG, "null context");
   if (layout < BWL_LAYOUT_AUTO || layout > BWL_LAYOUT_FIXED) return fail(BWL_ERR_ARG, "unknown layout");

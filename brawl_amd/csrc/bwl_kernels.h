@@ -166,6 +166,9 @@ size_t site_lds_bytes(const Geo &g);
 hipError_t launch_site_energy(const SiteArgs &a, hipStream_t s);
 size_t seq_lds_bytes(const Geo &g);
 hipError_t launch_metro_seq(const SeqArgs &a, hipStream_t s);
+// n chains (args_dev[0..n)), one wavefront each; key as for launch_wl_sweeps
+hipError_t launch_metro_seq_batch(const SeqArgs *args_dev, int n, size_t lds, hipStream_t s,
+                                  int key);
 size_t par_lds_bytes(const Geo &g, const NatGeo *nat, int threads);
 // Block size limit of the parallel-sampler kernels: the natural-layout
 // kernels with compile-time neighbour offsets and 32-bit count words run 16

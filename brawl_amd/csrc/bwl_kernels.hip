@@ -142,8 +142,7 @@ __global__ __launch_bounds__(64) void k_site_energy(SiteArgs A) {
 // All lanes run the same control flow; the MT19937 state is in LDS.
 // ------------------------------------------------------------------
 template <int KEY>
-__global__ __launch_bounds__(64) void k_metro_seq(SeqArgs A) {
-  extern __shared__ __align__(16) uint8_t smem[];
+__device__ __forceinline__ void metro_seq_run(const SeqArgs &A, uint8_t *smem) {
   const Geo &g = A.g;
   const int lane = threadIdx.x;
   int8_t *lat = (int8_t *)smem;
@@ -272,6 +271,21 @@ hipError_t launch_site_energy(const SiteArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// One chain (the context's), its arguments in the kernel arguments.
+template <int KEY>
+__global__ __launch_bounds__(64) void k_metro_seq(SeqArgs A) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  metro_seq_run<KEY>(A, smem);
+}
+
+// Many independent chains, one wavefront each (block b runs args[b]): the
+// reference's replica ensemble, every chain on its own MT19937 stream.
+template <int KEY>
+__global__ __launch_bounds__(64) void k_metro_seq_batch(const SeqArgs *__restrict__ args) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  metro_seq_run<KEY>(args[blockIdx.x], smem);
+}
+
 size_t seq_lds_bytes(const Geo &g) {
   return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + kSumBufBytes + sizeof(double) * g.S * g.S * g.R +
          sizeof(int32_t) * g.nsub * g.z + 64;
@@ -279,6 +293,7 @@ size_t seq_lds_bytes(const Geo &g) {
 
 namespace {
 const void *metro_seq_kernel(int key) { BWL_KERNEL_FOR_PAIR_SUM_KEY(k_metro_seq, key) }
+const void *metro_seq_batch_kernel(int key) { BWL_KERNEL_FOR_PAIR_SUM_KEY(k_metro_seq_batch, key) }
 }  // namespace
 
 hipError_t launch_metro_seq(const SeqArgs &a, hipStream_t s) {
@@ -289,6 +304,15 @@ hipError_t launch_metro_seq(const SeqArgs &a, hipStream_t s) {
   SeqArgs arg = a;
   void *kargs[] = {(void *)&arg};
   return hipLaunchKernel(fn, dim3(1), dim3(64), kargs, lds, s);
+}
+
+hipError_t launch_metro_seq_batch(const SeqArgs *args_dev, int n, size_t lds, hipStream_t s,
+                                  int key) {
+  const void *fn = metro_seq_batch_kernel(key);
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  void *kargs[] = {(void *)&args_dev};
+  return hipLaunchKernel(fn, dim3(n), dim3(64), kargs, lds, s);
 }
 
 // ------------------------------------------------------------------

@@ -318,6 +318,21 @@ module brawl_hip
       real(c_double), intent(out) :: kernel_ms
     end function
 
+    ! batched monte_carlo steps: one chain per context, one launch
+    integer(c_int) function bwl_mc_steps_batch(n, ctxs, beta, nbr_swap, n_trials, sample_every, &
+                                               energies, accepted, kernel_ms) &
+        bind(C, name="bwl_mc_steps_batch")
+      import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t
+      integer(c_int32_t), value :: n
+      type(c_ptr), intent(in) :: ctxs(*)
+      real(c_double), intent(in) :: beta(*)
+      integer(c_int32_t), value :: nbr_swap
+      integer(c_int64_t), value :: n_trials, sample_every
+      type(c_ptr), intent(in) :: energies(*)      ! double* per chain (may be null)
+      integer(c_int64_t), intent(out) :: accepted(*)
+      real(c_double), intent(out) :: kernel_ms
+    end function
+
     ! the parallel sampler's lattice layout in LDS (no reference counterpart)
     integer(c_int) function bwl_set_layout(ctx, layout) bind(C, name="bwl_set_layout")
       import :: c_int, c_ptr, c_int32_t

@@ -240,6 +240,29 @@ class Lattice:
         return AnnealResult(e, c, a, int(st.trials), int(st.accepted), float(st.kernel_ms), rho)
 
 
+def mc_steps_batch(lattices, betas, n_trials: int, *, nbr_swap=False, sample_every=0):
+    """Batched mc_steps: every lattice makes n_trials reference trials on its
+    own MT19937 stream, all in one launch.  Returns (accepted per lattice,
+    energies per lattice or None, device ms)."""
+    n = len(lattices)
+    if n == 0 or len(betas) != n:
+        raise ValueError("one beta per lattice")
+    L = _lib.load()
+    H = (C.c_void_p * n)(*[lat._h for lat in lattices])
+    b = np.ascontiguousarray(betas, dtype=np.float64)
+    acc = np.zeros(n, dtype=np.int64)
+    outs = None
+    DP = C.POINTER(C.c_double)
+    E = None
+    if sample_every:
+        outs = [np.zeros(n_trials // sample_every) for _ in range(n)]
+        E = (DP * n)(*[_p(o, C.c_double) for o in outs])
+    ms = C.c_double()
+    check(L.bwl_mc_steps_batch(n, H, _p(b, C.c_double), int(bool(nbr_swap)), int(n_trials),
+                               int(sample_every), E, _p(acc, C.c_int64), C.byref(ms)))
+    return acc, outs, float(ms.value)
+
+
 def anneal_batch(lattices, params):
     """Batched form of metropolis_simulated_annealing: one workgroup per
     chain, all chains in as few launches as possible.  `params` is a list of
@@ -270,5 +293,6 @@ def anneal_batch(lattices, params):
     return res, float(mk.value)
 
 
-__all__ = ["Lattice", "BrawlError", "AnnealResult", "anneal_batch", "beta_of", "warren_cowley",
+__all__ = ["Lattice", "BrawlError", "AnnealResult", "anneal_batch", "mc_steps_batch", "beta_of",
+           "warren_cowley",
            "FCC", "BCC", "K_B_IN_RY", "RY_TO_EV"]

@@ -113,6 +113,16 @@ int bwl_initial_setup(bwl_ctx *ctx, const int64_t *species_numbers,
  * (n_trials / sample_every values). */
 int bwl_mc_steps(bwl_ctx *ctx, double beta, int32_t nbr_swap, int64_t n_trials,
                  int64_t sample_every, double *energies, int64_t *accepted);
+/* Batched form of bwl_mc_steps: n contexts on one device, each making
+ * n_trials sequential trials at beta[i] on its own MT19937 stream, in one
+ * launch (one wavefront per context) -- the reference's MPI replica
+ * ensemble (comms.F90:122-160, one chain per rank), every chain
+ * bit-identical to the same context's bwl_mc_steps.  accepted (n values),
+ * energies (optional: n pointers of n_trials/sample_every values, NULL
+ * entries skipped) and *kernel_ms (optional, device time) are outputs. */
+int bwl_mc_steps_batch(int32_t n, bwl_ctx *const *ctxs, const double *beta, int32_t nbr_swap,
+                       int64_t n_trials, int64_t sample_every, double *const *energies,
+                       int64_t *accepted, double *kernel_ms);
 
 /* ---------------------------------------------------------------- */
 /* Parallel Metropolis sampler (statistical parity)                  */

@@ -449,3 +449,46 @@ def test_autotuned_batch_matches_single_runs(hip, oracle):
         assert abs(ref.total_energy(f) - r.energies_of_T[0] * ref.n_atoms) <= 1e-11 * ref.n_atoms
     for L in lats:
         L.close()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_sequential_batch_matches_single_runs(hip, oracle, mixed):
+    """The batched sequential mode (one wavefront per chain, the reference's
+    replica ensemble) gives every chain exactly what bwl_mc_steps gives it
+    alone: configuration, accepted count, sampled energies and stream; a
+    batch of config-1 replicas (one compiled sum variant) and a batch mixing
+    lattices and shell counts (the run-time variant)."""
+    from brawl_amd import lattice as lt, workloads
+    if mixed:
+        shapes = [(1, (8, 8, 8), 2, 2), (2, (4, 4, 4), 3, 3), (1, (3, 3, 3), 4, 4), (2, (6, 6, 6), 2, 1),
+                  (2, (5, 4, 3), 5, 7)]
+    else:
+        shapes = [(1, (8, 8, 8), 2, 2)] * 6
+    lats, starts, states, betas = [], [], [], []
+    for k, (lat, n, S, R) in enumerate(shapes):
+        v = workloads.random_v(S, [20.0 / j for j in range(1, R + 1)], 3 + k if mixed else 3)
+        L = hip.Lattice(lat, n, v)
+        L.seed(110179 + 11 * k)                  # rank k's static seed
+        cc = np.full(S, 1.0 / S)
+        L.initial_setup(concentrations=cc)
+        lats.append(L)
+        starts.append(L.get_config())
+        states.append(L.genrand_state())
+        betas.append(lt.beta_of(400.0 + 150.0 * k))
+    N = max(L.n_atoms for L in lats)
+    trials = 3 * N
+    acc, energies, ms = lt.mc_steps_batch(lats, betas, trials, sample_every=N)
+    _say("sequential batch", "mixed" if mixed else "replicas", "ms", ms, "accepted", acc)
+    for L, s0, (mt, mti), b, a, e in zip(lats, starts, states, betas, acc, energies):
+        with hip.Lattice(L.lattice, L.n, L.v_ex) as L2:
+            L2.set_config(s0)
+            L2.set_genrand_state(mt, mti)
+            a2, e2 = L2.mc_steps(b, trials, sample_every=N)
+            assert a2 == a
+            assert np.array_equal(e2, e)
+            assert np.array_equal(L2.get_config(), L.get_config())
+            m1, i1 = L.genrand_state()
+            m2, i2 = L2.genrand_state()
+            assert i1 == i2 and np.array_equal(m1, m2)
+    for L in lats:
+        L.close()
