@@ -77,6 +77,7 @@ def test_null_arguments():
     assert L.bwl_layout_info(None, 0, None, None) == _lib.BWL_ERR_ARG
     assert L.bwl_wl_replica_exchange(None, None, None, None) == _lib.BWL_ERR_ARG
     assert L.bwl_lds_bandwidth(0, None, None) == _lib.BWL_ERR_ARG
+    assert L.bwl_mc_steps_batch(0, None, None, 0, 1, 0, None, None, None) == _lib.BWL_ERR_ARG
 
 
 def test_no_cpu_fallback_without_gpu():
