@@ -390,6 +390,38 @@ def wang_landau_walkers(calls: int = 10):
                                  "exchanged": exchanged}}
 
 
+def wang_landau_run(tolerance: float):
+    """Config 4 end to end through WangLandau.run: every walker walked into
+    its window, pre-sampling, then sweeps calls of one sweep each with a
+    replica exchange after each, halving ln f from 1 at flatness 0.8 until
+    it falls below `tolerance`."""
+    from brawl_amd import wang_landau, workloads
+    c = workloads.config4()
+    wl = wang_landau.WangLandau(c["lattice"], c["n"], c["v_ex"], bins=c["bins"],
+                                num_windows=c["num_windows"], walkers_per_window=c["walkers_per_window"],
+                                bin_overlap=c["bin_overlap"], energy_min=c["energy_min"],
+                                energy_max=c["energy_max"], mc_sweeps=1)
+    wl.initial_setup(species_numbers=c["counts"])
+    t0 = time.perf_counter()
+    tr, entered = wl.enter_windows(10**8)
+    t1 = time.perf_counter()
+    log(f"[bench] Wang-Landau run: window entry {t1 - t0:.1f} s, {int(entered.sum())} walkers in")
+    res = wl.run(wl_f=c["wl_f"], tolerance=tolerance, flatness=c["flatness"], max_sweeps_calls=50000,
+                 max_pre_sampling_calls=50000, enter_max_trials=10**8)
+    t2 = time.perf_counter()
+    wl.close()
+    log(f"[bench] Wang-Landau run: {res.iterations} halvings, {res.sweeps_calls} sweeps calls, "
+        f"{t2 - t1:.1f} s")
+    return {"workload": "16 windows x 8 walkers, bcc 16^3 (8192 sites), S=4, 2 shells, 800 bins "
+                        "over [-40, 10] meV/atom, ln f from 1, flatness 0.8",
+            "tolerance": tolerance, "entry_s": t1 - t0, "entry_max_trials": int(tr.max()),
+            "run_s": t2 - t1, "halvings": res.iterations, "sweeps_calls": res.sweeps_calls,
+            "pre_sampling_calls": res.pre_sampling_calls, "trials": res.trials,
+            "device_ms": res.kernel_ms, "value": res.trials / (t2 - t1), "unit": "swap trials/s",
+            "exchanges": res.exchanges, "exchange_pairs": res.exchange_pairs,
+            "lng_span": float(res.lng.max() - res.lng.min())}
+
+
 def sequential_bit_exact(sweeps: int = 100):
     """Config 1 (fcc 8^3, binary, 2 shells, 600 K) through the sequential
     mode that consumes the reference's MT19937 stream (bwl_mc_steps; its
@@ -446,6 +478,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--wl-run", type=float, default=None, metavar="TOL",
+                    help="only: config 4's whole Wang-Landau run (window entry, pre-sampling, "
+                         "sweeps with replica exchanges) down to ln f < TOL")
     ap.add_argument("--only-served", action="store_true",
                     help="run only the served workload (for profiling it)")
     ap.add_argument("--served-requests", type=int, default=240)
@@ -469,6 +504,9 @@ def main():
 
     if args.only_served:
         print(json.dumps({"served": served(args)}))
+        return
+    if args.wl_run is not None:
+        print(json.dumps({"wang_landau_run": wang_landau_run(args.wl_run)}))
         return
     chain = workloads.config2()
     log(f"[bench] workload {chain.name}: N={chain.n_atoms} S={chain.S} R={chain.R}")
