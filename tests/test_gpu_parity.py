@@ -216,7 +216,7 @@ def test_parallel_sampler_conserves_composition_and_energy(hip, oracle, shape):
 
 def test_parallel_sampler_statistics_match_reference_sampler(hip, oracle):
     """The small config at 600 K: <E>/N, C and the Warren-Cowley alpha^(1)
-    of the unlike pair from the parallel sampler agree with the reference's
+    and alpha^(2) of the unlike pair from the parallel sampler agree with the reference's
     sequential sampler (the oracle) within 2 sigma of the difference, both
     equilibrated from the same kind of start; radial densities are sampled
     every sweep by both."""
@@ -227,7 +227,7 @@ def test_parallel_sampler_statistics_match_reference_sampler(hip, oracle):
     conc = c.counts / c.counts.sum()
     z = (12, 6)                         # fcc shells 1 and 2
     # reference sampler (oracle): 6 chains x (200 burn-in + 300 sampled sweeps)
-    e_ref, c_ref, a_ref = [], [], []
+    e_ref, c_ref, a_ref, a2_ref = [], [], [], []
     for k in range(6):
         gen = oracle.Generator(110179 + 11 * k)
         cfg = ref.initial_setup(gen, species_numbers=c.counts)
@@ -237,7 +237,8 @@ def test_parallel_sampler_statistics_match_reference_sampler(hip, oracle):
         e_ref.append(r["energies"][0])
         c_ref.append(r["C"][0])
         a_ref.append(warren_cowley(r["rho"][:, :, :, 0], z, conc)[0, 1, 1])
-    e_gpu, c_gpu, a_gpu = [], [], []
+        a2_ref.append(warren_cowley(r["rho"][:, :, :, 0], z, conc)[0, 1, 2])
+    e_gpu, c_gpu, a_gpu, a2_gpu = [], [], [], []
     for k in range(8):
         cfg = ref.initial_setup(oracle.Generator(999 + k), species_numbers=c.counts)
         with hip.Lattice(c.lattice, c.n, c.v_ex) as L:
@@ -247,7 +248,9 @@ def test_parallel_sampler_statistics_match_reference_sampler(hip, oracle):
             e_gpu.append(r.energies_of_T[0])
             c_gpu.append(r.C_of_T[0])
             a_gpu.append(warren_cowley(r.rho_of_T[:, :, :, 0], z, conc)[0, 1, 1])
-    for name, g_, r_ in (("<E>/N", e_gpu, e_ref), ("C", c_gpu, c_ref), ("alpha1_12", a_gpu, a_ref)):
+            a2_gpu.append(warren_cowley(r.rho_of_T[:, :, :, 0], z, conc)[0, 1, 2])
+    for name, g_, r_ in (("<E>/N", e_gpu, e_ref), ("C", c_gpu, c_ref), ("alpha1_12", a_gpu, a_ref),
+                         ("alpha2_12", a2_gpu, a2_ref)):
         g_, r_ = np.array(g_), np.array(r_)
         diff = g_.mean() - r_.mean()
         sigma = np.sqrt(g_.var(ddof=1) / len(g_) + r_.var(ddof=1) / len(r_))
