@@ -495,3 +495,47 @@ def test_sequential_batch_matches_single_runs(hip, oracle, mixed):
             assert i1 == i2 and np.array_equal(m1, m2)
     for L in lats:
         L.close()
+
+
+def test_parallel_sampler_statistics_quinary_six_bit_fields(hip, oracle):
+    """A quinary fcc 5^3 lattice with 3 shells at 1200 K, whose parallel
+    kernel packs its counts in 6-bit fields: <E>/N, C and alpha^(1) of one
+    unlike pair agree with the reference's sequential sampler (the oracle)
+    within 2 sigma of the difference."""
+    from brawl_amd import workloads
+    from brawl_amd.lattice import warren_cowley
+    lat, n, S, T = 1, (5, 5, 5), 5, 1200.0
+    v = workloads.random_v(S, [25.0, 12.5, 25.0 / 3], 77)
+    ref = oracle.OracleLattice(lat, n, v)
+    N = ref.n_atoms
+    counts = workloads.equal_counts(N, S)
+    conc = counts / counts.sum()
+    z = (12, 6, 24)
+    with hip.Lattice(lat, n, v) as L0:
+        assert L0.layout_info()[0] > 0 and L0.layouts()[0] == "fixed"
+    e_ref, c_ref, a_ref = [], [], []
+    for k in range(6):
+        gen = oracle.Generator(110179 + 11 * k)
+        cfg = ref.initial_setup(gen, species_numbers=counts)
+        r = ref.anneal(cfg, gen, T=T, n_mc_steps=300 * N, n_sample_steps=N, burn_in_start=True,
+                       n_burn_in_steps=200 * N, calculate_asro=True, n_sample_steps_asro=N,
+                       wc_range=4)
+        e_ref.append(r["energies"][0])
+        c_ref.append(r["C"][0])
+        a_ref.append(warren_cowley(r["rho"][:, :, :, 0], z, conc)[0, 1, 1])
+    e_gpu, c_gpu, a_gpu = [], [], []
+    for k in range(8):
+        cfg = ref.initial_setup(oracle.Generator(777 + k), species_numbers=counts)
+        with hip.Lattice(lat, n, v) as L:
+            L.set_config(cfg)
+            r = L.metropolis_simulated_annealing(T=T, n_sweeps=300, burn_in_sweeps=200, seed=k + 5,
+                                                 asro_every=1)
+            e_gpu.append(r.energies_of_T[0])
+            c_gpu.append(r.C_of_T[0])
+            a_gpu.append(warren_cowley(r.rho_of_T[:, :, :, 0], z, conc)[0, 1, 1])
+    for name, g_, r_ in (("<E>/N", e_gpu, e_ref), ("C", c_gpu, c_ref), ("alpha1_12", a_gpu, a_ref)):
+        g_, r_ = np.array(g_), np.array(r_)
+        diff = g_.mean() - r_.mean()
+        sigma = np.sqrt(g_.var(ddof=1) / len(g_) + r_.var(ddof=1) / len(r_))
+        _say("quinary", name, "gpu", g_.mean(), "ref", r_.mean(), "diff/sigma", diff / sigma)
+        assert abs(diff) <= 2.0 * sigma + 1e-9, name
