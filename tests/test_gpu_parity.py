@@ -539,3 +539,61 @@ def test_parallel_sampler_statistics_quinary_six_bit_fields(hip, oracle):
         sigma = np.sqrt(g_.var(ddof=1) / len(g_) + r_.var(ddof=1) / len(r_))
         _say("quinary", name, "gpu", g_.mean(), "ref", r_.mean(), "diff/sigma", diff / sigma)
         assert abs(diff) <= 2.0 * sigma + 1e-9, name
+
+
+@pytest.mark.parametrize("lat,n,S,R,T,layout", [
+    (2, (4, 5, 6), 3, 2, 900.0, "compact"),   # not cubic: the per-sublattice kernel
+    (1, (4, 4, 4), 6, 2, 1500.0, None),       # 6 species: 64-bit count words
+])
+def test_parallel_sampler_statistics_other_kernels(hip, oracle, lat, n, S, R, T, layout):
+    """The per-sublattice kernel (a box that is not cubic) and the 64-bit
+    count words (6 species): <E>/N, C and alpha^(1) of one unlike pair
+    agree with the reference's sequential sampler (the oracle) within 2
+    sigma of the difference."""
+    from brawl_amd import workloads
+    from brawl_amd.lattice import warren_cowley
+    v = workloads.random_v(S, [25.0 / k for k in range(1, R + 1)], 91 + S)
+    ref = oracle.OracleLattice(lat, n, v)
+    N = ref.n_atoms
+    counts = workloads.equal_counts(N, S)
+    conc = counts / counts.sum()
+    z = (12, 6, 24, 12, 24, 8)[:R] if lat == 1 else (8, 6, 12, 24, 8, 6)[:R]
+    with hip.Lattice(lat, n, v) as L0:
+        names = L0.layouts()
+        if layout:
+            assert names == [layout]
+    e_ref, c_ref, a_ref = [], [], []
+    for k in range(6):
+        gen = oracle.Generator(110179 + 11 * k)
+        cfg = ref.initial_setup(gen, species_numbers=counts)
+        r = ref.anneal(cfg, gen, T=T, n_mc_steps=300 * N, n_sample_steps=N, burn_in_start=True,
+                       n_burn_in_steps=200 * N, calculate_asro=True, n_sample_steps_asro=N,
+                       wc_range=R + 1)
+        e_ref.append(r["energies"][0])
+        c_ref.append(r["C"][0])
+        a_ref.append(warren_cowley(r["rho"][:, :, :, 0], z, conc)[0, 1, 1])
+    e_gpu, c_gpu, a_gpu = [], [], []
+    for k in range(8):
+        cfg = ref.initial_setup(oracle.Generator(555 + k), species_numbers=counts)
+        with hip.Lattice(lat, n, v) as L:
+            L.set_config(cfg)
+            # radial densities counted from each chain's final configuration
+            # (the in-anneal sampling needs a cubic box), 300 sampled sweeps
+            r = L.metropolis_simulated_annealing(T=T, n_sweeps=300, burn_in_sweeps=200, seed=k + 9)
+            e_gpu.append(r.energies_of_T[0])
+            c_gpu.append(r.C_of_T[0])
+            rho = ref.radial_densities(L.get_config(), R + 1, ref.lattice_shells(L.get_config(), R + 1))
+            a_gpu.append(warren_cowley(rho, z, conc)[0, 1, 1])
+    for name, g_, r_ in (("<E>/N", e_gpu, e_ref), ("C", c_gpu, c_ref)):
+        g_, r_ = np.array(g_), np.array(r_)
+        diff = g_.mean() - r_.mean()
+        sigma = np.sqrt(g_.var(ddof=1) / len(g_) + r_.var(ddof=1) / len(r_))
+        _say("kernels", n, S, name, "gpu", g_.mean(), "ref", r_.mean(), "diff/sigma", diff / sigma)
+        assert abs(diff) <= 2.0 * sigma + 1e-9, name
+    # alpha^(1) from single final configurations scatters more than the
+    # time averages of the reference chains: compare within 3 sigma of the
+    # GPU chains' spread
+    g_, r_ = np.array(a_gpu), np.array(a_ref)
+    sigma = np.sqrt(g_.var(ddof=1) / len(g_) + r_.var(ddof=1) / len(r_))
+    _say("kernels", n, S, "alpha1", g_.mean(), r_.mean(), (g_.mean() - r_.mean()) / sigma)
+    assert abs(g_.mean() - r_.mean()) <= 3.0 * sigma
