@@ -33,6 +33,10 @@ contains
   real(real64) :: e_ref, e_gpu, es_ref, es_gpu
   integer :: acc
   integer(int64) :: acc_total
+  type(bwl_anneal_params) :: ap
+  type(bwl_run_stats) :: st
+  real(c_double) :: e_T(1), c_T(1), a_T(1)
+  integer(c_int) :: arc
 
   if (lattice == 1) then
     setup%lattice = 'fcc'
@@ -131,6 +135,28 @@ contains
   end do
   write(u, '(a,1x,i0)') 'mc_accepted', acc_total
   write(u, '(a)', advance='no') 'mc_config'
+  do k = 1, 2*n
+    do j = 1, 2*n
+      do i = 1, 2*n
+        write(u, '(1x,i0)', advance='no') cfg_gpu(1, i, j, k)
+      end do
+    end do
+  end do
+  write(u, *)
+
+  ! 5. the whole temperature step through the parallel sampler, called as a
+  ! host calls it directly through the C-ABI (one sampled sweep: <E> is the
+  ! energy of the configuration it leaves)
+  ap = bwl_anneal_params(T=600.0_c_double, delta_T=0.0_c_double, T_steps=1, n_sweeps=1, &
+                         burn_in_sweeps=3, asro_every=0, seed=7_c_int64_t)
+  ! (a box too small for the sampler's rounds is refused with a status)
+  call bwl_check(bwl_set_config(bwl_context(), cfg_gpu), 'bwl_set_config')
+  arc = bwl_metropolis_anneal(bwl_context(), ap, e_T, c_T, a_T, st)
+  write(u, '(a,1x,i0)') 'anneal_status', arc
+  call bwl_check(bwl_get_config(bwl_context(), cfg_gpu), 'bwl_get_config')
+  write(u, '(a,1x,es26.17e3)') 'anneal_E', e_T(1)
+  write(u, '(a,1x,i0)') 'anneal_trials', st%trials
+  write(u, '(a)', advance='no') 'anneal_config'
   do k = 1, 2*n
     do j = 1, 2*n
       do i = 1, 2*n

@@ -5,8 +5,10 @@ adaptors (brawl_amd/fortran/brawl_hip_adaptors.f90) and evaluates the same
 configuration both ways.  Checks: total energy within 1e-12 relative of the
 reference's total_energy, site energies bit-identical to the reference's
 *_energy_Rshells, the device initial_setup identical to the reference's,
-and MC steps through the rebound pointer identical to the oracle's trial
-sequence."""
+MC steps through the rebound pointer identical to the oracle's trial
+sequence, and the parallel sampler's anneal called from Fortran through
+the C-ABI (composition kept, reported energy that of the configuration it
+returns)."""
 import ctypes as C
 import os
 
@@ -68,3 +70,15 @@ def test_fortran_host_rebinds_to_gpu(oracle, tmp_path, lattice, n, S, Rs):
     assert int(v["mc_accepted"]) == acc
     got = np.array([int(x) for x in v["mc_config"].split()], dtype=np.int8)
     assert np.array_equal(got, cfg.ravel(order="F"))
+    # the anneal called from Fortran: composition kept, and the energy it
+    # reports is the energy of the configuration it returned
+    if int(v["anneal_status"]) != 0:
+        # only a box too small for the sampler's rounds may be refused
+        assert (lattice, n) != ("fcc", 4) and int(v["anneal_status"]) == -2
+        return
+    fin = np.array([int(x) for x in v["anneal_config"].split()], dtype=np.int8)
+    assert np.array_equal(np.bincount(fin, minlength=S + 1), np.bincount(got, minlength=S + 1))
+    assert int(v["anneal_trials"]) >= 4 * N
+    fin_cfg = fin.reshape((2 * n, 2 * n, 2 * n), order="F")
+    e_fin = o.total_energy(np.asfortranarray(fin_cfg))
+    assert abs(float(v["anneal_E"]) * N - e_fin) <= 1e-11 * max(1.0, abs(e_fin))
