@@ -695,6 +695,12 @@ __global__ __launch_bounds__(NC != 0 && sizeof(CW) == 4 ? kMaxNatThreads : 512) 
         // parameters of the next kRoundBatch rounds, one thread each
         if (tid < kRoundBatch) rps[tid] = round_params(C, phase + (unsigned long long)tid);
         __syncthreads();
+        // Each thread draws its acceptance uniforms in order over the whole
+        // batch of rounds: its d-th draw is word d % 4 of Philox at counter
+        // (batch phase, tid | (d / 4) << 12, chain). One call serves four
+        // pairs even when a thread has only one or two pairs per round.
+        uint32_t draw = 0;
+        U4 rnd = {0, 0, 0, 0};
         int r = 0;
         for (; r < kRoundBatch && done < C.min_trials_per_sweep; ++r) {
           const RoundP &P = rps[r];
@@ -703,19 +709,16 @@ __global__ __launch_bounds__(NC != 0 && sizeof(CW) == 4 ? kMaxNatThreads : 512) 
           const int al = uni(P.al), ab = uni(P.ab), ac = uni(P.ac);
           const NatAffine F = nat_affine<LAT>(ax, L, stride, n_cells, uni(P.oA1), uni(P.oB1),
                                               uni(P.oC1), uni(P.oA2), uni(P.oB2), uni(P.oC2));
-          const unsigned long long ph = phase + (unsigned long long)r;
           // the fast free axis of the lanes is chosen per round axis by the
           // layout (bank spread)
           const bool bfast = (lane_order >> ax) & 1;
           uint32_t jl = jl0, ks = ks0, kf = kf0;
-          U4 rnd = {0, 0, 0, 0};
-          int k = 0;
-          for (int q = active ? tid : n2; q < n2; q += nthr, ++k) {
-            if ((k & 3) == 0)
-              rnd = philox4x32(U4{(uint32_t)ph, (uint32_t)(ph >> 32),
-                                  (uint32_t)tid | ((uint32_t)(k >> 2) << 12), C.chain_id},
+          for (int q = active ? tid : n2; q < n2; q += nthr, ++draw) {
+            if ((draw & 3) == 0)
+              rnd = philox4x32(U4{(uint32_t)phase, (uint32_t)(phase >> 32),
+                                  (uint32_t)tid | ((draw >> 2) << 12), C.chain_id},
                                C.seed_lo, C.seed_hi);
-            const uint32_t ru = (k & 3) == 0 ? rnd.x : (k & 3) == 1 ? rnd.y : (k & 3) == 2 ? rnd.z : rnd.w;
+            const uint32_t ru = (draw & 3) == 0 ? rnd.x : (draw & 3) == 1 ? rnd.y : (draw & 3) == 2 ? rnd.z : rnd.w;
             // grid-2 site, and its grid-1 partner (translation by (al, ab,
             // ac) in grid indices)
             const int kb = (int)(bfast ? kf : ks), kc = (int)(bfast ? ks : kf);
