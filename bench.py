@@ -57,7 +57,7 @@ def bytes_per_trial(z: int) -> int:
 # (232.46875 KiB, x1.0): the lattice is read once into LDS and written back
 # once, plus the per-T-step outputs and the register spills of the
 # radial-density reduction.
-TRAFFIC_BYTES = int(round((121.1875 * 2.0 + 240.46875) * 1024))
+TRAFFIC_BYTES = int(round((116.375 * 2.0 + 248.46875) * 1024))
 PLAIN_KERNEL = "k_metro_nat<bcc, R=4, 32-bit counts, 32 cells>"
 # radial densities (alpha^(1..4)) sampled every 10 sweeps of each T step
 ASRO_EVERY = 10
