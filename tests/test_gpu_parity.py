@@ -597,3 +597,35 @@ def test_parallel_sampler_statistics_other_kernels(hip, oracle, lat, n, S, R, T,
     sigma = np.sqrt(g_.var(ddof=1) / len(g_) + r_.var(ddof=1) / len(r_))
     _say("kernels", n, S, "alpha1", g_.mean(), r_.mean(), (g_.mean() - r_.mean()) / sigma)
     assert abs(g_.mean() - r_.mean()) <= 3.0 * sigma
+
+
+@pytest.mark.gpu
+def test_parallel_sampler_seed_and_stream_continuation(hip, oracle):
+    """The parallel sampler's draws depend on the seed and on the context's
+    round counter: the same seed from the same state gives the same chain
+    (configuration, energy and acceptance count), another seed another
+    chain, and a second call on a context continues the stream rather than
+    replaying the first call's draws."""
+    from brawl_amd import workloads
+    lat, n, S, R = 1, (8, 8, 8), 3, 2
+    v = workloads.random_v(S, [25.0 / k for k in range(1, R + 1)], 44)
+    counts = workloads.dirichlet_counts(workloads.n_atoms_of(lat, n), S, np.random.default_rng(3))
+    ref = oracle.OracleLattice(lat, n, v)
+    cfg = ref.initial_setup(oracle.Generator(110179), species_numbers=counts)
+
+    def run(L, seed):
+        L.set_config(cfg)
+        res = L.metropolis_simulated_annealing(T=1500.0, n_sweeps=4, seed=seed)
+        return L.get_config(), float(res.energies_of_T[0]), int(res.accepted)
+
+    _say("case fcc 8^3 S=3 R=2, seeds 5 and 6")
+    with hip.Lattice(lat, n, v) as A, hip.Lattice(lat, n, v) as B, hip.Lattice(lat, n, v) as C:
+        a1, b1, c1 = run(A, 5), run(B, 5), run(C, 6)
+        assert np.array_equal(a1[0], b1[0]) and a1[1:] == b1[1:]
+        assert not np.array_equal(a1[0], c1[0])
+        a2, b2 = run(A, 5), run(B, 5)   # same start and seed, later rounds
+        assert not np.array_equal(a1[0], a2[0])
+        assert np.array_equal(a2[0], b2[0]) and a2[1:] == b2[1:]
+        for out, _, _ in (a1, c1, a2):
+            assert np.array_equal(np.bincount(out.ravel(), minlength=S + 1),
+                                  np.bincount(cfg.ravel(), minlength=S + 1))
