@@ -257,6 +257,67 @@ int bwl_genrand_set_state(bwl_ctx *c, const uint32_t *mt, int32_t mti) {
   return BWL_OK;
 }
 
+// A lattice site of the context given as (b, i, j, k), 1-based.
+static int check_site(const Geo &g, const int32_t *s) {
+  if (s[0] != 1) return fail(BWL_ERR_ARG, "basis index must be 1");
+  const int x = s[1] - 1, y = s[2] - 1, z = s[3] - 1;
+  if (x < 0 || y < 0 || z < 0 || x >= g.m[0] || y >= g.m[1] || z >= g.m[2])
+    return fail(BWL_ERR_ARG, "site index out of range");
+  const bool occ = g.lattice == BWL_FCC ? (((x + y + z) & 1) == 0)
+                                        : ((x & 1) == (y & 1) && (y & 1) == (z & 1));
+  if (!occ) return fail(BWL_ERR_ARG, "site is not a lattice site");
+  return BWL_OK;
+}
+
+// Proposal draws on the device from the context's MT19937 stream: the
+// state goes up, the draws are made by one wavefront, the state comes back.
+static int draw_sites(bwl_ctx *c, int mode, int32_t n, const int32_t *site, int32_t *out) {
+  int rc = check_ctx(c, false);
+  if (rc) return rc;
+  if (!out || n < 1 || n > (1 << 20)) return fail(BWL_ERR_ARG, "bad count or null output");
+  if (mode == 1) {
+    if (!site) return fail(BWL_ERR_ARG, "null site");
+    rc = check_site(c->g, site);
+    if (rc) return rc;
+  }
+  if (c->mt.mti == 625) c->mt.seed(5489u);
+  const size_t vals = (size_t)n * (mode >= 2 ? 8 : 4);
+  rc = ensure_scratch(c, 624 * 4 + 64 + sizeof(int32_t) * vals);
+  if (rc) return rc;
+  char *base = (char *)c->d_scratch;
+  DrawArgs a;
+  memset(&a, 0, sizeof(a));
+  a.g = c->g;
+  a.mt = (uint32_t *)base;
+  a.mt_index = (int32_t *)(base + 624 * 4);
+  a.out = (int32_t *)(base + 624 * 4 + 64);
+  a.mode = mode;
+  a.n = mode == 1 ? 1 : n;
+  if (site) memcpy(a.site, site, sizeof(a.site));
+  int32_t mti = c->mt.mti;
+  HIP_OK(hipMemcpyAsync(a.mt, c->mt.mt, 624 * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(a.mt_index, &mti, 4, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(launch_draw_sites(a, c->stream));
+  HIP_OK(hipMemcpyAsync(c->mt.mt, a.mt, 624 * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(&mti, a.mt_index, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(out, a.out, sizeof(int32_t) * vals, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  c->mt.mti = mti;
+  return BWL_OK;
+}
+
+int bwl_random_site(bwl_ctx *c, int32_t n, int32_t *sites) {
+  return draw_sites(c, 0, n, nullptr, sites);
+}
+
+int bwl_random_nbr(bwl_ctx *c, const int32_t *site, int32_t *nbr) {
+  return draw_sites(c, 1, 1, site, nbr);
+}
+
+int bwl_select_sites(bwl_ctx *c, int32_t nbr_swap, int32_t n, int32_t *sites) {
+  return draw_sites(c, nbr_swap ? 3 : 2, n, nullptr, sites);
+}
+
 int bwl_initial_setup(bwl_ctx *c, const int64_t *species_numbers, const double *concentrations) {
   int rc = check_ctx(c, false);
   if (rc) return rc;
@@ -695,7 +756,7 @@ int anneal_plan(int n, bwl_ctx *const *ctxs, const bwl_anneal_params *params, An
   return BWL_OK;
 }
 
-int anneal_upload(AnnealPlan &P, char *base, hipStream_t s) {
+int anneal_upload(AnnealPlan &P, char *base, hipStream_t s, bool zero_by_kernel) {
   P.base = base;
   P.host_args.clear();
   for (AnnealPlan::Item &it : P.items) {
@@ -707,7 +768,10 @@ int anneal_upload(AnnealPlan &P, char *base, hipStream_t s) {
     it.a.rho_of_T = it.rho_n ? (double *)(base + it.off_rho) : nullptr;
     HIP_OK(hipMemcpyAsync(base + it.off_muls, it.plan.muls.data(), it.plan.muls.size() * 4,
                           hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemsetAsync(it.a.counters, 0, 3 * sizeof(long long), s));
+    if (zero_by_kernel)
+      HIP_OK(launch_zero_i64(it.a.counters, 3, s));
+    else
+      HIP_OK(hipMemsetAsync(it.a.counters, 0, 3 * sizeof(long long), s));
   }
   for (AnnealPlan::Group &gr : P.groups) {
     std::vector<ParArgs> host;

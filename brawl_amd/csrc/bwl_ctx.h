@@ -120,7 +120,8 @@ struct AnnealPlan {
 int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int &threads,
                   size_t &lds, bool *natural = nullptr, int layout = 0);
 int anneal_plan(int n, bwl_ctx *const *ctxs, const bwl_anneal_params *params, AnnealPlan &P);
-int anneal_upload(AnnealPlan &P, char *scratch_dev, hipStream_t s);
+// zero_by_kernel: clear the counters with a kernel instead of a memset
+int anneal_upload(AnnealPlan &P, char *scratch_dev, hipStream_t s, bool zero_by_kernel = false);
 int anneal_launch(const AnnealPlan &P, const hipStream_t *streams);
 int anneal_collect(const AnnealPlan &P, bwl_ctx *const *ctxs, double *const *energies_of_T,
                    double *const *C_of_T, double *const *acceptance_of_T, bwl_run_stats *stats,

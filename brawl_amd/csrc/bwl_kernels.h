@@ -40,6 +40,20 @@ struct SeqArgs {
   long long *accepted;
 };
 
+// Proposal draws from a context's MT19937 stream (random_site.f90:127-225,
+// wang-landau.F90:1527-1539), one wavefront:
+//   mode 0: n sites (rdm_site), mode 1: one neighbour of `site` (rdm_nbr),
+//   mode 2: n pairs (rdm_site, rdm_site), mode 3: n pairs (rdm_site, rdm_nbr).
+// Sites are written as 4 int32 (b, i, j, k), 1-based.
+struct DrawArgs {
+  Geo g;
+  uint32_t *mt;        // 624 words, in/out
+  int32_t *mt_index;   // in/out
+  int32_t mode, n;
+  int32_t site[4];     // mode 1 input
+  int32_t *out;        // 4 * n (modes 0, 1) or 8 * n (modes 2, 3) values
+};
+
 struct ParArgs {
   Geo g;
   NatGeo nat;            // used by the natural-layout kernel
@@ -166,6 +180,10 @@ size_t site_lds_bytes(const Geo &g);
 hipError_t launch_site_energy(const SiteArgs &a, hipStream_t s);
 size_t seq_lds_bytes(const Geo &g);
 hipError_t launch_metro_seq(const SeqArgs &a, hipStream_t s);
+hipError_t launch_draw_sites(const DrawArgs &a, hipStream_t s);
+// Sets n 64-bit words to zero with a kernel (no memset node in a captured
+// graph).
+hipError_t launch_zero_i64(long long *p, int n, hipStream_t s);
 // n chains (args_dev[0..n)), one wavefront each; key as for launch_wl_sweeps
 hipError_t launch_metro_seq_batch(const SeqArgs *args_dev, int n, size_t lds, hipStream_t s,
                                   int key);

@@ -210,6 +210,62 @@ __device__ __forceinline__ void metro_seq_run(const SeqArgs &A, uint8_t *smem) {
 }
 
 // ------------------------------------------------------------------
+// Proposal draws (rdm_site, rdm_nbr, mc_select) from the reference stream,
+// one wavefront (the MT19937 twist is wavefront-wide); lane 0 writes.
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_draw_sites(DrawArgs A) {
+  __shared__ uint32_t st[624];
+  const int lane = threadIdx.x;
+  for (int i = lane; i < 624; i += 64) st[i] = A.mt[i];
+  __syncthreads();
+  int mti = A.mt_index[0];
+  const Geo &g = A.g;
+  auto put = [&](int32_t *o, SiteXYZ s) {
+    if (lane == 0) {
+      o[0] = 1;
+      o[1] = s.x + 1;
+      o[2] = s.y + 1;
+      o[3] = s.z + 1;
+    }
+  };
+  if (A.mode == 1) {
+    SiteXYZ s;
+    s.x = A.site[1] - 1;
+    s.y = A.site[2] - 1;
+    s.z = A.site[3] - 1;
+    put(A.out, random_nbr_dev(g, s, st, mti));
+  } else {
+    for (int q = 0; q < A.n; ++q) {
+      SiteXYZ s1 = random_site_dev(g, st, mti);
+      if (A.mode == 0) {
+        put(A.out + 4 * q, s1);
+        continue;
+      }
+      SiteXYZ s2 = A.mode == 3 ? random_nbr_dev(g, s1, st, mti) : random_site_dev(g, st, mti);
+      put(A.out + 8 * q, s1);
+      put(A.out + 8 * q + 4, s2);
+    }
+  }
+  __syncthreads();
+  for (int i = lane; i < 624; i += 64) A.mt[i] = st[i];
+  if (lane == 0) A.mt_index[0] = mti;
+}
+
+hipError_t launch_draw_sites(const DrawArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_draw_sites, dim3(1), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(64) void k_zero_i64(long long *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+hipError_t launch_zero_i64(long long *p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(64), 0, s, p, n);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------
 hipError_t launch_pack(const int8_t *ref, int8_t *compact, const Geo &g, hipStream_t s) {

@@ -137,6 +137,27 @@ module brawl_hip
       integer(c_int32_t), value :: mti
     end function
 
+    integer(c_int) function bwl_random_site(ctx, n, sites) bind(C, name="bwl_random_site")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: n
+      integer(c_int32_t), intent(out) :: sites(4, *)
+    end function
+
+    integer(c_int) function bwl_random_nbr(ctx, site, nbr) bind(C, name="bwl_random_nbr")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(in) :: site(4)
+      integer(c_int32_t), intent(out) :: nbr(4)
+    end function
+
+    integer(c_int) function bwl_select_sites(ctx, nbr_swap, n, sites) bind(C, name="bwl_select_sites")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nbr_swap, n
+      integer(c_int32_t), intent(out) :: sites(4, 2, *)
+    end function
+
     integer(c_int) function bwl_initial_setup(ctx, species_numbers, concentrations) &
         bind(C, name="bwl_initial_setup")
       import :: c_int, c_ptr

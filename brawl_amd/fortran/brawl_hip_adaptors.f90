@@ -1,13 +1,17 @@
 !> brawl_hip_adaptors -- procedures with the reference's own interfaces
-!> (hamiltonian, neighbour, monte_carlo: derived_types.f90:373-486) that run
-!> on the GPU through brawl_hip.  A BraWl host rebinds its procedure
-!> pointers with one line each (initialise.F90:157-241, metropolis.F90:
-!> 171-175):
+!> (hamiltonian, neighbour, rand_site, rand_neighbour, monte_carlo:
+!> derived_types.f90:373-486; wl_params%mc_select: :349) that run on the
+!> GPU through brawl_hip.  A BraWl host rebinds its procedure pointers with
+!> one line each (initialise.F90:157-241, metropolis.F90:171-175,
+!> wang-landau.F90:110-114):
 !>
 !>   call bwl_attach(setup)                  ! after V_ex is read
 !>   setup%full_energy => bwl_full_energy
 !>   setup%nbr_energy  => bwl_nbr_energy
+!>   setup%rdm_site    => bwl_rdm_site
+!>   setup%rdm_nbr     => bwl_rdm_nbr
 !>   setup%mc_step     => bwl_mc_step_lattice ! or bwl_mc_step_nbr
+!>   wl_setup%mc_select => bwl_mc_select      ! or bwl_mc_select_nbr
 !>
 !> This module uses the reference's modules (kinds, constants,
 !> derived_types, shared_data), so it is compiled inside a BraWl build.
@@ -33,7 +37,8 @@ module brawl_hip_adaptors
   private
   public :: bwl_attach, bwl_detach, bwl_context, bwl_seed_stream, bwl_full_energy, &
             bwl_nbr_energy, bwl_mc_step_lattice, bwl_mc_step_nbr, bwl_metropolis_sweeps, &
-            bwl_initial_setup_gpu
+            bwl_initial_setup_gpu, bwl_rdm_site, bwl_rdm_nbr, bwl_mc_select, &
+            bwl_mc_select_nbr
 
   type(c_ptr), save :: ctx = c_null_ptr
 
@@ -153,28 +158,86 @@ contains
     accepted = int(mc_steps(config, beta, nbr, int(n_trials, c_int64_t)), int64)
   end function bwl_metropolis_sweeps
 
-  !> initial_setup (initialise.F90:434-617) from the context's stream; the
-  !> result is written to `config`.
-  subroutine bwl_initial_setup_gpu(setup, config)
+  !> `rand_site` (bound as setup%rdm_site; bcc_random_site /
+  !> fcc_random_site, random_site.f90:127-193): one site drawn from the
+  !> context's stream as the reference draws it.
+  function bwl_rdm_site(setup) result(site)
     class(run_params), intent(in) :: setup
+    integer, dimension(4) :: site
+    integer(c_int32_t) :: s(4, 1)
+    call bwl_check(bwl_random_site(ctx, 1_c_int32_t, s), 'bwl_random_site')
+    site = int(s(:, 1))
+  end function bwl_rdm_site
+
+  !> `rand_neighbour` (bound as setup%rdm_nbr; *_random_nbr,
+  !> random_site.f90:152-225): a shell-1 neighbour of `site`, one draw.
+  function bwl_rdm_nbr(setup, site) result(nbr)
+    class(run_params), intent(in) :: setup
+    integer, dimension(4), intent(in) :: site
+    integer, dimension(4) :: nbr
+    integer(c_int32_t) :: s(4), t(4)
+    s = int(site, c_int32_t)
+    call bwl_check(bwl_random_nbr(ctx, s, t), 'bwl_random_nbr')
+    nbr = int(t)
+  end function bwl_rdm_nbr
+
+  !> wl_params%mc_select for lattice-wide swaps (select_sites,
+  !> wang-landau.F90:1534-1539): two random sites.
+  subroutine bwl_mc_select(rdm1, rdm2)
+    integer, dimension(4) :: rdm1, rdm2
+    integer(c_int32_t) :: s(4, 2, 1)
+    call bwl_check(bwl_select_sites(ctx, 0_c_int32_t, 1_c_int32_t, s), 'bwl_select_sites')
+    rdm1 = int(s(:, 1, 1))
+    rdm2 = int(s(:, 2, 1))
+  end subroutine bwl_mc_select
+
+  !> wl_params%mc_select for nearest-neighbour swaps (select_sites_nbr,
+  !> wang-landau.F90:1527-1532): a random site and a shell-1 neighbour.
+  subroutine bwl_mc_select_nbr(rdm1, rdm2)
+    integer, dimension(4) :: rdm1, rdm2
+    integer(c_int32_t) :: s(4, 2, 1)
+    call bwl_check(bwl_select_sites(ctx, 1_c_int32_t, 1_c_int32_t, s), 'bwl_select_sites')
+    rdm1 = int(s(:, 1, 1))
+    rdm2 = int(s(:, 2, 1))
+  end subroutine bwl_mc_select_nbr
+  !> initial_setup (initialise.F90:434-617) from the context's stream; the
+  !> result is written to `config`.  The concentrations are taken from
+  !> elements 1..n_species of setup%species_concentrations, which the
+  !> reference allocates 0:n_species with element 0 = 0 (io.f90:242).  On
+  !> the species-numbers path the reference also stores the REAL(4)
+  !> concentrations count/n_atoms in setup (initialise.F90:468-472); so
+  !> does this adaptor.
+  subroutine bwl_initial_setup_gpu(setup, config)
+    class(run_params), intent(inout) :: setup
     integer(array_int), dimension(:,:,:,:), intent(out) :: config
     integer(c_int64_t), allocatable, target :: sn(:)
     real(c_double), allocatable, target :: cc(:)
     type(c_ptr) :: psn, pcc
+    integer :: ns, ub, i
+    ns = setup%n_species
     psn = c_null_ptr
     pcc = c_null_ptr
     if (allocated(setup%species_numbers)) then
-      allocate(sn(size(setup%species_numbers)))
-      sn = setup%species_numbers
+      allocate(sn(ns))
+      sn = setup%species_numbers(lbound(setup%species_numbers, 1):lbound(setup%species_numbers, 1)+ns-1)
       psn = c_loc(sn)
     end if
     if (allocated(setup%species_concentrations)) then
-      allocate(cc(size(setup%species_concentrations)))
-      cc = setup%species_concentrations
+      ub = ubound(setup%species_concentrations, 1)
+      allocate(cc(ns))
+      cc = setup%species_concentrations(ub-ns+1:ub)
       pcc = c_loc(cc)
     end if
     call bwl_check(bwl_initial_setup(ctx, psn, pcc), 'bwl_initial_setup')
     call bwl_check(bwl_get_config(ctx, config), 'bwl_get_config')
+    if (allocated(setup%species_numbers) .and. allocated(setup%species_concentrations)) then
+      if (sum(setup%species_numbers) == setup%n_atoms) then
+        ub = ubound(setup%species_concentrations, 1)
+        do i = 1, ns
+          setup%species_concentrations(ub-ns+i) = real(real(setup%species_numbers(i)) / real(setup%n_atoms), c_double)
+        end do
+      end if
+    end if
   end subroutine bwl_initial_setup_gpu
 
 end module brawl_hip_adaptors

@@ -9,6 +9,8 @@ methods carry the reference's procedure names:
     nbr_energy                    setup%nbr_energy    (bw_hamiltonian.f90:898-1910)
     pair_energy                   pair_energy         (bw_hamiltonian.f90:99-114)
     initial_setup                 initial_setup       (initialise.F90:434-617)
+    rdm_site / rdm_nbr            setup%rdm_site, setup%rdm_nbr (random_site.f90:127-225)
+    select_sites                  wl_params%mc_select (wang-landau.F90:1527-1539)
     mc_step / mc_steps            setup%mc_step       (metropolis.F90:751-891)
     metropolis_simulated_annealing  the temperature loop (metropolis.F90:193-437)
     lattice_shells / radial_densities  analytics.f90:205-404
@@ -20,6 +22,7 @@ library's status code; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import warnings
 from dataclasses import dataclass
 
 import numpy as np
@@ -168,6 +171,26 @@ class Lattice:
                                         None if cc is None else _p(cc, C.c_double)))
         return self.get_config()
 
+    def rdm_site(self, n: int = 1) -> np.ndarray:
+        """n draws of setup%rdm_site from the context's stream: int32 (n, 4)
+        rows (b, i, j, k), 1-based."""
+        out = np.zeros((int(n), 4), dtype=np.int32)
+        check(self._L.bwl_random_site(self._h, int(n), _p(out, C.c_int32)))
+        return out
+
+    def rdm_nbr(self, site) -> np.ndarray:
+        """setup%rdm_nbr(site): a random shell-1 neighbour (one draw)."""
+        s = np.ascontiguousarray(site, dtype=np.int32)
+        out = np.zeros(4, dtype=np.int32)
+        check(self._L.bwl_random_nbr(self._h, _p(s, C.c_int32), _p(out, C.c_int32)))
+        return out
+
+    def select_sites(self, n: int = 1, nbr_swap: bool = False) -> np.ndarray:
+        """n calls of wl_params%mc_select: int32 (n, 2, 4), (rdm1, rdm2)."""
+        out = np.zeros((int(n), 2, 4), dtype=np.int32)
+        check(self._L.bwl_select_sites(self._h, int(bool(nbr_swap)), int(n), _p(out, C.c_int32)))
+        return out
+
     # -- sequential (bit-exact) sampler -------------------------------------
     def mc_steps(self, beta: float, n_trials: int, *, nbr_swap=False, sample_every=0):
         """n_trials reference trials; returns (accepted, energies or None)."""
@@ -267,10 +290,23 @@ def anneal_batch(lattices, params):
     """Batched form of metropolis_simulated_annealing: one workgroup per
     chain, all chains in as few launches as possible.  `params` is a list of
     dicts with the keyword arguments of metropolis_simulated_annealing.
-    Returns (list of AnnealResult, makespan_ms)."""
+    Returns (list of AnnealResult, makespan_ms).
+
+    A chain's random numbers depend only on its seed and its position in
+    that stream, so chains given equal seeds from equal stream positions
+    draw the same numbers (the same pairs and acceptance uniforms) and their
+    statistics are not independent.  Chain i without a "seed" key gets its
+    own, derived from the master seed and i by SplitMix64; equal explicit
+    seeds draw a warning."""
+    from .workloads import MASTER_SEED, derive_seed
     n = len(lattices)
     if n != len(params) or n == 0:
         raise ValueError("one parameter set per lattice")
+    seeds = [int(p["seed"]) if "seed" in p else derive_seed(MASTER_SEED, i)
+             for i, p in enumerate(params)]
+    if len(set(seeds)) != n:
+        warnings.warn("anneal_batch: chains share a seed; their random streams are identical",
+                      stacklevel=2)
     L = _lib.load()
     P = (AnnealParams * n)()
     outs = []
@@ -278,7 +314,7 @@ def anneal_batch(lattices, params):
         T_steps = int(p.get("T_steps", 1))
         P[i] = AnnealParams(float(p["T"]), float(p.get("delta_T", 0.0)), T_steps,
                             int(p["n_sweeps"]), int(p.get("burn_in_sweeps", 0)), 0,
-                            int(p.get("seed", 20261202)) & (2**64 - 1))
+                            seeds[i] & (2**64 - 1))
         outs.append((np.zeros(T_steps), np.zeros(T_steps), np.zeros(T_steps)))
     H = (C.c_void_p * n)(*[lat._h for lat in lattices])
     DP = C.POINTER(C.c_double)

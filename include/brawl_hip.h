@@ -97,6 +97,23 @@ int bwl_genrand_seed(bwl_ctx *ctx, uint32_t seed);
  * step with the device. */
 int bwl_genrand_get_state(bwl_ctx *ctx, uint32_t *mt, int32_t *mti);
 int bwl_genrand_set_state(bwl_ctx *ctx, const uint32_t *mt, int32_t mti);
+/* `rand_site` interface, bound as setup%rdm_site (derived_types.f90:
+ * 424-434; bcc_random_site / fcc_random_site, random_site.f90:127-139,
+ * 182-193): n consecutive draws from the context's MT19937 stream, each
+ * made as the reference makes it (genrand for k, then i, then j).  sites
+ * receives int32 (4, n): (b, i, j, k) per site, 1-based. */
+int bwl_random_site(bwl_ctx *ctx, int32_t n, int32_t *sites);
+/* `rand_neighbour` interface, bound as setup%rdm_nbr (derived_types.f90:
+ * 447-458; bcc_random_nbr / fcc_random_nbr, random_site.f90:152-171,
+ * 206-225): one draw from the context's stream picks a shell-1 neighbour
+ * of `site` (b, i, j, k), wrapped into the box; nbr receives 4 int32. */
+int bwl_random_nbr(bwl_ctx *ctx, const int32_t *site, int32_t *nbr);
+/* wl_params%mc_select (derived_types.f90:349, bound at wang-landau.F90:
+ * 110-114; select_sites / select_sites_nbr, :1527-1539): n consecutive
+ * site pairs (rdm1 = rdm_site, rdm2 = rdm_site, or rdm_nbr(rdm1) when
+ * nbr_swap != 0) from the context's stream; sites receives int32
+ * (4, 2, n). */
+int bwl_select_sites(bwl_ctx *ctx, int32_t nbr_swap, int32_t n, int32_t *sites);
 /* initial_setup (initialise.F90:434-617) from the context's stream: uses
  * species_numbers (length S) when they sum to n_atoms, else
  * concentrations (length S).  Result is uploaded as the configuration. */

@@ -195,6 +195,13 @@ class OracleLattice:
         lib().bwo_random_site(self.ref, C.byref(gen.state), _ptr(s, C.c_int32))
         return tuple(int(x) for x in s)
 
+    def random_nbr(self, gen, site):
+        """(i, j, k) of a random shell-1 neighbour of site (i, j, k)."""
+        s = np.ascontiguousarray(site, dtype=np.int32)
+        out = np.zeros(3, dtype=np.int32)
+        lib().bwo_random_nbr(self.ref, C.byref(gen.state), _ptr(s, C.c_int32), _ptr(out, C.c_int32))
+        return tuple(int(x) for x in out)
+
     def mc_step(self, cfg, beta, gen, nbr_swap=False) -> int:
         return lib().bwo_mc_step(self.ref, self._cfg(cfg), beta, int(nbr_swap), C.byref(gen.state))
 

@@ -45,7 +45,9 @@ def lib():
             "ref_total_energy": (C.c_double, [i32] * 6 + [dp, i8p]),
             "ref_site_energies": (None, [i32] * 6 + [dp, i8p, dp]),
             "ref_initial_setup": (None, [i32] * 6 + [i64p, i8p]),
+            "ref_initial_setup_conc": (None, [i32] * 6 + [dp, i8p]),
             "ref_random_sites": (None, [i32] * 6 + [i32p]),
+            "ref_random_nbrs": (None, [i32] * 6 + [i32p]),
             "ref_radial_densities": (None, [i32] * 6 + [i8p, dp, dp]),
             "ref_metropolis_sweeps": (C.c_int64, [i32] * 6 + [dp, i8p, C.c_double, i32, i32,
                                                              dp, dp]),
@@ -133,9 +135,26 @@ def initial_setup(lattice, n, species_numbers, rank=0) -> np.ndarray:
 
 
 @_isolated
+def initial_setup_conc(lattice, n, concentrations, rank=0) -> np.ndarray:
+    """initial_setup on the concentrations path (species numbers 0)."""
+    cc = np.ascontiguousarray(concentrations, dtype=np.float64)
+    cfg = np.zeros(tuple(2 * x for x in n), dtype=np.int8, order="F")
+    lib().ref_initial_setup_conc(lattice, *n, len(cc), rank, _p(cc, C.c_double), _p(cfg, C.c_int8))
+    return cfg
+
+
+@_isolated
 def random_sites(lattice, n, count, rank=0) -> np.ndarray:
     out = np.zeros((count, 4), dtype=np.int32)
     lib().ref_random_sites(lattice, *n, rank, count, _p(out, C.c_int32))
+    return out
+
+
+@_isolated
+def random_nbrs(lattice, n, count, rank=0) -> np.ndarray:
+    """count (rdm_site, rdm_nbr) pairs: int32 (count, 2, 4)."""
+    out = np.zeros((count, 2, 4), dtype=np.int32)
+    lib().ref_random_nbrs(lattice, *n, rank, count, _p(out, C.c_int32))
     return out
 
 

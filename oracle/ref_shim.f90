@@ -37,10 +37,13 @@ contains
     setup%interaction_range = nr
     setup%wc_range = 1
     setup%static_seed = .true.
-    ! initialise_function_pointers checks that the concentrations sum to 1
-    allocate(setup%species_numbers(ns), setup%species_concentrations(ns))
+    ! initialise_function_pointers checks that the concentrations sum to 1;
+    ! the concentrations are allocated 0:ns with element 0 = 0, as
+    ! read_inputs does (io.f90:242-246): initial_setup sums from element 0
+    allocate(setup%species_numbers(ns), setup%species_concentrations(0:ns))
     setup%species_numbers = 0_int64
     setup%species_concentrations = 1.0_real64 / real(ns, real64)
+    setup%species_concentrations(0) = 0.0_real64
     call initialise_function_pointers(setup)
     if (allocated(config)) deallocate(config)
     allocate(config(setup%n_basis, 2*n1, 2*n2, 2*n3))
@@ -115,6 +118,26 @@ contains
     cfg(1:nc) = int(reshape(config, [nc]), c_int8_t)
   end subroutine ref_initial_setup
 
+  !> initial_setup on its concentrations path (initialise.F90:474-505):
+  !> species_numbers left at 0, concentrations(1:ns) as given, element 0 = 0.
+  subroutine ref_initial_setup_conc(lattice, n1, n2, n3, ns, rank, conc, cfg) &
+      bind(C, name="ref_initial_setup_conc")
+    integer(c_int32_t), value :: lattice, n1, n2, n3, ns, rank
+    real(c_double), intent(in) :: conc(*)
+    integer(c_int8_t), intent(out) :: cfg(*)
+    type(run_params) :: setup
+    integer(c_int) :: seed
+    integer :: nc
+    call make_setup(setup, lattice, n1, n2, n3, ns, 1)
+    setup%species_numbers = 0_int64
+    setup%species_concentrations(0) = 0.0_real64
+    setup%species_concentrations(1:ns) = conc(1:ns)
+    seed = f90_init_genrand(0_c_int, int(rank, c_int), 0_c_int)
+    call initial_setup(setup, config)
+    nc = size(config)
+    cfg(1:nc) = int(reshape(config, [nc]), c_int8_t)
+  end subroutine ref_initial_setup_conc
+
   !> `count` draws of setup%rdm_site (random_site.f90:127-193) after
   !> f90_init_genrand for `rank`; sites(4, count).
   subroutine ref_random_sites(lattice, n1, n2, n3, rank, count, sites) &
@@ -130,6 +153,24 @@ contains
       sites(:, q) = setup%rdm_site()
     end do
   end subroutine ref_random_sites
+
+  !> `count` pairs (setup%rdm_site, setup%rdm_nbr of it) after
+  !> f90_init_genrand for `rank`, as select_sites_nbr draws them
+  !> (wang-landau.F90:1527-1532); sites(4, 2, count).
+  subroutine ref_random_nbrs(lattice, n1, n2, n3, rank, count, sites) &
+      bind(C, name="ref_random_nbrs")
+    integer(c_int32_t), value :: lattice, n1, n2, n3, rank, count
+    integer(c_int32_t), intent(out) :: sites(4, 2, *)
+    type(run_params) :: setup
+    integer(c_int) :: seed
+    integer :: q
+    call make_setup(setup, lattice, n1, n2, n3, 1, 1)
+    seed = f90_init_genrand(0_c_int, int(rank, c_int), 0_c_int)
+    do q = 1, count
+      sites(:, 1, q) = setup%rdm_site()
+      sites(:, 2, q) = setup%rdm_nbr(sites(:, 1, q))
+    end do
+  end subroutine ref_random_nbrs
 
   !> lattice_shells + radial_densities (analytics.f90:205-404).
   subroutine ref_radial_densities(lattice, n1, n2, n3, ns, wc, cfg, shells_out, rho) &

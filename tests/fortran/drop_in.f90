@@ -6,14 +6,21 @@
 !>   int drop_in_run(int lattice, int n, int S, int R, int n_trials,
 !>                   const char *out_file)
 !> It writes `key value` lines to out_file and returns the first error
-!> status of the GPU library (0 if none).
+!> status of the GPU library (0 if none).  Two more entry points check the
+!> set-up and proposal adaptors:
+!>   int drop_in_setup(int lattice, int n, int S, int rank, int use_numbers,
+!>                     const int64_t *numbers, const double *conc,
+!>                     int8_t *cfg_gpu, int8_t *cfg_ref,
+!>                     double *conc_gpu, double *conc_ref)
+!>   int drop_in_draws(int lattice, int n, int rank, int count,
+!>                     int *mismatches)   /* 4 counts */
 module drop_in
 
   use iso_c_binding
   use kinds, only : real64, int64
   use constants, only : array_int
   use shared_data, only : config, V_ex
-  use derived_types, only : run_params
+  use derived_types, only : run_params, wl_params
   use c_functions, only : f90_init_genrand
   use initialise, only : initialise_function_pointers, initial_setup
   use brawl_hip
@@ -67,7 +74,9 @@ contains
   else
     setup%n_atoms = 2*n*n*n
   end if
-  allocate(setup%species_numbers(ns), setup%species_concentrations(ns))
+  ! concentrations allocated 0:ns with element 0 = 0, as read_inputs does
+  ! (io.f90:242-246); initial_setup sums them from element 0
+  allocate(setup%species_numbers(ns), setup%species_concentrations(0:ns))
   setup%species_numbers = int(setup%n_atoms / ns, int64)
   setup%species_numbers(1) = setup%species_numbers(1) + int(setup%n_atoms - ns*(setup%n_atoms/ns), int64)
   setup%species_concentrations = 0.0_real64
@@ -169,5 +178,147 @@ contains
   call bwl_detach()
   status = bwl_first_error
   end function drop_in_run
+
+  !> A setup as read_inputs leaves it (io.f90:242-246: concentrations
+  !> 0:ns, element 0 = 0); the caller fills in numbers or concentrations.
+  subroutine make_setup(setup, lattice, n, ns)
+    type(run_params), intent(out) :: setup
+    integer(c_int), intent(in) :: lattice, n, ns
+    if (lattice == 1) then
+      setup%lattice = 'fcc'
+      setup%n_atoms = 4*n*n*n
+    else
+      setup%lattice = 'bcc'
+      setup%n_atoms = 2*n*n*n
+    end if
+    setup%n_1 = n
+    setup%n_2 = n
+    setup%n_3 = n
+    setup%n_species = ns
+    setup%interaction_range = 1
+    setup%wc_range = 2
+    setup%static_seed = .true.
+    allocate(setup%species_numbers(ns), setup%species_concentrations(0:ns))
+    setup%species_numbers = 0_int64
+    setup%species_concentrations = 0.0_real64
+  end subroutine make_setup
+
+  !> initial_setup of `rank`'s static-seed stream both ways: the
+  !> reference's routine and the GPU adaptor, each on its own setup, on the
+  !> species-numbers path (use_numbers /= 0) or the concentrations path.
+  !> Writes both configurations and both setups' concentrations(1:ns)
+  !> afterwards (the reference stores count/n_atoms there on the numbers
+  !> path).
+  function drop_in_setup(lattice, n, ns, rank, use_numbers, numbers, conc, cfg_gpu, cfg_ref, &
+                         conc_gpu, conc_ref) result(status) bind(C, name="drop_in_setup")
+    integer(c_int), value :: lattice, n, ns, rank, use_numbers
+    integer(c_int64_t), intent(in) :: numbers(*)
+    real(c_double), intent(in) :: conc(*)
+    integer(c_int8_t), intent(out) :: cfg_gpu(*), cfg_ref(*)
+    real(c_double), intent(out) :: conc_gpu(*), conc_ref(*)
+    integer(c_int) :: status
+    type(run_params) :: s_ref, s_gpu
+    integer(array_int), allocatable :: c_gpu(:,:,:,:)
+    integer :: seed, nc
+    bwl_stop_on_error = .false.
+    bwl_first_error = BWL_OK
+    call make_setup(s_ref, lattice, n, ns)
+    call make_setup(s_gpu, lattice, n, ns)
+    if (use_numbers /= 0) then
+      s_ref%species_numbers = numbers(1:ns)
+      s_gpu%species_numbers = numbers(1:ns)
+    else
+      s_ref%species_concentrations(1:ns) = conc(1:ns)
+      s_gpu%species_concentrations(1:ns) = conc(1:ns)
+    end if
+    call initialise_function_pointers(s_ref)
+    call initialise_function_pointers(s_gpu)
+    if (allocated(config)) deallocate(config)
+    if (allocated(V_ex)) deallocate(V_ex)
+    allocate(config(s_ref%n_basis, 2*n, 2*n, 2*n))
+    allocate(V_ex(ns, ns, 1))
+    V_ex = 0.0_real64
+    config = 0_array_int
+    ! the reference
+    seed = f90_init_genrand(0_c_int, int(rank, c_int), 0_c_int)
+    call initial_setup(s_ref, config)
+    nc = size(config)
+    cfg_ref(1:nc) = int(reshape(config, [nc]), c_int8_t)
+    conc_ref(1:ns) = s_ref%species_concentrations(1:ns)
+    ! the adaptor, from the same stream on the device
+    allocate(c_gpu(s_gpu%n_basis, 2*n, 2*n, 2*n))
+    call bwl_attach(s_gpu)
+    call bwl_seed_stream(rank)
+    call bwl_initial_setup_gpu(s_gpu, c_gpu)
+    cfg_gpu(1:nc) = int(reshape(c_gpu, [nc]), c_int8_t)
+    conc_gpu(1:ns) = s_gpu%species_concentrations(1:ns)
+    call bwl_detach()
+    status = bwl_first_error
+  end function drop_in_setup
+
+  !> The proposal adaptors against the reference's own draws from the same
+  !> stream: setup%rdm_site, setup%rdm_nbr (of the site just drawn) and
+  !> wl_params%mc_select both ways (select_sites: two rdm_site; and
+  !> select_sites_nbr: rdm_site then rdm_nbr, wang-landau.F90:1527-1539),
+  !> `count` draws each.  mism(1:4) = mismatching draws of rdm_site,
+  !> rdm_nbr, mc_select, mc_select_nbr.
+  function drop_in_draws(lattice, n, rank, count, mism) result(status) bind(C, name="drop_in_draws")
+    integer(c_int), value :: lattice, n, rank, count
+    integer(c_int), intent(out) :: mism(4)
+    integer(c_int) :: status
+    type(run_params) :: s_ref, s_gpu
+    type(wl_params) :: wl
+    integer, allocatable :: ref(:,:,:)
+    integer, dimension(4) :: a, b
+    integer :: seed, q, mode
+    bwl_stop_on_error = .false.
+    bwl_first_error = BWL_OK
+    mism = 0
+    call make_setup(s_ref, lattice, n, 2)
+    s_ref%species_concentrations(1:2) = 0.5_real64
+    call make_setup(s_gpu, lattice, n, 2)
+    s_gpu%species_concentrations(1:2) = 0.5_real64
+    call initialise_function_pointers(s_ref)
+    call initialise_function_pointers(s_gpu)
+    if (allocated(V_ex)) deallocate(V_ex)
+    allocate(V_ex(2, 2, 1))
+    V_ex = 0.0_real64
+    call bwl_attach(s_gpu)
+    s_gpu%rdm_site => bwl_rdm_site
+    s_gpu%rdm_nbr => bwl_rdm_nbr
+    allocate(ref(4, 2, count))
+    do mode = 1, 4
+      ! the reference's draws
+      seed = f90_init_genrand(0_c_int, int(rank, c_int), 0_c_int)
+      do q = 1, count
+        ref(:, 1, q) = s_ref%rdm_site()
+        if (mode == 2 .or. mode == 4) then
+          ref(:, 2, q) = s_ref%rdm_nbr(ref(:, 1, q))
+        else if (mode == 3) then
+          ref(:, 2, q) = s_ref%rdm_site()
+        end if
+      end do
+      ! the rebound pointers, same stream on the device
+      call bwl_seed_stream(rank)
+      if (mode == 3) wl%mc_select => bwl_mc_select
+      if (mode == 4) wl%mc_select => bwl_mc_select_nbr
+      do q = 1, count
+        if (mode <= 2) then
+          a = s_gpu%rdm_site()
+          b = ref(:, 2, q)
+          if (mode == 2) b = s_gpu%rdm_nbr(a)
+        else
+          call wl%mc_select(a, b)
+        end if
+        if (any(a /= ref(:, 1, q))) then
+          mism(mode) = mism(mode) + 1
+        else if (mode >= 2 .and. any(b /= ref(:, 2, q))) then
+          mism(mode) = mism(mode) + 1
+        end if
+      end do
+    end do
+    call bwl_detach()
+    status = bwl_first_error
+  end function drop_in_draws
 
 end module drop_in

@@ -88,3 +88,31 @@ def test_no_cpu_fallback_without_gpu():
         pytest.skip("a GPU is present")
     rc, msg = _create((1, 4, 4, 4, 2, 2), np.zeros(8))
     assert rc == _lib.BWL_ERR_DEVICE
+
+
+def test_anneal_batch_seeds_are_distinct_by_default(monkeypatch):
+    """Chains without a seed key get distinct derived seeds; equal explicit
+    seeds warn (their streams would be identical).  The library call is
+    replaced by a recorder: no device is touched."""
+    import warnings
+
+    from brawl_amd import _lib, lattice
+
+    seen = {}
+
+    class Fake:
+        def bwl_metropolis_anneal_batch(self, n, H, P, *rest):
+            seen["seeds"] = [P[i].seed for i in range(n)]
+            return 0
+
+    class Lat:
+        _h = None
+
+    monkeypatch.setattr(_lib, "load", lambda: Fake())
+    lats = [Lat(), Lat(), Lat()]
+    lattice.anneal_batch(lats, [dict(T=500.0, n_sweeps=1)] * 3)
+    assert len(set(seen["seeds"])) == 3
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        lattice.anneal_batch(lats, [dict(T=500.0, n_sweeps=1, seed=7)] * 3)
+    assert any("share a seed" in str(x.message) for x in w)

@@ -82,3 +82,75 @@ def test_fortran_host_rebinds_to_gpu(oracle, tmp_path, lattice, n, S, Rs):
     fin_cfg = fin.reshape((2 * n, 2 * n, 2 * n), order="F")
     e_fin = o.total_energy(np.asfortranarray(fin_cfg))
     assert abs(float(v["anneal_E"]) * N - e_fin) <= 1e-11 * max(1.0, abs(e_fin))
+
+
+def _setup_both(lattice, n, S, rank, numbers=None, conc=None):
+    lib = C.CDLL(R.DROP_IN)
+    f = lib.drop_in_setup
+    f.restype = C.c_int
+    P = C.POINTER
+    f.argtypes = [C.c_int] * 5 + [P(C.c_int64), P(C.c_double), P(C.c_int8), P(C.c_int8),
+                                  P(C.c_double), P(C.c_double)]
+    m = 2 * n
+    sn = np.zeros(S, dtype=np.int64) if numbers is None else np.asarray(numbers, dtype=np.int64)
+    cc = np.zeros(S) if conc is None else np.asarray(conc, dtype=np.float64)
+    cg = np.zeros(m ** 3, dtype=np.int8)
+    cr = np.zeros(m ** 3, dtype=np.int8)
+    kg, kr = np.zeros(S), np.zeros(S)
+    print("drop_in_setup", lattice, n, S, rank, "numbers" if numbers is not None else "conc", flush=True)
+    rc = f(lattice, n, S, rank, int(numbers is not None), sn.ctypes.data_as(P(C.c_int64)),
+           cc.ctypes.data_as(P(C.c_double)), cg.ctypes.data_as(P(C.c_int8)),
+           cr.ctypes.data_as(P(C.c_int8)), kg.ctypes.data_as(P(C.c_double)),
+           kr.ctypes.data_as(P(C.c_double)))
+    assert rc == 0, f"GPU library error {rc}"
+    shape = (m, m, m)
+    return (cg.reshape(shape, order="F"), cr.reshape(shape, order="F"), kg, kr)
+
+
+@pytest.mark.parametrize("rank", [0, 1, 2, 3])
+def test_fortran_initial_setup_concentrations_path_reproduces_reference_test_02(rank):
+    """The reference's test 02 (bcc 4^3, 4 species at 0.25, static seeds of
+    ranks 0-3) sets up from concentrations held in a 0:S array
+    (io.f90:242): the rebound adaptor reproduces the golden initial
+    configuration of each rank, as does the reference's own routine."""
+    from conftest import load_golden
+    g = load_golden("ref02_parallel_metropolis.npz")
+    n = int(g["n"][0])
+    cg, cr, kg, kr = _setup_both(int(g["lattice"]), n, 4, rank, conc=g["concentrations"])
+    assert np.array_equal(cr, g[f"initial_config{rank}"])
+    assert np.array_equal(cg, g[f"initial_config{rank}"])
+    assert np.array_equal(kg, kr)
+
+
+@pytest.mark.parametrize("lattice,n,S,conc", [(1, 3, 5, [0.2, 0.2, 0.2, 0.2, 0.2]),
+                                              (2, 3, 3, [0.1, 0.35, 0.55])])
+def test_fortran_initial_setup_concentrations_rounding(lattice, n, S, conc):
+    cg, cr, kg, kr = _setup_both(lattice, n, S, 1, conc=conc)
+    assert np.array_equal(cg, cr)
+    assert np.array_equal(kg, kr)
+
+
+@pytest.mark.parametrize("lattice,n,numbers", [(1, 3, [30, 40, 38]), (2, 4, [33, 31, 32, 32])])
+def test_fortran_initial_setup_numbers_path_sets_concentrations(lattice, n, numbers):
+    """On the species-numbers path the reference also stores the REAL(4)
+    concentrations count/n_atoms in setup (initialise.F90:468-472)."""
+    cg, cr, kg, kr = _setup_both(lattice, n, len(numbers), 0, numbers=numbers)
+    assert np.array_equal(cg, cr)
+    assert np.array_equal(kg, kr)
+    assert kg.sum() > 0.99
+
+
+@pytest.mark.parametrize("lattice,n", [(1, 3), (2, 4), (1, 5)])
+def test_fortran_proposal_adaptors_match_reference_draws(lattice, n):
+    """setup%rdm_site, setup%rdm_nbr and wl_params%mc_select (both modes)
+    rebound to the device draw the same sites, from the same stream, as the
+    reference's own procedures."""
+    lib = C.CDLL(R.DROP_IN)
+    f = lib.drop_in_draws
+    f.restype = C.c_int
+    f.argtypes = [C.c_int] * 4 + [C.POINTER(C.c_int)]
+    mism = (C.c_int * 4)()
+    print("drop_in_draws", lattice, n, flush=True)
+    rc = f(lattice, n, 2, 300, mism)
+    assert rc == 0, f"GPU library error {rc}"
+    assert list(mism) == [0, 0, 0, 0]

@@ -629,3 +629,84 @@ def test_parallel_sampler_seed_and_stream_continuation(hip, oracle):
         for out, _, _ in (a1, c1, a2):
             assert np.array_equal(np.bincount(out.ravel(), minlength=S + 1),
                                   np.bincount(cfg.ravel(), minlength=S + 1))
+
+
+@pytest.mark.parametrize("lat,n", [(1, (3, 4, 5)), (2, (4, 2, 3)), (2, (16, 16, 16)), (1, (8, 8, 8))])
+def test_proposal_draws_bit_exact(hip, oracle, lat, n):
+    """rdm_site, rdm_nbr and mc_select (both modes) from the context's
+    MT19937 stream on the device: the sites and the stream position equal
+    the oracle's (which is pinned to the reference's own random_site)."""
+    with hip.Lattice(lat, n, np.zeros((1, 1, 1))) as L:
+        ref = oracle.OracleLattice(lat, n, np.zeros((1, 1, 1)))
+        L.seed(110179 + 22)
+        gen = oracle.Generator(110179 + 22)
+        sites = L.rdm_site(700)
+        assert np.all(sites[:, 0] == 1)
+        assert np.array_equal(sites[:, 1:], np.array([ref.random_site(gen) for _ in range(700)]))
+        for q in range(40):
+            s = ref.random_site(gen)
+            got_s = L.rdm_site(1)[0]
+            assert tuple(got_s[1:]) == s
+            assert tuple(L.rdm_nbr(got_s)[1:]) == ref.random_nbr(gen, s)
+        for nbr in (False, True):
+            pairs = L.select_sites(300, nbr_swap=nbr)
+            for q in range(300):
+                s1 = ref.random_site(gen)
+                s2 = ref.random_nbr(gen, s1) if nbr else ref.random_site(gen)
+                assert tuple(pairs[q, 0, 1:]) == s1 and tuple(pairs[q, 1, 1:]) == s2
+        mt, mti = L.genrand_state()
+        mt_ref, mti_ref = gen.state_arrays()
+        assert mti == mti_ref and np.array_equal(mt, mt_ref)
+        with pytest.raises(hip.BrawlError):
+            L.rdm_nbr([1, 1, 2, 1])   # (0, 1, 0) is not a site of either lattice
+
+
+def test_config1_full_run_bit_exact(hip, oracle):
+    """The small config at its stated length: fcc 8^3, 2 species, 2 shells,
+    600 K, 1000 sweeps of 2048 lattice-wide swap trials through the
+    sequential mode.  The energy after every sweep (the reference's
+    summation order), the accepted count, the final configuration, its
+    integer pair histogram, the RNG position and the derived <E>, C and
+    alpha^(1) equal the oracle's metropolis loop (metropolis.F90:193-437)."""
+    from brawl_amd import workloads
+    c = workloads.config1()
+    ref = oracle.OracleLattice(c.lattice, c.n, c.v_ex)
+    gen = oracle.Generator(110179)
+    cfg = ref.initial_setup(gen, species_numbers=c.counts)
+    N = c.n_atoms
+    out = ref.anneal(cfg, gen, T=c.T, n_mc_steps=1000 * N, n_sample_steps=N, trajectory=True)
+    beta = hip.beta_of(c.T)
+    with hip.Lattice(c.lattice, c.n, c.v_ex) as L:
+        L.seed(110179)
+        L.initial_setup(species_numbers=c.counts)
+        e0 = L.total_energy()
+        _say("config1: 1000 sweeps x 2048 sequential trials")
+        acc, traj = L.mc_steps(beta, 1000 * N, sample_every=N)
+        # the starting energy comes from the integer histogram (1e-12); the
+        # sampled ones are summed in the reference's order (bit-exact)
+        assert abs(e0 - out["trajectory"][0]) <= 1e-12 * abs(out["trajectory"][0])
+        assert np.array_equal(traj, out["trajectory"][1:])
+        assert acc / (1000.0 * N) == out["acceptance"][0]
+        fin = L.get_config()
+        assert np.array_equal(fin, cfg)
+        assert np.array_equal(L.pair_histogram(), ref.histogram(cfg))
+        mt, mti = L.genrand_state()
+        mt_ref, mti_ref = gen.state_arrays()
+        assert mti == mti_ref and np.array_equal(mt, mt_ref)
+        # <E>/N and C as metropolis.F90:414-437 forms them from the samples
+        n_save = 1000
+        step_e = step_esq = 0.0
+        for x in traj:          # left to right, as the reference accumulates
+            step_e += x
+            step_esq += x * x
+        assert step_e / n_save / N == out["energies"][0]
+        sim_temp = c.T * K_B
+        cv = (step_esq / n_save - (step_e / n_save) ** 2) / (sim_temp * c.T) / N
+        assert cv == out["C"][0]
+        # alpha^(1) of the final configuration from the device's radial densities
+        rho = L.radial_densities(3)
+        shells = ref.lattice_shells(cfg, 3)
+        assert np.array_equal(rho, ref.radial_densities(cfg, 3, shells))
+
+
+K_B = 8.167333262e-5 / 13.605693122990

@@ -63,6 +63,25 @@ def test_initial_setup_bit_exact_against_reference(oracle, lat, n, S, Rs, rank):
     assert np.array_equal(cfg, R.initial_setup(lat, n, counts, rank))
 
 
+@pytest.mark.parametrize("lat,n,conc", [
+    (2, (4, 4, 4), [0.25, 0.25, 0.25, 0.25]),          # the reference's test 02
+    (1, (3, 3, 3), [0.2, 0.2, 0.2, 0.2, 0.2]),         # 108 sites: rounding correction
+    (1, (3, 4, 5), [0.1, 0.35, 0.55]),
+    (2, (5, 3, 4), [0.7, 0.3]),
+    (2, (3, 3, 3), [0.15, 0.15, 0.2, 0.2, 0.3]),
+])
+@pytest.mark.parametrize("rank", [0, 2])
+def test_initial_setup_concentrations_path_against_reference(oracle, lat, n, conc, rank):
+    """initial_setup from concentrations (initialise.F90:474-505: nint of
+    REAL(4) n_sites times c, then the round-robin correction), with the
+    reference's array allocated 0:S (io.f90:242)."""
+    o = oracle.OracleLattice(lat, n, np.zeros((len(conc), len(conc), 1)))
+    cfg = o.initial_setup(oracle.Generator(rank=rank), concentrations=np.array(conc))
+    ref = R.initial_setup_conc(lat, n, conc, rank)
+    assert np.array_equal(cfg, ref)
+    assert np.bincount(ref[ref > 0], minlength=len(conc) + 1)[1:].sum() == o.n_atoms
+
+
 @pytest.mark.parametrize("lat,n", [(1, (3, 4, 5)), (2, (4, 2, 3)), (1, (8, 8, 8)), (2, (16, 16, 16))])
 def test_random_sites_against_reference(oracle, lat, n):
     o = oracle.OracleLattice(lat, n, np.zeros((1, 1, 1)))
@@ -71,6 +90,24 @@ def test_random_sites_against_reference(oracle, lat, n):
     ref = R.random_sites(lat, n, 2000, rank=2)
     assert np.all(ref[:, 0] == 1)
     assert np.array_equal(ours, ref[:, 1:])
+
+
+@pytest.mark.parametrize("lat,n", [(1, (3, 4, 5)), (2, (4, 2, 3)), (2, (16, 16, 16))])
+def test_random_nbrs_against_reference(oracle, lat, n):
+    """rdm_site then rdm_nbr of it (select_sites_nbr), wrapping included."""
+    o = oracle.OracleLattice(lat, n, np.zeros((1, 1, 1)))
+    gen = oracle.Generator(rank=1)
+    ours = []
+    for _ in range(1500):
+        s = o.random_site(gen)
+        ours.append((s, o.random_nbr(gen, s)))
+    ref = R.random_nbrs(lat, n, 1500, rank=1)
+    assert np.all(ref[:, :, 0] == 1)
+    assert np.array_equal(np.array(ours, dtype=np.int32), ref[:, :, 1:])
+    # every drawn neighbour is a shell-1 neighbour under the periodic wrap
+    m = np.array([2 * x for x in n])
+    d = (ref[:, 1, 1:] - ref[:, 0, 1:] + m // 2) % m - m // 2
+    assert np.all(np.abs(d).sum(axis=1) == (3 if lat == 2 else 2))
 
 
 @pytest.mark.parametrize("lat,n,S,wc", [(1, (4, 4, 4), 3, 3), (1, (6, 6, 6), 5, 6), (2, (8, 8, 8), 4, 8),
