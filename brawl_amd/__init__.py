@@ -7,7 +7,7 @@ include/brawl_hip.h (libbrawl_hip.so, built in-tree).  This package is the
 host-side mirror of the reference's procedure interface over that ABI.
 """
 from .lattice import (BCC, FCC, K_B_IN_RY, RY_TO_EV, AnnealResult, BrawlError, Lattice,
-                      anneal_batch, beta_of)
+                      anneal_batch, beta_of, warren_cowley)
 
-__all__ = ["Lattice", "BrawlError", "AnnealResult", "anneal_batch", "beta_of", "FCC", "BCC",
-           "K_B_IN_RY", "RY_TO_EV"]
+__all__ = ["Lattice", "BrawlError", "AnnealResult", "anneal_batch", "beta_of", "warren_cowley",
+           "FCC", "BCC", "K_B_IN_RY", "RY_TO_EV"]

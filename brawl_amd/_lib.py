@@ -54,7 +54,8 @@ class AnnealParams(C.Structure):
 
 
 class RunStats(C.Structure):
-    _fields_ = [("trials", C.c_int64), ("accepted", C.c_int64), ("kernel_ms", C.c_double)]
+    _fields_ = [("trials", C.c_int64), ("accepted", C.c_int64), ("kernel_ms", C.c_double),
+                ("e_drift_abs", C.c_double), ("e_drift_rel", C.c_double)]
 
 
 class NsParams(C.Structure):

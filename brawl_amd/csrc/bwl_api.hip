@@ -702,7 +702,7 @@ int anneal_plan(int n, bwl_ctx *const *ctxs, const bwl_anneal_params *params, An
     it.off_muls = total;
     total += 256 * ((it.plan.muls.size() * 4 + 255) / 256);
     it.off_out = total;
-    total += 256 * ((3 * it.T * sizeof(double) + 255) / 256);
+    total += 256 * ((5 * it.T * sizeof(double) + 255) / 256);
     it.off_cnt = total;
     total += 256;
     it.rho_n = p->asro_every > 0 ? (size_t)g.S * g.S * (g.R + 1) * it.T : 0;
@@ -764,6 +764,7 @@ int anneal_upload(AnnealPlan &P, char *base, hipStream_t s, bool zero_by_kernel)
     it.a.energies_of_T = (double *)(base + it.off_out);
     it.a.C_of_T = it.a.energies_of_T + it.T;
     it.a.acc_of_T = it.a.C_of_T + it.T;
+    it.a.e_drift = it.a.acc_of_T + it.T;
     it.a.counters = (long long *)(base + it.off_cnt);
     it.a.rho_of_T = it.rho_n ? (double *)(base + it.off_rho) : nullptr;
     HIP_OK(hipMemcpyAsync(base + it.off_muls, it.plan.muls.data(), it.plan.muls.size() * 4,
@@ -800,8 +801,8 @@ int anneal_collect(const AnnealPlan &P, bwl_ctx *const *ctxs, double *const *ene
   std::vector<long long> cnt(3 * n);
   for (size_t i = 0; i < n; ++i) {
     const AnnealPlan::Item &it = P.items[i];
-    out[i].resize(3 * it.T);
-    HIP_OK(hipMemcpyAsync(out[i].data(), it.a.energies_of_T, 3 * it.T * sizeof(double),
+    out[i].resize(5 * it.T);
+    HIP_OK(hipMemcpyAsync(out[i].data(), it.a.energies_of_T, 5 * it.T * sizeof(double),
                           hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(&cnt[3 * i], it.a.counters, 3 * sizeof(long long), hipMemcpyDeviceToHost, s));
     if (rho_of_T && rho_of_T[i] && it.rho_n)
@@ -819,6 +820,16 @@ int anneal_collect(const AnnealPlan &P, bwl_ctx *const *ctxs, double *const *ene
       stats[i].trials = cnt[3 * i + 2];
       stats[i].accepted = cnt[3 * i + 1];
       stats[i].kernel_ms = kernel_ms;
+      // largest difference between the carried energy and its recount over
+      // the T steps, absolute (Ry) and relative to the recounted energy
+      double da = 0.0, dr = 0.0;
+      for (size_t t = 0; t < T; ++t) {
+        const double d = fabs(out[i][3 * T + t]), e = fabs(out[i][4 * T + t]);
+        da = d > da ? d : da;
+        if (e > 0.0 && d / e > dr) dr = d / e;
+      }
+      stats[i].e_drift_abs = da;
+      stats[i].e_drift_rel = dr;
     }
     ctxs[i]->phase = (unsigned long long)cnt[3 * i];
   }

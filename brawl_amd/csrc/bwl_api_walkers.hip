@@ -322,6 +322,7 @@ int bwl_nested_sampling_batch(int32_t n, bwl_ctx *const *ctxs, const bwl_ns_para
       stats[i].trials = cnt[0];
       stats[i].accepted = cnt[1];
       stats[i].kernel_ms = ms;
+      stats[i].e_drift_abs = stats[i].e_drift_rel = 0.0;
     }
     if (cnt[2] != 0) status = fail(BWL_ERR_STATE, "nested sampling: no second species found for a trial");
   }

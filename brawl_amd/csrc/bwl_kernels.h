@@ -79,6 +79,8 @@ struct ParArgs {
   unsigned long long phase0;  // first round counter (resumable streams)
   // outputs
   double *energies_of_T, *C_of_T, *acc_of_T;
+  double *e_drift;       // 2 x T_steps: per T step, the carried energy minus its recount at
+                         // the step's end, then the recounts
   double *e_trace;       // optional [T_steps][n_sweeps]
   int32_t asro_every;    // sampled sweeps between radial-density samples (0: none)
   double *rho_of_T;      // (S, S, R+1, T_steps) when asro_every > 0

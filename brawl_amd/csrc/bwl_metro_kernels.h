@@ -37,7 +37,8 @@
 // in FP64 from the count differences:
 //   dE = sum_n sum_s (c_a[n][s] - c_b[n][s]) * (V(beta,s,n) - V(alpha,s,n))
 // (exact because a and b are not neighbours).  The energy is recounted from
-// the lattice at the start of every temperature step and carried through
+// the lattice at the end of every temperature step (and before the first)
+// and carried through
 // the sweeps by summing the accepted dE.
 #pragma once
 
@@ -657,11 +658,8 @@ __global__ __launch_bounds__(NC != 0 && sizeof(CW) == 4 ? kMaxNatThreads : 512) 
   unsigned long long phase = C.phase0;
   long long acc_total = 0, trials_total = 0;
 
-  for (int t = 0; t < C.T_steps; ++t) {
-    const double temp = C.T0 + (double)t * C.dT;
-    const double sim_temp = temp * kKbInRyPar;
-    const double beta = 1.0 / sim_temp;
-    // energy recount at the start of the temperature step
+  // Energy of the lattice recounted from its shell counts (all threads).
+  auto recount = [&]() -> double {
     double part = 0.0;
     const int nsites = N.D0 * N.D1 * N.D2;
     for (int i = active ? tid : nsites; i < nsites; i += nthr) {
@@ -682,7 +680,17 @@ __global__ __launch_bounds__(NC != 0 && sizeof(CW) == 4 ? kMaxNatThreads : 512) 
         for (int s = 0; s < S; ++s) part += (double)((cnt[n] >> (FIELD * s)) & ((1u << FIELD) - 1)) * vr[s];
       }
     }
-    double E = 0.5 * block_sum(part, red);
+    return 0.5 * block_sum(part, red);
+  };
+  // The energy is recounted before the first temperature step and at the end
+  // of every step; between recounts it is carried as the sum of the accepted
+  // dE, and the carried value's difference from the recount is reported.
+  double E = recount();
+
+  for (int t = 0; t < C.T_steps; ++t) {
+    const double temp = C.T0 + (double)t * C.dT;
+    const double sim_temp = temp * kKbInRyPar;
+    const double beta = 1.0 / sim_temp;
     double step_E = 0.0, step_Esq = 0.0;
     unsigned long long acc_local = 0;
     long long trials_sampled = 0;
@@ -850,6 +858,12 @@ __global__ __launch_bounds__(NC != 0 && sizeof(CW) == 4 ? kMaxNatThreads : 512) 
       C.acc_of_T[t] = (double)acc_sum / (double)trials_sampled;
       acc_total += (long long)acc_sum;
     }
+    const double Er = recount();
+    if (tid == 0) {
+      C.e_drift[t] = E - Er;
+      C.e_drift[C.T_steps + t] = Er;
+    }
+    E = Er;
   }
   __syncthreads();
   nat_unstage<LAT>(lat, C.compact, g, N, FIELD);
@@ -903,10 +917,8 @@ __global__ __launch_bounds__(512) void k_metro_cmp(const ParArgs *__restrict__ c
   long long acc_total = 0, trials_total = 0;
   const int cells = g.n[0] * g.n[1] * g.n[2];
 
-  for (int t = 0; t < C.T_steps; ++t) {
-    const double temp = C.T0 + (double)t * C.dT;
-    const double sim_temp = temp * kKbInRyPar;
-    const double beta = 1.0 / sim_temp;
+  // Energy of the lattice recounted from its shell counts (all threads).
+  auto recount = [&]() -> double {
     double part = 0.0;
     for (int i = active ? tid : g.nsub * cells; i < g.nsub * cells; i += nthr) {
       int s = i / cells;
@@ -926,7 +938,16 @@ __global__ __launch_bounds__(512) void k_metro_cmp(const ParArgs *__restrict__ c
           part += (double)((cnt[n] >> (FIELD * b)) & ((1 << FIELD) - 1)) * va[b];
       }
     }
-    double E = 0.5 * block_sum(part, red);
+    return 0.5 * block_sum(part, red);
+  };
+  // recounted before the first step and at the end of every step (the
+  // carried value's difference from the recount is reported)
+  double E = recount();
+
+  for (int t = 0; t < C.T_steps; ++t) {
+    const double temp = C.T0 + (double)t * C.dT;
+    const double sim_temp = temp * kKbInRyPar;
+    const double beta = 1.0 / sim_temp;
     double step_E = 0.0, step_Esq = 0.0;
     unsigned long long acc_local = 0;
     long long trials_sampled = 0;
@@ -1022,6 +1043,12 @@ __global__ __launch_bounds__(512) void k_metro_cmp(const ParArgs *__restrict__ c
       C.acc_of_T[t] = (double)acc_sum / (double)trials_sampled;
       acc_total += (long long)acc_sum;
     }
+    const double Er = recount();
+    if (tid == 0) {
+      C.e_drift[t] = E - Er;
+      C.e_drift[C.T_steps + t] = Er;
+    }
+    E = Er;
   }
   __syncthreads();
   unstage_lattice(lat, C.compact, g, FIELD);

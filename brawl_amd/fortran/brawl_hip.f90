@@ -37,6 +37,7 @@ module brawl_hip
   type, bind(C) :: bwl_run_stats
     integer(c_int64_t) :: trials, accepted
     real(c_double) :: kernel_ms
+    real(c_double) :: e_drift_abs, e_drift_rel
   end type bwl_run_stats
 
   type, bind(C) :: bwl_ns_params

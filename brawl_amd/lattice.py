@@ -54,6 +54,8 @@ class AnnealResult:
     accepted: int
     kernel_ms: float
     rho_of_T: np.ndarray = None  # (S, S, R+1, T_steps) radial densities, if sampled
+    e_drift_abs: float = 0.0     # carried energy vs FP64 recounts: largest |difference| (Ry)
+    e_drift_rel: float = 0.0     # ... relative to the recounted energy
 
 
 def warren_cowley(rho: np.ndarray, shell_sizes, concentrations) -> np.ndarray:
@@ -260,7 +262,8 @@ class Lattice:
         else:
             check(self._L.bwl_metropolis_anneal(self._h, C.byref(p), _p(e, C.c_double),
                                                 _p(c, C.c_double), _p(a, C.c_double), C.byref(st)))
-        return AnnealResult(e, c, a, int(st.trials), int(st.accepted), float(st.kernel_ms), rho)
+        return AnnealResult(e, c, a, int(st.trials), int(st.accepted), float(st.kernel_ms), rho,
+                            float(st.e_drift_abs), float(st.e_drift_rel))
 
 
 def mc_steps_batch(lattices, betas, n_trials: int, *, nbr_swap=False, sample_every=0):
@@ -324,7 +327,8 @@ def anneal_batch(lattices, params):
     st = (RunStats * n)()
     mk = C.c_double()
     check(L.bwl_metropolis_anneal_batch(n, H, P, E, Cv, A, st, C.byref(mk)))
-    res = [AnnealResult(o[0], o[1], o[2], int(s.trials), int(s.accepted), float(s.kernel_ms))
+    res = [AnnealResult(o[0], o[1], o[2], int(s.trials), int(s.accepted), float(s.kernel_ms), None,
+                        float(s.e_drift_abs), float(s.e_drift_rel))
            for o, s in zip(outs, st)]
     return res, float(mk.value)
 

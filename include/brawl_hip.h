@@ -150,9 +150,10 @@ int bwl_mc_steps_batch(int32_t n, bwl_ctx *const *ctxs, const double *beta, int3
  * pair trials (sites of a round are further apart than the interaction
  * range); proposals and acceptances use Philox4x32-10 keyed by `seed`,
  * continuing the context's stream position across calls.  The energy is
- * recounted from the lattice at the start of each T step and carried
- * through the sweeps as the sum of the accepted dE, sampled after every
- * sweep; outputs per T step follow metropolis.F90:414-437 (<E>/N, C,
+ * recounted from the lattice before the first T step and at the end of
+ * each, and carried through the sweeps as the sum of the accepted dE,
+ * sampled after every sweep (stats: the carried value's largest difference
+ * from the recounts); outputs per T step follow metropolis.F90:414-437 (<E>/N, C,
  * acceptance rate with equal-species trials counted as accepted). */
 typedef struct {
   double T;            /* start temperature (K) */
@@ -169,6 +170,11 @@ typedef struct {
   int64_t trials;        /* swap trials made (all T steps) */
   int64_t accepted;      /* accepted, equal species counted as accepted */
   double kernel_ms;      /* device time of the sampler kernel */
+  /* Parallel sampler only (0 elsewhere): the carried energy (recount at the
+   * start, plus the accepted dE) against an FP64 recount of the lattice at
+   * the end of every T step, the largest |difference| in Ry and the largest
+   * |difference| / |recount|. */
+  double e_drift_abs, e_drift_rel;
 } bwl_run_stats;
 
 int bwl_metropolis_anneal(bwl_ctx *ctx, const bwl_anneal_params *p,

@@ -124,6 +124,15 @@ class Generator:
         C.memmove(C.byref(g.state), C.byref(self.state), C.sizeof(MT))
         return g
 
+    @classmethod
+    def from_state(cls, mt, mti) -> "Generator":
+        """A stream at a given position: the 624 words and the index."""
+        g = cls.__new__(cls)
+        g.state = MT()
+        g.state.mt[:] = [int(x) for x in np.asarray(mt, dtype=np.uint32)]
+        g.state.mti = int(mti)
+        return g
+
     def state_arrays(self):
         return np.frombuffer(bytes(self.state.mt), dtype=np.uint32).copy(), int(self.state.mti)
 

@@ -710,3 +710,72 @@ def test_config1_full_run_bit_exact(hip, oracle):
 
 
 K_B = 8.167333262e-5 / 13.605693122990
+
+
+def _z(a, b):
+    """(mean a - mean b) / standard error of the difference (Welch)."""
+    a, b = np.asarray(a), np.asarray(b)
+    se = np.sqrt(a.var(axis=0, ddof=1) / len(a) + b.var(axis=0, ddof=1) / len(b))
+    d = a.mean(axis=0) - b.mean(axis=0)
+    return np.where(se > 0, d / np.where(se > 0, se, 1.0), np.where(d == 0, 0.0, np.inf))
+
+
+def test_config2_anneal_statistics_over_seeds(hip):
+    """The headline config (bcc 32^3, 4 species, 4 shells, 3000 -> 100 K in
+    50 steps of 200 sweeps) through the parallel sampler, 16 independent
+    seeds, against the reference's sequential sampler over 8 seeds
+    (tests/golden/config2_stats.npz, from the oracle's restatement of
+    metropolis.F90:193-437; tests/golden/make_config2_stats.py).  At 3000 K,
+    at the heat-capacity peak and at 100 K: <E>/N and C within 2 sigma each
+    (sigma: standard error of the difference of the means); the
+    Warren-Cowley alpha^(n)(a, b), n = 1..4, a <= b (120 values), at least
+    90 % within 2 sigma and all within 3.5 sigma.  Every anneal's carried
+    energy stays within 1e-12 (relative) of the FP64 recount at the end of
+    each temperature step."""
+    import os
+
+    from conftest import GOLDEN
+    from brawl_amd import warren_cowley, workloads
+    path = os.path.join(GOLDEN, "config2_stats.npz")
+    if not os.path.exists(path):
+        pytest.skip("config2_stats.npz not generated")
+    ref = dict(np.load(path))
+    c = workloads.config2()
+    S, R = 4, 4
+    conc = np.full(S, 0.25)
+    shells = [8, 6, 12, 24]
+    gpu_E, gpu_C, gpu_a = [], [], []
+    for k in range(16):
+        with hip.Lattice(c.lattice, c.n, c.v_ex) as L:
+            L.seed(110179 + 11 * (100 + k))
+            L.initial_setup(species_numbers=c.counts)
+            res = L.metropolis_simulated_annealing(
+                T=c.T, delta_T=c.delta_T, T_steps=c.T_steps, n_sweeps=c.n_sweeps,
+                seed=workloads.derive_seed(workloads.MASTER_SEED, 900 + k), asro_every=20)
+        _say(f"seed {k}: {res.kernel_ms:.0f} ms, drift {res.e_drift_abs:.2e} Ry "
+             f"({res.e_drift_rel:.2e} relative)")
+        assert res.e_drift_rel <= 1e-12
+        gpu_E.append(res.energies_of_T)
+        gpu_C.append(res.C_of_T)
+        gpu_a.append(warren_cowley(res.rho_of_T, shells, conc))
+    cpu_a = [warren_cowley(r, shells, conc) for r in ref["rho_of_T"]]
+    Ts = ref["T"]
+    t_peak = int(np.argmax(ref["C_of_T"].mean(axis=0)))
+    iu = np.triu_indices(S)
+    za = []
+    for t in (0, t_peak, len(Ts) - 1):
+        zE = _z(np.array(gpu_E)[:, t], ref["energies_of_T"][:, t])
+        zC = _z(np.array(gpu_C)[:, t], ref["C_of_T"][:, t])
+        g = np.array([a[:, :, 1:, t][iu] for a in gpu_a])     # (seeds, 10 pairs, 4 shells)
+        r = np.array([a[:, :, 1:, t][iu] for a in cpu_a])
+        zt = _z(g, r).ravel()
+        za.extend(zt)
+        _say(f"T = {Ts[t]:.1f} K: <E>/N gpu {np.mean(np.array(gpu_E)[:, t]):.6e} "
+             f"ref {ref['energies_of_T'][:, t].mean():.6e} (z {zE:+.2f}); "
+             f"C gpu {np.mean(np.array(gpu_C)[:, t]):.4e} ref {ref['C_of_T'][:, t].mean():.4e} "
+             f"(z {zC:+.2f}); alpha z in [{zt.min():+.2f}, {zt.max():+.2f}]")
+        assert abs(zE) <= 2.0 and abs(zC) <= 2.0, (Ts[t], zE, zC)
+    za = np.abs(np.array(za))
+    _say(f"alpha: {np.mean(za <= 2.0):.3f} within 2 sigma, largest {za.max():.2f}")
+    assert np.mean(za <= 2.0) >= 0.9
+    assert za.max() <= 3.5

@@ -71,13 +71,14 @@ def _solo(hip, r):
         with Lattice(r["lat"], r["n"], r["v"]) as L:
             L.seed(r["seed"])
             lim, st = nested_sampling.nested_sampling(L, n_walkers=r["n_walkers"], n_steps=r["n_steps"],
-                                                      n_iter=r["n_iter"], concentrations=r["conc"])
+                                                      n_iter=r["n_iter"], concentrations=r.get("conc"),
+                                                      species_numbers=r.get("sn"))
             return dict(e_limits=lim, trials=st.trials)
     wl = wang_landau.WangLandau(r["lat"], r["n"], r["v"], bins=r["bins"], num_windows=r["num_windows"],
                                 walkers_per_window=r["walkers_per_window"], bin_overlap=r["bin_overlap"],
                                 energy_min=r["energy_min"], energy_max=r["energy_max"],
                                 mc_sweeps=r["mc_sweeps"], rank_offset=r["rank_offset"])
-    wl.initial_setup(concentrations=r["conc"])
+    wl.initial_setup(concentrations=r.get("conc"), species_numbers=r.get("sn"))
     wl.sweeps(r["wl_f"], average=True)
     lng, hist = wl.lng_hist()
     out = dict(lng=lng[:: r["walkers_per_window"]].copy(), hist=hist[:: r["walkers_per_window"]].copy(),
@@ -92,12 +93,13 @@ def _submit(srv, r):
                                  seed=r["seed"], want_config=True)
     if r["kind"] == "ns":
         return srv.submit_ns(r["lat"], r["n"], r["v"], r["seed"], n_walkers=r["n_walkers"],
-                             n_steps=r["n_steps"], n_iter=r["n_iter"], concentrations=r["conc"])
+                             n_steps=r["n_steps"], n_iter=r["n_iter"], concentrations=r.get("conc"),
+                             species_numbers=r.get("sn"))
     return srv.submit_wl(r["lat"], r["n"], r["v"], bins=r["bins"], num_windows=r["num_windows"],
                          walkers_per_window=r["walkers_per_window"], bin_overlap=r["bin_overlap"],
                          energy_min=r["energy_min"], energy_max=r["energy_max"],
                          mc_sweeps=r["mc_sweeps"], wl_f=r["wl_f"], rank_offset=r["rank_offset"],
-                         concentrations=r["conc"])
+                         concentrations=r.get("conc"), species_numbers=r.get("sn"))
 
 
 def _check_same(r, got, solo):
@@ -150,3 +152,54 @@ def test_server_rejects_oversized_requests(hip):
                           concentrations=np.full(5, 0.2))
         assert e.value.code == -4
         assert srv.memory() == (0, 1 << 20)
+
+
+def _from_served(q):
+    """A request of the served-stream workload as the helpers above take it."""
+    r = dict(kind=q.kind, lat=q.lattice, n=q.n, v=q.v_ex, seed=q.seed)
+    if q.kind == "anneal":
+        r.update(cfg=q.config, T=q.params["T"], n_sweeps=q.params["n_sweeps"])
+    elif q.kind == "ns":
+        r.update(sn=q.counts, **q.params)
+    else:
+        r.update(sn=q.counts, **q.params)
+    return r
+
+
+def test_served_mix_at_stated_sizes(hip, oracle):
+    """Requests of the served mixed stream at the workload's own sizes
+    (anneals of fcc 8^3-28^3 / bcc 8^3-32^3 with 2-5 species and 100-500
+    sweeps; quinary fcc 6^3 nested sampling with 128-640 walkers, walk
+    length 100, 200 iterations; Wang-Landau runs of bcc 6^3-10^3 with 4-8
+    walkers over 20 sweeps), admitted as they arrive under a budget that
+    holds a few of them at a time: each result equals the same run made
+    alone, and every nested-sampling run's 200 energy limits equal the
+    oracle's run from the same MT19937 seed."""
+    from brawl_amd import workloads
+    from brawl_amd.serve import Server
+    stream = workloads.config5(n_requests=36)
+    kinds = [q.kind for q in stream]
+    _say("served mix", {k: kinds.count(k) for k in set(kinds)})
+    assert {"anneal", "ns", "wl"} <= set(kinds)
+    reqs = [_from_served(q) for q in stream]
+    solo = [_solo(hip, r) for r in reqs]
+    with Server(device_bytes=48 << 20, max_launches=4, batch_wait_us=2000) as srv:
+        ids = []
+        for k, r in enumerate(reqs):
+            ids.append(_submit(srv, r))
+            srv.progress()
+        srv.drain()
+        assert sorted(srv.poll()) == sorted(ids)
+        assert srv.memory()[0] == 0
+        for r, rid, s in zip(reqs, ids, solo):
+            got = srv.result(rid)
+            assert got["state"] == "done", got
+            _check_same(r, got, s)
+    for r, rid, s in zip(reqs, ids, solo):
+        if r["kind"] != "ns":
+            continue
+        ref = oracle.OracleLattice(r["lat"], r["n"], r["v"])
+        _say("ns vs oracle", r["n_walkers"], "walkers")
+        lim = ref.nested_sampling(oracle.Generator(r["seed"]), n_walkers=r["n_walkers"],
+                                  n_steps=r["n_steps"], n_iter=r["n_iter"], species_numbers=r["sn"])
+        assert np.array_equal(s["e_limits"], lim)
