@@ -38,7 +38,7 @@ EXPORTS = (
     "bwl_server_submit_ns", "bwl_server_submit_wl", "bwl_server_progress", "bwl_server_poll",
     "bwl_server_info", "bwl_server_drain", "bwl_server_memory", "bwl_lds_bandwidth",
     "bwl_set_layout", "bwl_layouts", "bwl_layout_info", "bwl_mc_steps_batch",
-    "bwl_random_site", "bwl_random_nbr", "bwl_select_sites",
+    "bwl_random_site", "bwl_random_nbr", "bwl_select_sites", "bwl_metropolis_anneal_noalloc",
 )
 
 
@@ -124,6 +124,7 @@ def load():
         "bwl_select_sites": (C.c_int, [vp, C.c_int32, C.c_int32, i32p]),
         "bwl_mc_steps": (C.c_int, [vp, C.c_double, C.c_int32, C.c_int64, C.c_int64, dp, i64p]),
         "bwl_metropolis_anneal": (C.c_int, [vp, P(AnnealParams), dp, dp, dp, P(RunStats)]),
+        "bwl_metropolis_anneal_noalloc": (C.c_int, [vp, P(AnnealParams), dp, dp, dp, P(RunStats)]),
         "bwl_metropolis_anneal_batch": (C.c_int, [C.c_int32, P(vp), P(AnnealParams), P(dp), P(dp),
                                                   P(dp), P(RunStats), dp]),
         "bwl_metropolis_anneal_rho": (C.c_int, [vp, P(AnnealParams), dp, dp, dp, dp, P(RunStats)]),

@@ -194,6 +194,21 @@ int bwl_metropolis_anneal_rho(bwl_ctx *ctx, const bwl_anneal_params *p, double *
                               double *C_of_T, double *acceptance_of_T, double *rho_of_T,
                               bwl_run_stats *stats);
 
+/* No counterpart in the reference: bwl_metropolis_anneal for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops that forbid hipMalloc).  Same arguments, kernels and results as
+ * bwl_metropolis_anneal on the same context, seed and stream position; the
+ * call's scratch (round multipliers, per-T outputs, counters, launch
+ * arguments) is a 1 MiB array in device memory that the library holds per
+ * GPU until the process ends, so it makes no hipMalloc, hipFree or other
+ * allocating call.  Inputs whose scratch exceeds 1 MiB (about 25,000 T
+ * steps) fail with BWL_ERR_ARG before anything runs.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (from several host
+ * threads or contexts at once) race; calls made one after another do not. */
+int bwl_metropolis_anneal_noalloc(bwl_ctx *ctx, const bwl_anneal_params *p,
+                                  double *energies_of_T, double *C_of_T,
+                                  double *acceptance_of_T, bwl_run_stats *stats);
+
 /* Layout of the parallel sampler's lattice in LDS, the tunable part of the
  * kernel choice (no counterpart in the reference).  BWL_LAYOUT_AUTO (the
  * default) takes the first of FIXED, NATURAL, COMPACT that exists for the
