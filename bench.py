@@ -3,9 +3,10 @@
 Workload (one "step"): the quaternary bcc anneal -- bcc 32^3 conventional
 cells (65536 sites), 4 equiatomic species, 4 shells with V^(n) ~ N(0, 30/n
 meV) (zero-mean per shell), T from 3000 K to 100 K in 50 linear steps of 200
-sweeps each, lattice-wide swaps; the energy is recounted from the lattice at
-the start of every T step and carried through the sweeps as the sum of the
-accepted dE, sampled after every sweep.  One step = the whole anneal
+sweeps each, lattice-wide swaps; the energy is recounted from the lattice
+before the first T step and at the end of each (the carried value's largest
+difference from the recounts is reported) and carried through the sweeps as
+the sum of the accepted dE, sampled after every sweep.  One step = the whole anneal
 (>= 655.36 M swap trials) from the same synthetic start, on an HBM-resident
 lattice, through the C-ABI.
 
@@ -155,6 +156,121 @@ def cpu_baseline(chain, budget_s: float):
             "port": port}
 
 
+def _wl_walker_sample(k: int, n_calls: int):
+    """One config-4 walker of the middle window through the reference's own
+    compiled `sweeps` loop (ref_wl_sweeps: rdm_site, pair_energy, pair_swap,
+    bin_index, log(genrand()) as wang-landau.F90:539-632, full_energy at the
+    start of every call): entered into its window on the oracle first
+    (untimed), then n_calls calls of one sweep timed around the call.
+    Returns (trials, seconds)."""
+    from brawl_amd import workloads
+    from oracle import oracle as orc
+    from oracle import ref as oref
+    c = workloads.config4()
+    lat = orc.OracleLattice(c["lattice"], c["n"], c["v_ex"])
+    windows, edges = lat.wl_setup(c["bins"], c["num_windows"], c["bin_overlap"], c["energy_min"],
+                                  c["energy_max"])
+    s, e = (int(x) for x in windows[c["num_windows"] // 2])
+    gen = orc.Generator(rank=k)
+    cfg = lat.initial_setup(gen, species_numbers=c["counts"])
+    lat.wl_enter_window(cfg, gen, edges[s - 1], edges[e], c["energy_min"], c["energy_max"],
+                        species_numbers=c["counts"])
+    lng, hist = np.zeros(c["bins"]), np.zeros(e - s + 1)
+    t0 = time.perf_counter()
+    oref._wl_sweeps(c["lattice"], c["n"], c["v_ex"], cfg, rank=k, nbr_swap=0, mc_sweeps=1,
+                    n_calls=n_calls, bin_edges=edges, start_idx=s, end_idx=e, wl_f=1.0, lng=lng,
+                    hist=hist)
+    return n_calls * lat.n_atoms, time.perf_counter() - t0
+
+
+def ns_request_shape():
+    """The nested-sampling class of the served stream: quinary fcc 6^3 (864
+    sites), 4 shells of 30/n meV, walk length 100; 384 walkers (the middle of
+    its 128-640)."""
+    from brawl_amd import workloads
+    q = next(r for r in workloads.config5(n_requests=64) if r.kind == "ns")
+    return q, 384
+
+
+def _ns_sample(k: int, n_iter: int):
+    """One nested-sampling run of that shape through the reference's own
+    compiled routines (ref_nested_sampling: initial_setup, full_energy,
+    rdm_site, pair_energy, pair_swap, genrand in the order of
+    nested_sampling.f90:78-200), its iteration loop timed.  Returns
+    (walk trials, seconds)."""
+    from oracle import ref as oref
+    q, walkers = ns_request_shape()
+    _, trials, secs = oref._nested_sampling(q.lattice, q.n, q.v_ex, q.counts, rank=k,
+                                            n_walkers=walkers, n_steps=100, n_iter=n_iter)
+    return trials, secs
+
+
+def cpu_walkers(budget_s: float):
+    """The reference's walker loops on the host cores, one chain per core
+    (the reference runs one walker per MPI rank): Wang-Landau `sweeps` on a
+    config-4 walker and the nested-sampling loop on the served stream's NS
+    shape, each on 1 core and on every available core at once (up to 16,
+    the host share of one GPU).  None without the reference build."""
+    from oracle import ref as oref
+    if not oref.available():
+        return None
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    out = {}
+    # Wang-Landau: calibrate with one call, then ~budget/4 seconds per leg
+    tr, dt = oref.run_parallel(_wl_walker_sample, [(0, 2)])[0]
+    calls = max(2, int(0.25 * budget_s / max(dt / 2, 1e-6)))
+    tr1, dt1 = oref.run_parallel(_wl_walker_sample, [(0, calls)])[0]
+    many = oref.run_parallel(_wl_walker_sample, [(k, calls) for k in range(cores)])
+    out["wang_landau"] = {
+        "value": tr1 / dt1, "unit": "swap trials/s", "cores": 1, "kind": "reference",
+        "all_cores": {"value": sum(t for t, _ in many) / max(d for _, d in many), "cores": cores},
+        "sample": f"one config-4 walker (bcc 16^3, 2 shells, middle window of 16) through the "
+                  f"reference's compiled sweeps loop, {calls} calls of 8192 trials; all cores: "
+                  f"{cores} walkers (ranks 0..{cores - 1}) at once"}
+    # nested sampling
+    tr, dt = oref.run_parallel(_ns_sample, [(0, 20)])[0]
+    iters = max(20, int(0.25 * budget_s / max(dt / 20, 1e-6)))
+    tr1, dt1 = oref.run_parallel(_ns_sample, [(0, iters)])[0]
+    many = oref.run_parallel(_ns_sample, [(k, iters) for k in range(cores)])
+    out["nested_sampling"] = {
+        "value": tr1 / dt1, "unit": "swap trials/s", "cores": 1, "kind": "reference",
+        "all_cores": {"value": sum(t for t, _ in many) / max(d for _, d in many), "cores": cores},
+        "sample": f"quinary fcc 6^3, 4 shells, 384 walkers, walk length 100, {iters} iterations "
+                  f"through the reference's compiled routines; all cores: {cores} runs at once"}
+    return out
+
+
+def nested_sampling_runs(n_runs: int = 64, n_iter: int = 200):
+    """Nested-sampling runs of the served stream's NS shape on the device:
+    one run alone (one wavefront), then n_runs runs (their own seeds) in
+    one batched launch."""
+    from brawl_amd import Lattice, nested_sampling
+    q, walkers = ns_request_shape()
+    p = dict(n_walkers=walkers, n_steps=100, n_iter=n_iter, species_numbers=q.counts)
+    with Lattice(q.lattice, q.n, q.v_ex) as L:
+        L.seed(110179)
+        nested_sampling.nested_sampling(L, **p)                 # warm-up
+        L.seed(110190)
+        _, st = nested_sampling.nested_sampling(L, **p)
+    one = st.trials / (st.kernel_ms / 1e3)
+    lats = []
+    for k in range(n_runs):
+        Lk = Lattice(q.lattice, q.n, q.v_ex)
+        Lk.seed(110179 + 11 * k)
+        lats.append(Lk)
+    res, mk = nested_sampling.nested_sampling_batch(lats, [p] * n_runs)
+    for Lk in lats:
+        Lk.close()
+    trials = sum(s.trials for _, s in res)
+    log(f"[bench] nested sampling: one run {one:.3e} trials/s; {n_runs} runs {trials} trials in "
+        f"{mk:.1f} ms")
+    return {"workload": f"quinary fcc 6^3 (864 sites), 4 shells, {walkers} walkers, walk length 100, "
+                        f"{n_iter} iterations",
+            "one_run": {"value": one, "unit": "swap trials/s", "us_per_trial": 1e6 / one},
+            "batched": {"runs": n_runs, "trials": trials, "makespan_ms": mk,
+                        "value": trials / (mk / 1e3), "unit": "swap trials/s"}}
+
+
 def batched(steps: int, warmup: int, tune: bool = True):
     """The batched form: 256 uneven Metropolis chains (fcc/bcc, 8^3..24^3
     cells, 2-5 species with Dirichlet(1) counts, 3 shells, T in [200,2000] K,
@@ -285,12 +401,10 @@ def _plain_run(r):
     return out, trials, dt, k_ms / 1e3
 
 
-def served(args):
-    """Config 5 through the request server, then the same requests plain."""
-    from brawl_amd import workloads
+def _serve_stream(args, reqs):
+    """Submit reqs at their arrival times to a fresh server, drain it;
+    returns (results, submission delays after arrival, wall s, peak bytes)."""
     from brawl_amd.serve import Server
-    reqs = workloads.config5(args.served_requests, args.served_rate)
-    log(f"[bench] served: {len(reqs)} requests at {args.served_rate:g}/s")
     with Server(device_bytes=args.served_budget_mb << 20, max_launches=args.served_slots,
                 batch_wait_us=args.served_wait_us) as srv:
         ids, late = [], []
@@ -305,10 +419,10 @@ def served(args):
         wall = time.perf_counter() - t0
         res = [srv.result(i) for i in ids]
         peak_mem = srv.memory()[1]
-    trials = sum(x["trials"] for x in res)
-    first = min(x["t_submit_ms"] for x in res)
-    last = max(x["t_done_ms"] for x in res)
-    makespan_s = (last - first) / 1e3
+    return res, late, wall, peak_mem
+
+
+def _latency_by_class(reqs, res, late):
     classes = {}
     for r, x, lt in zip(reqs, res, late):
         lat_ms = x["t_done_ms"] - x["t_submit_ms"] + lt * 1e3
@@ -317,6 +431,56 @@ def served(args):
         c["k"].append(x["kernel_ms"])
         c["trials"] += x["trials"]
         c["n"] += 1
+    return classes
+
+
+def served_sweep(args, rates=(200.0, 400.0, 800.0, 1600.0, 3200.0, 6400.0)):
+    """The served stream at increasing Poisson arrival rates (the same
+    request mix, args.served_requests requests each): aggregate swap
+    trials/s over the makespan and per-class p50/p99 latency from arrival.
+    A rate is sustained while the server keeps up: the requests arriving in
+    the second half wait no longer (p99) than twice those of the first half;
+    past that, a backlog builds and latency grows with the length of the
+    stream."""
+    from brawl_amd import workloads
+    out = []
+    for lam in rates:
+        reqs = workloads.config5(args.served_requests, lam)
+        res, late, wall, _ = _serve_stream(args, reqs)
+        trials = sum(x["trials"] for x in res)
+        first = min(x["t_submit_ms"] for x in res)
+        last = max(x["t_done_ms"] for x in res)
+        lat = np.array([x["t_done_ms"] - x["t_submit_ms"] + lt * 1e3 for x, lt in zip(res, late)])
+        half = len(lat) // 2
+        p99_a, p99_b = float(np.percentile(lat[:half], 99)), float(np.percentile(lat[half:], 99))
+        classes = _latency_by_class(reqs, res, late)
+        row = {"rate_hz": lam, "offered_trials_per_s": trials / reqs[-1].arrival_s,
+               "value": trials / ((last - first) / 1e3), "makespan_ms": last - first,
+               "p99_first_half_ms": p99_a, "p99_second_half_ms": p99_b,
+               "sustained": p99_b <= 2.0 * p99_a,
+               "per_class": {k: {"p50_ms": float(np.percentile(c["lat"], 50)),
+                                 "p99_ms": float(np.percentile(c["lat"], 99))}
+                             for k, c in classes.items()}}
+        log(f"[bench] served sweep {lam:g}/s: {row['value']:.3e} trials/s, p99 {p99_a:.0f} / "
+            f"{p99_b:.0f} ms, sustained {row['sustained']}")
+        out.append(row)
+    sustained = [r["rate_hz"] for r in out if r["sustained"]]
+    return {"requests_per_rate": args.served_requests, "rates": out,
+            "max_sustained_rate_hz": max(sustained) if sustained else None,
+            "peak_value": max(r["value"] for r in out)}
+
+
+def served(args):
+    """Config 5 through the request server, then the same requests plain."""
+    from brawl_amd import workloads
+    reqs = workloads.config5(args.served_requests, args.served_rate)
+    log(f"[bench] served: {len(reqs)} requests at {args.served_rate:g}/s")
+    res, late, wall, peak_mem = _serve_stream(args, reqs)
+    trials = sum(x["trials"] for x in res)
+    first = min(x["t_submit_ms"] for x in res)
+    last = max(x["t_done_ms"] for x in res)
+    makespan_s = (last - first) / 1e3
+    classes = _latency_by_class(reqs, res, late)
     log(f"[bench] served: {trials} trials, makespan {makespan_s * 1e3:.1f} ms, wall {wall * 1e3:.1f} ms")
     # plain: one request at a time, and the served results checked against it
     plain_s, plain_dev_s, plain_trials, mismatch = 0.0, 0.0, 0, 0
@@ -503,7 +667,9 @@ def main():
     from brawl_amd import _lib
 
     if args.only_served:
-        print(json.dumps({"served": served(args)}))
+        sv = served(args)
+        sv["saturation"] = served_sweep(args)
+        print(json.dumps({"served": sv}))
         return
     if args.wl_run is not None:
         print(json.dumps({"wang_landau_run": wang_landau_run(args.wl_run)}))
@@ -517,6 +683,13 @@ def main():
         cpu = cpu_baseline(chain, args.cpu_budget)
         log(f"[bench] cpu baseline: {cpu['value']:.3e} trials/s ({cpu['kind']}), "
             f"{cpu['replicas']['value']:.3e} with {cpu['replicas']['cores']} chains")
+        walkers_cpu = cpu_walkers(args.cpu_budget)
+        if walkers_cpu:
+            cpu["walkers"] = walkers_cpu
+            log(f"[bench] cpu walkers: WL {walkers_cpu['wang_landau']['value']:.3e} / "
+                f"{walkers_cpu['wang_landau']['all_cores']['value']:.3e}, NS "
+                f"{walkers_cpu['nested_sampling']['value']:.3e} / "
+                f"{walkers_cpu['nested_sampling']['all_cores']['value']:.3e} trials/s")
     # start configuration: equiatomic random occupation (reference initial_setup)
     lat = Lattice(chain.lattice, chain.n, chain.v_ex)
     lat.seed(110179)
@@ -572,13 +745,22 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s",
                      "frac": achieved / peak, "traffic": TRAFFIC_BYTES,
                      "kernel": PLAIN_KERNEL, "kernel_ms": float(np.mean(kernel_ms)),
-                     "bytes_per_launch": trials_per_launch * bpt},
+                     "bytes_per_launch": trials_per_launch * bpt,
+                     "limited_by": "VALU and LDS instruction issue of the one CU that holds the "
+                                   "chain (the lattice is resident in LDS; HBM carries only "
+                                   "the load and the write-back, 'traffic')",
+                     "frac_of_one_cu_lds": None},
         "results": {"E_per_atom_first_T": float(last.energies_of_T[0]),
                     "E_per_atom_last_T": float(last.energies_of_T[-1]),
                     "C_max": float(np.max(last.C_of_T)),
                     "acceptance_first_last": [float(last.acceptance_of_T[0]),
                                               float(last.acceptance_of_T[-1])],
-                    "alpha_last_T_shells_1_to_4": _alpha_summary(last, chain)},
+                    "alpha_last_T_shells_1_to_4": _alpha_summary(last, chain),
+                    "carried_energy_vs_recount": {
+                        "max_abs_Ry": max(r.e_drift_abs for r in results),
+                        "max_rel": max(r.e_drift_rel for r in results),
+                        "note": "the carried energy (recount + accepted dE) against an FP64 "
+                                "recount at the end of every T step, worst over the timed steps"}},
     }
     lat.close()
     if not args.no_batched:
@@ -586,10 +768,13 @@ def main():
         out["batched"]["vs_plain"] = out["batched"]["value"] / value
     if not args.no_served:
         out["served"] = served(args)
+        out["served"]["saturation"] = served_sweep(args)
     if not args.no_batched:
         out["wang_landau_walkers"] = wang_landau_walkers()
+        out["nested_sampling_runs"] = nested_sampling_runs()
         out["sequential_bit_exact"] = sequential_bit_exact()
     out["roofline_lds"] = lds_roofline(achieved, out.get("batched"))
+    out["roofline"]["frac_of_one_cu_lds"] = out["roofline_lds"]["plain_frac_of_one_cu"]
     out["cpu_baseline"] = cpu
     print(json.dumps(out))
 

@@ -55,7 +55,8 @@ def lib():
                                                                          C.c_double, dp, dp]),
             "ref_wl_enter": (C.c_int64, [i32] * 6 + [dp, i8p, i32, C.c_double, C.c_double,
                                                      C.c_float, C.c_float, C.c_int64]),
-            "ref_nested_sampling": (C.c_int32, [i32] * 6 + [dp, i32, i64p, i32, i32, i32, dp]),
+            "ref_nested_sampling": (C.c_int32, [i32] * 6 + [dp, i32, i64p, i32, i32, i32, dp, i64p,
+                                                            dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -184,9 +185,8 @@ def _metropolis_sweeps(lattice, n, v_ex, cfg, beta, n_sweeps, rank=0):
 metropolis_sweeps = _isolated(_metropolis_sweeps)
 
 
-@_isolated
-def wl_sweeps(lattice, n, v_ex, cfg, *, rank, nbr_swap, mc_sweeps, n_calls, bin_edges, start_idx,
-              end_idx, wl_f, lng, hist):
+def _wl_sweeps(lattice, n, v_ex, cfg, *, rank, nbr_swap, mc_sweeps, n_calls, bin_edges, start_idx,
+               end_idx, wl_f, lng, hist):
     """n_calls `sweeps` calls of one walker from the stream of `rank`
     (ref_wl_sweeps); returns (accepted, configuration, ln g, histogram)."""
     v = np.asfortranarray(v_ex, dtype=np.float64)
@@ -202,6 +202,9 @@ def wl_sweeps(lattice, n, v_ex, cfg, *, rank, nbr_swap, mc_sweeps, n_calls, bin_
     return int(acc), np.array(c), lg, h
 
 
+wl_sweeps = _isolated(_wl_sweeps)
+
+
 @_isolated
 def wl_enter(lattice, n, v_ex, cfg, *, rank, min_e, max_e, energy_min, energy_max, max_trials):
     """enter_energy_window of one walker from the stream of `rank`
@@ -214,20 +217,31 @@ def wl_enter(lattice, n, v_ex, cfg, *, rank, min_e, max_e, energy_min, energy_ma
     return int(t), np.array(c)
 
 
-@_isolated
-def nested_sampling(lattice, n, v_ex, species_numbers, *, rank, n_walkers, n_steps, n_iter):
+def _nested_sampling(lattice, n, v_ex, species_numbers, *, rank, n_walkers, n_steps, n_iter):
     """The nested-sampling loop from the stream of `rank`
-    (ref_nested_sampling); returns the n_iter energy limits."""
+    (ref_nested_sampling); returns (the n_iter energy limits, walk trials,
+    seconds of the iteration loop)."""
     v = np.asfortranarray(v_ex, dtype=np.float64)
     S, _, R = v.shape
     sn = np.ascontiguousarray(species_numbers, dtype=np.int64)
     out = np.zeros(n_iter)
+    tr, secs = C.c_int64(0), C.c_double(0.0)
     rc = lib().ref_nested_sampling(lattice, *n, S, R, _p(v, C.c_double), int(rank),
                                    _p(sn, C.c_int64), int(n_walkers), int(n_steps), int(n_iter),
-                                   _p(out, C.c_double))
+                                   _p(out, C.c_double), C.byref(tr), C.byref(secs))
     if rc:
         raise RuntimeError(f"ref_nested_sampling returned {rc}")
-    return out
+    return out, int(tr.value), float(secs.value)
+
+
+@_isolated
+def nested_sampling(lattice, n, v_ex, species_numbers, *, rank, n_walkers, n_steps, n_iter):
+    """The n_iter energy limits of the reference's nested-sampling loop."""
+    return _nested_sampling(lattice, n, v_ex, species_numbers, rank=rank, n_walkers=n_walkers,
+                            n_steps=n_steps, n_iter=n_iter)[0]
+
+
+nested_sampling_timed = _isolated(_nested_sampling)
 
 
 def metropolis_sweeps_replicas(lattice, n, v_ex, cfgs, beta, n_sweeps):
@@ -259,5 +273,38 @@ def metropolis_sweeps_replicas(lattice, n, v_ex, cfgs, beta, n_sweeps):
         p.join()
         if status != "ok":
             raise RuntimeError(f"reference replica failed: {val}")
+        out.append(val)
+    return out
+
+
+def run_parallel(fn, arg_list):
+    """fn(*args) for every args in arg_list, each in its own forked process,
+    all at once; returns the results in order (fn must return picklable
+    values)."""
+    ctx = mp.get_context("fork")
+    procs = []
+    for args in arg_list:
+        rd, wr = ctx.Pipe(duplex=False)
+
+        def child(wr=wr, args=args):
+            try:
+                wr.send(("ok", fn(*args)))
+            except BaseException as e:  # report, never hang the parent
+                wr.send(("err", repr(e)))
+            wr.close()
+
+        p = ctx.Process(target=child)
+        p.start()
+        wr.close()
+        procs.append((p, rd))
+    out = []
+    for p, rd in procs:
+        try:
+            status, val = rd.recv()
+        except EOFError:
+            status, val = "err", f"exit code {p.exitcode}"
+        p.join()
+        if status != "ok":
+            raise RuntimeError(f"reference worker failed: {val}")
         out.append(val)
     return out

@@ -416,15 +416,19 @@ contains
   !> (initial_setup with species_numbers, genrand, setup%full_energy,
   !> setup%rdm_site, pair_energy, pair_swap); the energies file, trajectory
   !> and screen output are left out.  n_walkers >= 4, so the first
-  !> walk-length check follows a walk.  e_limits(n_iter) out; returns 0.
+  !> walk-length check follows a walk.  e_limits(n_iter) out, with the walk
+  !> trials made and the seconds of the iteration loop; returns 0.
   function ref_nested_sampling(lattice, n1, n2, n3, ns, nr, vex, rank, species_numbers, n_walkers, &
-                               n_steps, n_iter, e_limits) result(status) &
+                               n_steps, n_iter, e_limits, trials, seconds) result(status) &
       bind(C, name="ref_nested_sampling")
     integer(c_int32_t), value :: lattice, n1, n2, n3, ns, nr, rank, n_walkers, n_steps, n_iter
     real(c_double), intent(in) :: vex(*)
     integer(c_int64_t), intent(in) :: species_numbers(*)
     real(c_double), intent(out) :: e_limits(*)
+    integer(c_int64_t), intent(out) :: trials
+    real(c_double), intent(out) :: seconds
     integer(c_int32_t) :: status
+    integer(int64) :: c0, c1, rate
     type(run_params) :: setup
     integer(c_int) :: seed
     integer(array_int), allocatable :: walkers(:,:,:,:,:)
@@ -449,6 +453,8 @@ contains
     end do
     extra_steps = 0
     n_acc = 0
+    trials = 0_c_int64_t
+    call system_clock(c0, rate)
     do i_iter = 1, n_iter
       i_max = maxloc(walker_energies)
       ener_limit = walker_energies(i_max(1))
@@ -463,6 +469,7 @@ contains
       walkers(:,:,:,:,i_max(1)) = walkers(:,:,:,:,irnd)
       walker_energies(i_max(1)) = walker_energies(irnd)
       n_acc = 0
+      trials = trials + int(n_steps + extra_steps, c_int64_t)
       do i_step = 1, n_steps + extra_steps
         rdm1 = setup%rdm_site()
         site1 = walkers(rdm1(1), rdm1(2), rdm1(3), rdm1(4), i_max(1))
@@ -483,6 +490,8 @@ contains
         end if
       end do
     end do
+    call system_clock(c1)
+    seconds = real(c1 - c0, c_double) / real(rate, c_double)
     status = 0
   end function ref_nested_sampling
 

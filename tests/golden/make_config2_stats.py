@@ -16,6 +16,14 @@ rho_of_T (4, 4, 5, 50).  About 25 minutes of one core per seed; the seeds
 run in parallel processes:
 
     python tests/golden/make_config2_stats.py [--seeds 8] [--procs 8]
+
+With --equilibrium T the same lattice is held at the one temperature T:
+--burn-in unsampled sweeps from the random start, then --sampled sweeps with
+the energy after each and radial densities every 20
+(tests/golden/config2_equilibrium_<T>K.npz):
+
+    python tests/golden/make_config2_stats.py --equilibrium 1500 --burn-in 1000
+    python tests/golden/make_config2_stats.py --equilibrium 1100 --burn-in 5000
 """
 from __future__ import annotations
 
@@ -34,6 +42,7 @@ if ROOT not in sys.path:
 
 ASRO_EVERY = 20   # sweeps between radial-density samples
 WC = 5            # on-site shell + the 4 interaction shells
+SAMPLED = 2000    # equilibrium runs: sampled sweeps after the burn-in
 
 
 def run_seed(k: int):
@@ -52,26 +61,55 @@ def run_seed(k: int):
     return k, out["energies"], out["C"], out["acceptance"], out["rho"]
 
 
+def run_seed_equilibrium(k: int, T: float, burn_in: int):
+    """Seed k held at T: burn_in sweeps (metropolis.F90:214-224, burn-in
+    before the first step), then SAMPLED sweeps."""
+    from brawl_amd import workloads
+    from oracle import oracle as orc
+    c = workloads.config2()
+    o = orc.OracleLattice(c.lattice, c.n, c.v_ex)
+    gen = orc.Generator(rank=k)
+    cfg = o.initial_setup(gen, species_numbers=c.counts)
+    N = o.n_atoms
+    t0 = time.time()
+    out = o.anneal(cfg, gen, T=T, T_steps=1, n_mc_steps=SAMPLED * N, n_sample_steps=N,
+                   burn_in_start=True, n_burn_in_steps=burn_in * N, calculate_asro=True,
+                   n_sample_steps_asro=ASRO_EVERY * N, wc_range=WC)
+    print(f"seed {k} at {T} K: {time.time() - t0:.0f} s", flush=True)
+    return k, out["energies"], out["C"], out["acceptance"], out["rho"]
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--seeds", type=int, default=8)
     ap.add_argument("--procs", type=int, default=8)
-    ap.add_argument("--out", default=os.path.join(HERE, "config2_stats.npz"))
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--equilibrium", type=float, default=None, metavar="T")
+    ap.add_argument("--burn-in", type=int, default=1000)
     a = ap.parse_args()
     from brawl_amd import workloads
     c = workloads.config2()
     with mp.get_context("fork").Pool(a.procs) as pool:
-        res = sorted(pool.map(run_seed, range(a.seeds)))
+        if a.equilibrium is None:
+            res = sorted(pool.map(run_seed, range(a.seeds)))
+        else:
+            res = sorted(pool.starmap(run_seed_equilibrium,
+                                      [(k, a.equilibrium, a.burn_in) for k in range(a.seeds)]))
+    if a.equilibrium is None:
+        out = a.out or os.path.join(HERE, "config2_stats.npz")
+        T, n_sweeps, burn = c.T + c.delta_T * np.arange(c.T_steps), c.n_sweeps, 0
+    else:
+        out = a.out or os.path.join(HERE, f"config2_equilibrium_{a.equilibrium:.0f}K.npz")
+        T, n_sweeps, burn = np.array([a.equilibrium]), SAMPLED, a.burn_in
     np.savez_compressed(
-        a.out,
+        out,
         seeds=np.array([r[0] for r in res]),
-        T=c.T + c.delta_T * np.arange(c.T_steps),
-        n_sweeps=c.n_sweeps, asro_every=ASRO_EVERY, wc_range=WC,
+        T=T, n_sweeps=n_sweeps, burn_in=burn, asro_every=ASRO_EVERY, wc_range=WC,
         energies_of_T=np.array([r[1] for r in res]),
         C_of_T=np.array([r[2] for r in res]),
         acceptance_of_T=np.array([r[3] for r in res]),
         rho_of_T=np.array([r[4] for r in res]))
-    print("wrote", a.out)
+    print("wrote", out)
 
 
 if __name__ == "__main__":

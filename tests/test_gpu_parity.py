@@ -758,6 +758,8 @@ def test_config2_anneal_statistics_over_seeds(hip):
         gpu_E.append(res.energies_of_T)
         gpu_C.append(res.C_of_T)
         gpu_a.append(warren_cowley(res.rho_of_T, shells, conc))
+    _save("config2_anneal_gpu.npz", E=np.array(gpu_E), C=np.array(gpu_C),
+          alpha=np.array(gpu_a))
     cpu_a = [warren_cowley(r, shells, conc) for r in ref["rho_of_T"]]
     Ts = ref["T"]
     t_peak = int(np.argmax(ref["C_of_T"].mean(axis=0)))
@@ -779,3 +781,68 @@ def test_config2_anneal_statistics_over_seeds(hip):
     _say(f"alpha: {np.mean(za <= 2.0):.3f} within 2 sigma, largest {za.max():.2f}")
     assert np.mean(za <= 2.0) >= 0.9
     assert za.max() <= 3.5
+
+
+def _save(name, **arrays):
+    """Per-seed arrays of a statistics test, written to $BRAWL_TEST_OUT (when
+    set) for inspection beside the reference fixtures."""
+    import os
+    d = os.environ.get("BRAWL_TEST_OUT")
+    if d:
+        os.makedirs(d, exist_ok=True)
+        np.savez_compressed(os.path.join(d, name), **arrays)
+
+
+@pytest.mark.parametrize("temp", [1500, 1100])
+def test_config2_equilibrium_statistics(hip, temp):
+    """The headline lattice held at one temperature long enough to reach
+    equilibrium: 1500 K (short-range order, fast relaxation; 1000 unsampled
+    sweeps from a random start) and 1100 K, just above the anneal's heat-
+    capacity peak (1047 K), where relaxation is slow (5000 unsampled
+    sweeps); then 2000 sampled sweeps.  The parallel sampler over 16 seeds
+    against the reference's sequential sampler over 8
+    (tests/golden/config2_equilibrium_<T>K.npz, the oracle's restatement of
+    metropolis.F90:193-437 with burn-in): <E>/N and C within 2 sigma, and
+    the Warren-Cowley alpha^(n)(a, b), n = 1..4, a <= b (40 values), at
+    least 90 % within 2 sigma and all within 3.5 sigma.  Equilibrium values
+    do not depend on how a sampler proposes its moves; an anneal's values at
+    the transition do."""
+    import os
+
+    from conftest import GOLDEN
+    from brawl_amd import warren_cowley, workloads
+    path = os.path.join(GOLDEN, f"config2_equilibrium_{temp}K.npz")
+    if not os.path.exists(path):
+        pytest.skip(f"{os.path.basename(path)} not generated")
+    ref = dict(np.load(path))
+    T = float(ref["T"][0])
+    c = workloads.config2()
+    S = 4
+    conc = np.full(S, 0.25)
+    shells = [8, 6, 12, 24]
+    gE, gC, ga = [], [], []
+    for k in range(16):
+        with hip.Lattice(c.lattice, c.n, c.v_ex) as L:
+            L.seed(110179 + 11 * (200 + k))
+            _say(f"equilibrium {temp} K, seed {k}")
+            L.initial_setup(species_numbers=c.counts)
+            res = L.metropolis_simulated_annealing(
+                T=T, T_steps=1, n_sweeps=int(ref["n_sweeps"]), burn_in_sweeps=int(ref["burn_in"]),
+                seed=workloads.derive_seed(workloads.MASTER_SEED, 1900 + k),
+                asro_every=int(ref["asro_every"]))
+        assert res.e_drift_rel <= 1e-12
+        gE.append(res.energies_of_T[0])
+        gC.append(res.C_of_T[0])
+        ga.append(warren_cowley(res.rho_of_T, shells, conc)[:, :, 1:, 0])
+    _save(f"config2_equilibrium_{temp}K_gpu.npz", E=np.array(gE), C=np.array(gC),
+          alpha=np.array(ga))
+    zE = _z(gE, ref["energies_of_T"][:, 0])
+    zC = _z(gC, ref["C_of_T"][:, 0])
+    iu = np.triu_indices(S)
+    ra = [warren_cowley(r, shells, conc)[:, :, 1:, 0] for r in ref["rho_of_T"]]
+    za = np.abs(_z(np.array([a[iu] for a in ga]), np.array([a[iu] for a in ra])).ravel())
+    _say(f"T = {T} K: <E>/N gpu {np.mean(gE):.6e} ref {ref['energies_of_T'][:, 0].mean():.6e} "
+         f"(z {zE:+.2f}); C gpu {np.mean(gC):.4e} ref {ref['C_of_T'][:, 0].mean():.4e} (z {zC:+.2f}); "
+         f"alpha {np.mean(za <= 2.0):.3f} within 2 sigma, largest {za.max():.2f}")
+    assert abs(zE) <= 2.0 and abs(zC) <= 2.0
+    assert np.mean(za <= 2.0) >= 0.9 and za.max() <= 3.5
