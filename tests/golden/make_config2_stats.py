@@ -20,10 +20,12 @@ run in parallel processes:
 With --equilibrium T the same lattice is held at the one temperature T:
 --burn-in unsampled sweeps from the random start, then --sampled sweeps with
 the energy after each and radial densities every 20
-(tests/golden/config2_equilibrium_<T>K.npz):
+(tests/golden/config2_equilibrium_<T>K.npz), with each seed's <E>/N over
+the first and the second half of the sampled sweeps (e_halves, a check that
+the run was stationary):
 
     python tests/golden/make_config2_stats.py --equilibrium 1500 --burn-in 1000
-    python tests/golden/make_config2_stats.py --equilibrium 1100 --burn-in 5000
+    python tests/golden/make_config2_stats.py --equilibrium 1250 --burn-in 3000
 """
 from __future__ import annotations
 
@@ -74,9 +76,13 @@ def run_seed_equilibrium(k: int, T: float, burn_in: int):
     t0 = time.time()
     out = o.anneal(cfg, gen, T=T, T_steps=1, n_mc_steps=SAMPLED * N, n_sample_steps=N,
                    burn_in_start=True, n_burn_in_steps=burn_in * N, calculate_asro=True,
-                   n_sample_steps_asro=ASRO_EVERY * N, wc_range=WC)
-    print(f"seed {k} at {T} K: {time.time() - t0:.0f} s", flush=True)
-    return k, out["energies"], out["C"], out["acceptance"], out["rho"]
+                   n_sample_steps_asro=ASRO_EVERY * N, wc_range=WC, trajectory=True)
+    # stationarity: <E>/N over the first and the second half of the sampled
+    # sweeps (the trajectory's first entry is the state before them)
+    e = out["trajectory"][1:] / N
+    halves = np.array([e[:SAMPLED // 2].mean(), e[SAMPLED // 2:].mean()])
+    print(f"seed {k} at {T} K: {time.time() - t0:.0f} s, halves {halves}", flush=True)
+    return k, out["energies"], out["C"], out["acceptance"], out["rho"], halves
 
 
 def main() -> None:
@@ -101,8 +107,9 @@ def main() -> None:
     else:
         out = a.out or os.path.join(HERE, f"config2_equilibrium_{a.equilibrium:.0f}K.npz")
         T, n_sweeps, burn = np.array([a.equilibrium]), SAMPLED, a.burn_in
+    extra = {} if a.equilibrium is None else {"e_halves": np.array([r[5] for r in res])}
     np.savez_compressed(
-        out,
+        out, **extra,
         seeds=np.array([r[0] for r in res]),
         T=T, n_sweeps=n_sweeps, burn_in=burn, asro_every=ASRO_EVERY, wc_range=WC,
         energies_of_T=np.array([r[1] for r in res]),
