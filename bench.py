@@ -87,7 +87,7 @@ def calibrate_copy():
 
 def cpu_baseline(chain, budget_s: float):
     """The reference's serial path on a bounded sample of the same anneal
-    (its first temperature step, as many sweeps as fit the time budget):
+    (three of its temperature steps, as many sweeps as fit the time budget):
     when the reference build is present (oracle/_ref, compiled from the
     reference's sources: its own rdm_site, pair_energy, pair_swap, genrand
     in the order of monte_carlo_step_lattice, full_energy after every sweep,
@@ -136,24 +136,40 @@ def cpu_baseline(chain, budget_s: float):
                             "sample": f"{cores} oracle chains (one per core), {rsweeps} sweeps "
                                       f"each, {rdt:.1f} s"}
         return port
-    beta = beta_of(chain.T)
-    _, _, one, _ = oref.metropolis_sweeps(chain.lattice, chain.n, chain.v_ex, cfg, beta, 1)
-    rs = max(1, int(budget_s / max(one, 1e-6)))
-    _, _, secs, _ = oref.metropolis_sweeps(chain.lattice, chain.n, chain.v_ex, cfg, beta, rs)
+    # three temperatures of the anneal (its first step, the heat-capacity
+    # peak, its last step), a third of the budget each, from the same start
+    temps = [chain.T, chain.T + chain.delta_T * 33, chain.T + chain.delta_T * (chain.T_steps - 1)]
+    betas = [beta_of(t) for t in temps]
+    _, _, one, _ = oref.metropolis_sweeps(chain.lattice, chain.n, chain.v_ex, cfg, betas[0], 1)
+    rs = max(1, int(budget_s / 3 / max(one, 1e-6)))
+    per_T, secs = [], 0.0
+    for t, b in zip(temps, betas):
+        _, _, st, _ = oref.metropolis_sweeps(chain.lattice, chain.n, chain.v_ex, cfg, b, rs)
+        per_T.append({"T": round(t, 1), "value": rs * N / st})
+        secs += st
     rsweeps = max(1, rs // 2)
     t0 = time.perf_counter()
-    reps = oref.metropolis_sweeps_replicas(chain.lattice, chain.n, chain.v_ex, [cfg] * cores, beta,
-                                           rsweeps)
+    reps = oref.run_parallel(_ref_chain, [(chain, cfg, k, betas[k % 3], rsweeps) for k in range(cores)])
     rdt = time.perf_counter() - t0
-    return {"value": rs * N / secs, "unit": "swap trials/s", "cores": 1, "kind": "reference",
-            "sample": f"first T step (3000 K) of the same anneal through the reference's own "
-                      f"compiled routines, {rs} sweeps = {rs * N} trials with the per-sweep "
+    return {"value": 3 * rs * N / secs, "unit": "swap trials/s", "cores": 1, "kind": "reference",
+            "sample": f"three T steps of the same anneal ({temps[0]:.0f} K, {temps[1]:.0f} K, "
+                      f"{temps[2]:.0f} K) from the same random start through the reference's own "
+                      f"compiled routines, {rs} sweeps = {rs * N} trials each with the per-sweep "
                       f"energy recount, {secs:.1f} s in the trial loops",
+            "per_T": per_T,
             "replicas": {"value": cores * rsweeps * N / rdt, "cores": cores,
                          "sample": f"{cores} reference chains in parallel processes (ranks "
-                                   f"0..{cores - 1}), {rsweeps} sweeps each, {rdt:.1f} s wall "
-                                   f"(longest chain {max(r[2] for r in reps):.1f} s in its loops)"},
+                                   f"0..{cores - 1}, the three temperatures in turn), {rsweeps} "
+                                   f"sweeps each, {rdt:.1f} s wall (longest chain "
+                                   f"{max(r[2] for r in reps):.1f} s in its loops); 16 cores is the "
+                                   f"host share of one GPU"},
             "port": port}
+
+
+def _ref_chain(chain, cfg, k, beta, n_sweeps):
+    """One reference chain (rank k's stream) at beta for n_sweeps sweeps."""
+    from oracle import ref as oref
+    return oref._metropolis_sweeps(chain.lattice, chain.n, chain.v_ex, cfg, beta, n_sweeps, rank=k)
 
 
 def _wl_walker_sample(k: int, n_calls: int):
