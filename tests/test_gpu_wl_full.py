@@ -21,11 +21,12 @@ judges only flatness, so its error saturates instead of shrinking with
 ln f (at 1e-4 and at 1e-6 alike, about 0.03 at m = 1 and 0.2 at m = 6
 here): the spread of single runs, not the standard error of their mean,
 is the scale its result is held to."""
-import itertools
 import sys
 
 import numpy as np
 import pytest
+
+from exact_dos import exact_dos as _exact_dos
 
 pytestmark = pytest.mark.gpu
 
@@ -130,31 +131,6 @@ def test_config4_full_size_entry_sweeps_and_exchange(hip, oracle):
         recount_all()
     _say("config4: pairs exchanged over 2 calls:", exchanged)
     wl.close()
-
-
-def _exact_dos():
-    """Exact counts of configurations of 5 solutes on bcc 3^3 (54 sites) by
-    number of solute-solute shell-1 bonds (with the reference list's
-    multiplicities)."""
-    from oracle import oracle as orc
-    lat, n = 2, (3, 3, 3)
-    grid = tuple(2 * x for x in n)
-    sites = [(i, j, k) for k in range(grid[2]) for j in range(grid[1]) for i in range(grid[0])
-             if i % 2 == j % 2 == k % 2]
-    idx = {s: q for q, s in enumerate(sites)}
-    offs = np.ctypeslib.as_array(orc.lib().bwo_shell_offsets(lat, 1), shape=(24,)).reshape(8, 3)
-    N = len(sites)
-    A = np.zeros((N, N), dtype=np.int64)
-    for q, s in enumerate(sites):
-        for d in offs:
-            A[q, idx[tuple((s[x] + int(d[x])) % grid[x] for x in range(3))]] += 1
-    combs = np.array(list(itertools.combinations(range(N), 5)), dtype=np.int16)
-    m = np.zeros(len(combs), dtype=np.int64)
-    for x in range(5):
-        for y in range(x + 1, 5):
-            m += A[combs[:, x], combs[:, y]]
-    levels, counts = np.unique(m, return_counts=True)
-    return sites, combs, m, levels, counts
 
 
 def _oracle_wl_run(oracle, ref, species_numbers, edges, window, emin, emax, ranks, wl_f,
