@@ -241,13 +241,24 @@ __host__ __device__ constexpr bool nat_dims_fixed(int lattice, int R, int n, Nat
   return true;
 }
 
-// Padded LDS address of the interior cell (s, cx, cy, cz).
-__host__ __device__ inline int padded_addr(const Geo &g, int s, int cx, int cy, int cz) {
+// The scalars of a geometry that the sequential samplers' trial loops use,
+// without the shell table.
+struct SeqGeo {
+  int32_t lattice;
+  int32_t n[3], m[3], P[3];
+  int32_t h, sub_stride, S, R, z;
+};
+
+// Padded LDS address of the interior cell (s, cx, cy, cz).  GEO is Geo or
+// SeqGeo.
+template <class GEO>
+__host__ __device__ inline int padded_addr(const GEO &g, int s, int cx, int cy, int cz) {
   return s * g.sub_stride + ((cz + g.h) * g.P[1] + (cy + g.h)) * g.P[0] + (cx + g.h);
 }
 
 // Padded address of the site at 0-based doubled coordinates (x,y,z).
-__host__ __device__ inline int padded_addr_xyz(const Geo &g, int x, int y, int z) {
+template <class GEO>
+__host__ __device__ inline int padded_addr_xyz(const GEO &g, int x, int y, int z) {
   return padded_addr(g, sub_of(g.lattice, x & 1, y & 1), x >> 1, y >> 1, z >> 1);
 }
 

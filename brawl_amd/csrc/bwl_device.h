@@ -60,7 +60,8 @@ __device__ inline void unstage_lattice(const int8_t *lat, int8_t *__restrict__ c
 // Store `v` at interior cell (s,cx,cy,cz) and at every periodic image of it
 // inside the padded box.  Images are split over `nl` cooperating lanes
 // (lane `l`); a single thread passes l = 0, nl = 1.
-__device__ inline void write_site(int8_t *lat, const Geo &g, int s, int cx, int cy, int cz,
+template <class GEO>
+__device__ inline void write_site(int8_t *lat, const GEO &g, int s, int cx, int cy, int cz,
                                   int8_t v, int l, int nl) {
   if (g.h <= g.n[0] && g.h <= g.n[1] && g.h <= g.n[2]) {
     // halo no wider than the box (every lattice but the smallest): the first

@@ -193,10 +193,12 @@ __device__ inline void init_pair_lanes(const Geo &g, PairLanes &pl) {
 // The ordered sum of pair_energy_lanes from the lanes' values (entries
 // 0..z-1 site A's, z..2z-1 site B's): e_A + e_B, each site's shells left to
 // right, each shell from 0.0 in list order.
-__device__ inline double pair_sum_loop(const Geo &g, const PairLanes &pl,
+template <class GEO>
+__device__ inline double pair_sum_loop(const GEO &g, const PairLanes &pl,
                                        const double (&vals)[6], double *eA_out);
 
-__device__ inline double pair_sum(const Geo &g, const PairLanes &pl, const double (&vals)[6],
+template <class GEO>
+__device__ inline double pair_sum(const GEO &g, const PairLanes &pl, const double (&vals)[6],
                                   double *eA_out) {
   switch (pl.fixed) {
     case 16 + 1: return fixed_pair_sum<1, 1>(vals, eA_out);
@@ -217,7 +219,8 @@ __device__ inline double pair_sum(const Geo &g, const PairLanes &pl, const doubl
 }
 
 // The same sum for any shell structure, driven by the lanes' shell-end mask.
-__device__ inline double pair_sum_loop(const Geo &g, const PairLanes &pl,
+template <class GEO>
+__device__ inline double pair_sum_loop(const GEO &g, const PairLanes &pl,
                                        const double (&vals)[6], double *eA_out) {
   const int z = g.z;
   double eA = 0.0, e = 0.0, es = 0.0;
@@ -246,7 +249,8 @@ __device__ inline double pair_sum_loop(const Geo &g, const PairLanes &pl,
 }
 
 // Both ordered sums of pair_energy_swap_lanes.
-__device__ inline void pair_sum2(const Geo &g, const PairLanes &pl, const double (&v1)[6],
+template <class GEO>
+__device__ inline void pair_sum2(const GEO &g, const PairLanes &pl, const double (&v1)[6],
                                  const double (&v2)[6], double &e1, double &e2) {
   switch (pl.fixed) {
     case 16 + 1: fixed_pair_sum2<1, 1>(v1, v2, e1, e2); return;
@@ -271,8 +275,8 @@ __device__ inline void pair_sum2(const Geo &g, const PairLanes &pl, const double
 // must match pl.fixed; the straight-line sums read the entries back from
 // `sumbuf` in LDS), or chosen at run time (KEY < 0): a kernel compiled for
 // one variant holds only that sum's code.
-template <int KEY>
-__device__ inline void pair_sum2_t(const Geo &g, const PairLanes &pl, const double (&v1)[6],
+template <int KEY, class GEO>
+__device__ inline void pair_sum2_t(const GEO &g, const PairLanes &pl, const double (&v1)[6],
                                    const double (&v2)[6], double *sumbuf, double &e1, double &e2) {
   if constexpr (KEY > 0) {
     fixed_pair_sum2_lds<KEY / 16, KEY % 16>(v1, v2, sumbuf, e1, e2);
@@ -342,8 +346,8 @@ constexpr int8_t kMarkA = 0x41, kMarkB = 0x42;
 // neighbour entry that is A or B itself (marked) takes that site's species
 // before or after the swap, every other entry its neighbour's species, so
 // both sums are bit-identical to gathers before and after a real swap.
-template <int KEY = -1>
-__device__ inline void pair_energy_swap_lanes(const int8_t *lat, const Geo &g, const int32_t *off,
+template <int KEY = -1, class GEO>
+__device__ inline void pair_energy_swap_lanes(const int8_t *lat, const GEO &g, const int32_t *off,
                                               const double *vex, const PairLanes &pl, int addrA,
                                               int subA, int addrB, int subB, int spA, int spB,
                                               double *sumbuf, double &e_before, double &e_after) {
@@ -376,7 +380,8 @@ struct SiteXYZ {
 
 // Write species va at site a and vb at site b (every image of each; the
 // whole wavefront helps), between barriers.
-__device__ inline void put_pair(int8_t *lat, const Geo &g, SiteXYZ a, int suba, int8_t va, SiteXYZ b,
+template <class GEO>
+__device__ inline void put_pair(int8_t *lat, const GEO &g, SiteXYZ a, int suba, int8_t va, SiteXYZ b,
                                 int subb, int8_t vb) {
   const int lane = threadIdx.x & 63;
   __syncthreads();
@@ -387,7 +392,8 @@ __device__ inline void put_pair(int8_t *lat, const Geo &g, SiteXYZ a, int suba, 
 
 // bcc_random_site / fcc_random_site (random_site.f90:127-139, 182-193):
 // floor(c*genrand()*n) equals (r*c*n) >> 32 exactly since r*c*n < 2^53.
-__device__ inline SiteXYZ random_site_dev(const Geo &g, uint32_t *st, int &mti) {
+template <class GEO>
+__device__ inline SiteXYZ random_site_dev(const GEO &g, uint32_t *st, int &mti) {
   uint32_t r1, r2, r3;
   mt_next3(st, mti, r1, r2, r3);
   int k = (int)(((uint64_t)r1 * (uint64_t)(2 * g.n[2])) >> 32) + 1;
@@ -411,7 +417,8 @@ static __constant__ int8_t kRsFcc[12][3] = {{0, 1, 1},  {0, 1, -1},  {0, -1, 1},
                                      {1, 1, 0},  {1, -1, 0},  {1, 0, 1},  {1, 0, -1},
                                      {-1, 1, 0}, {-1, -1, 0}, {-1, 0, 1}, {-1, 0, -1}};
 
-__device__ inline SiteXYZ random_nbr_dev(const Geo &g, SiteXYZ s, uint32_t *st, int &mti) {
+template <class GEO>
+__device__ inline SiteXYZ random_nbr_dev(const GEO &g, SiteXYZ s, uint32_t *st, int &mti) {
   uint32_t r = mt_next(st, mti);
   const int8_t *d;
   if (g.lattice == 2)

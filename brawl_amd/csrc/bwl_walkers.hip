@@ -207,9 +207,31 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
   double *hist = (double *)(smem + o);
   o += sizeof(double) * hb;
   int32_t *offl = (int32_t *)(smem + o);
+  o += sizeof(int32_t) * g.nsub * g.z;
+  // the trial loop's geometry and window, copied to LDS and read back into
+  // registers once: values read from the argument block may be re-read from
+  // it at every use when scalar registers run short
+  int32_t *gw = (int32_t *)(smem + ((o + 15) & ~(size_t)15));
 
   for (int i = lane; i < 624; i += 64) st[i] = A.mt[i];
   for (int i = lane; i < g.S * g.S * g.R; i += 64) vex[i] = A.vex[i];
+  if (lane == 0) {
+    gw[0] = g.lattice;
+    for (int a = 0; a < 3; ++a) {
+      gw[1 + a] = g.n[a];
+      gw[4 + a] = g.m[a];
+      gw[7 + a] = g.P[a];
+    }
+    gw[10] = g.h;
+    gw[11] = g.sub_stride;
+    gw[12] = g.S;
+    gw[13] = g.R;
+    gw[14] = g.z;
+    gw[15] = A.bins;
+    gw[16] = A.start_idx;
+    gw[17] = A.end_idx;
+    gw[18] = A.nbr_swap;
+  }
   stage_offsets(offl, A.off, g);
   PairLanes pl;
   init_pair_lanes(g, pl);
@@ -219,19 +241,32 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
   stage_lattice(lat, A.compact, g, 0);
   __syncthreads();
   int mti = A.mt_index[0];
-  const int bins = A.bins;
+  SeqGeo sg;
+  sg.lattice = gw[0];
+  for (int a = 0; a < 3; ++a) {
+    sg.n[a] = gw[1 + a];
+    sg.m[a] = gw[4 + a];
+    sg.P[a] = gw[7 + a];
+  }
+  sg.h = gw[10];
+  sg.sub_stride = gw[11];
+  sg.S = gw[12];
+  sg.R = gw[13];
+  sg.z = gw[14];
+  const int bins = gw[15], start_idx = gw[16], end_idx = gw[17], nbr_swap = gw[18];
+  const double wl_f = A.wl_f;
   const long long cadence = (long long)(int)(0.02 * (double)(float)g.n_atoms);
   double e_un = ref_total_energy(lat, g, offl, vex);
   int ibin = wl_bin_index(e_un, edges, bins);  // bin of e_un, kept with it
   long long accepted = 0;
   const long long total = (long long)A.mc_sweeps * g.n_atoms;
   for (long long i = 1; i <= total; ++i) {
-    const SiteXYZ r1 = random_site_dev(g, st, mti);
-    const SiteXYZ r2 = A.nbr_swap ? random_nbr_dev(g, r1, st, mti) : random_site_dev(g, st, mti);
-    const int a1 = padded_addr_xyz(g, r1.x, r1.y, r1.z);
-    const int a2 = padded_addr_xyz(g, r2.x, r2.y, r2.z);
-    const int sub1 = sub_of(g.lattice, r1.x & 1, r1.y & 1);
-    const int sub2 = sub_of(g.lattice, r2.x & 1, r2.y & 1);
+    const SiteXYZ r1 = random_site_dev(sg, st, mti);
+    const SiteXYZ r2 = nbr_swap ? random_nbr_dev(sg, r1, st, mti) : random_site_dev(sg, st, mti);
+    const int a1 = padded_addr_xyz(sg, r1.x, r1.y, r1.z);
+    const int a2 = padded_addr_xyz(sg, r2.x, r2.y, r2.z);
+    const int sub1 = sub_of(sg.lattice, r1.x & 1, r1.y & 1);
+    const int sub2 = sub_of(sg.lattice, r2.x & 1, r2.y & 1);
     const int s1 = lat[a1], s2 = lat[a2];
     // The reference also evaluates pair_energy and swaps when the species
     // are equal; neither changes the state, so both are skipped here.
@@ -239,15 +274,15 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
     if (s1 != s2) {
       // energies before and after the swap from one gather (sites marked)
       double pu, ps;
-      put_pair(lat, g, r1, sub1, kMarkA, r2, sub2, kMarkB);
-      pair_energy_swap_lanes<KEY>(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, sumbuf, pu, ps);
+      put_pair(lat, sg, r1, sub1, kMarkA, r2, sub2, kMarkB);
+      pair_energy_swap_lanes<KEY>(lat, sg, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, sumbuf, pu, ps);
       e_sw = e_un - pu + ps;
     }
     bool acc = false;
     // e_sw == e_un when the species are equal (no swap): same bin
     int jbin = s1 != s2 ? wl_bin_index(e_sw, edges, bins) : ibin;
     const bool hit = cadence > 0 && (i % cadence) == 0;
-    if (jbin > A.start_idx - 1 && jbin < A.end_idx + 1) {
+    if (jbin > start_idx - 1 && jbin < end_idx + 1) {
       const double li = (ibin >= 1 && ibin <= bins) ? lng[ibin - 1] : 0.0;
       const double u = genrand_dev(st, mti);
       if (log_lt_dev(u, li - lng[jbin - 1])) {
@@ -261,12 +296,12 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
     } else {
       jbin = ibin;
     }
-    if (s1 != s2) put_pair(lat, g, r1, sub1, (int8_t)(acc ? s2 : s1), r2, sub2, (int8_t)(acc ? s1 : s2));
+    if (s1 != s2) put_pair(lat, sg, r1, sub1, (int8_t)(acc ? s2 : s1), r2, sub2, (int8_t)(acc ? s1 : s2));
     // a walker outside its window would index outside the reference's
     // arrays; such updates are dropped
     if (lane == 0) {
-      if (hit && jbin >= A.start_idx && jbin <= A.end_idx) hist[jbin - A.start_idx] += 1.0;
-      if (jbin >= 1 && jbin <= bins) lng[jbin - 1] = lng[jbin - 1] + A.wl_f;
+      if (hit && jbin >= start_idx && jbin <= end_idx) hist[jbin - start_idx] += 1.0;
+      if (jbin >= 1 && jbin <= bins) lng[jbin - 1] = lng[jbin - 1] + wl_f;
     }
     __syncthreads();
   }
@@ -652,8 +687,10 @@ size_t ns_energies_lds_bytes(const Geo &g) {
 }
 
 size_t wl_lds_bytes(const Geo &g, int bins, int window_bins) {
-  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + kSumBufBytes + sizeof(double) * g.S * g.S * g.R +
-         sizeof(double) * (size_t)(2 * bins + 1 + window_bins) + sizeof(int32_t) * g.nsub * g.z;
+  const size_t o = (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + kSumBufBytes +
+                   sizeof(double) * g.S * g.S * g.R + sizeof(double) * (size_t)(2 * bins + 1 + window_bins) +
+                   sizeof(int32_t) * g.nsub * g.z;
+  return ((o + 15) & ~(size_t)15) + sizeof(int32_t) * 20;  // + the loop's geometry words
 }
 
 hipError_t launch_wl_sweeps(const WlArgs *args_dev, int n, size_t lds, hipStream_t s, int key) {

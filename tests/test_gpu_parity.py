@@ -29,11 +29,14 @@ def hip():
 
 
 def _energy_tol(L, cfg, e_ref):
-    """1e-12 relative, or the rounding noise of the reference's own
-    left-to-right sum of N*z terms (~ N*z*eps*max|V|) when that is larger."""
+    """(tolerance, which bound): 1e-12 relative, or the rounding noise of
+    the reference's own left-to-right sum of N*z terms (~ N*z*eps*max|V|)
+    when that is larger (energies near zero, where a relative bound asks
+    for more digits than the reference's sum itself carries)."""
     z = sum(oracle_shell_sizes(L))
     noise = 8.0 * L.n_atoms * z * np.abs(L.v_ex).max() * 1.1e-16
-    return max(1e-12 * abs(e_ref), noise)
+    rel = 1e-12 * abs(e_ref)
+    return (rel, "1e-12 relative") if rel >= noise else (noise, "summation noise N*z*eps*max|V|")
 
 
 def oracle_shell_sizes(L):
@@ -76,7 +79,10 @@ def test_histogram_energy_and_layout(hip, oracle, lat, n, S, R):
         e_hist = ref.energy_from_histogram(h)
         assert e == e_hist                        # same integers, same contraction order
         e_ref = ref.total_energy(cfg)             # reference order
-        assert abs(e - e_ref) <= _energy_tol(ref, cfg, e_ref)
+        tol, bound = _energy_tol(ref, cfg, e_ref)
+        _say(f"E = {e_ref:.12e} Ry, |E - E_ref| = {abs(e - e_ref):.2e} "
+             f"({abs(e - e_ref) / max(abs(e_ref), 1e-300):.1e} relative), bound: {bound} = {tol:.2e}")
+        assert abs(e - e_ref) <= tol
 
 
 @pytest.mark.parametrize("lat,n,S,R", CASES[:7])
