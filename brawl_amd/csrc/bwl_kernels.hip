@@ -154,26 +154,41 @@ __device__ __forceinline__ void metro_seq_run(const SeqArgs &A, uint8_t *smem) {
   double *vex = (double *)(smem + o);
   o += sizeof(double) * g.S * g.S * g.R;
   int32_t *offl = (int32_t *)(smem + o);
+  o += sizeof(int32_t) * g.nsub * g.z;
+  // the trial loop's geometry and swap mode (see put_seq_geo)
+  int32_t *gw = (int32_t *)(smem + ((o + 15) & ~(size_t)15));
 
   for (int i = lane; i < 624; i += 64) st[i] = A.mt[i];
   for (int i = lane; i < g.S * g.S * g.R; i += 64) vex[i] = A.vex[i];
+  put_seq_geo(gw, g);
+  if (lane == 0) {
+    gw[kSeqGeoWords] = A.nbr_swap;
+    gw[kSeqGeoWords + 1] = A.energies != nullptr && A.sample_every > 0;
+  }
+  put_loop64(gw, 2, (unsigned long long)A.n_trials);
+  put_loop64(gw, 4, (unsigned long long)A.sample_every);
+  put_loop64(gw, 6, (unsigned long long)__double_as_longlong(A.beta));
   stage_offsets(offl, A.off, g);
   PairLanes pl;
   init_pair_lanes(g, pl);
   stage_lattice(lat, A.compact, g, 0);
   __syncthreads();
   int mti = A.mt_index[0];
-  const double beta = A.beta;
   long long accepted = 0;
   long long sample_k = 0;
+  const SeqGeo sg = get_seq_geo(gw);
+  const int nbr_swap = gw[kSeqGeoWords];
+  const bool sampling = gw[kSeqGeoWords + 1] != 0;
+  const long long n_trials = (long long)get_loop64(gw, 2), sample_every = (long long)get_loop64(gw, 4);
+  const double beta = __longlong_as_double((long long)get_loop64(gw, 6));
 
-  for (long long trial = 1; trial <= A.n_trials; ++trial) {
-    SiteXYZ s1 = random_site_dev(g, st, mti);
-    SiteXYZ s2 = A.nbr_swap ? random_nbr_dev(g, s1, st, mti) : random_site_dev(g, st, mti);
-    int addr1 = padded_addr_xyz(g, s1.x, s1.y, s1.z);
-    int addr2 = padded_addr_xyz(g, s2.x, s2.y, s2.z);
-    int sub1 = sub_of(g.lattice, s1.x & 1, s1.y & 1);
-    int sub2 = sub_of(g.lattice, s2.x & 1, s2.y & 1);
+  for (long long trial = 1; trial <= n_trials; ++trial) {
+    SiteXYZ s1 = random_site_dev(sg, st, mti);
+    SiteXYZ s2 = nbr_swap ? random_nbr_dev(sg, s1, st, mti) : random_site_dev(sg, st, mti);
+    int addr1 = padded_addr_xyz(sg, s1.x, s1.y, s1.z);
+    int addr2 = padded_addr_xyz(sg, s2.x, s2.y, s2.z);
+    int sub1 = sub_of(sg.lattice, s1.x & 1, s1.y & 1);
+    int sub2 = sub_of(sg.lattice, s2.x & 1, s2.y & 1);
     int sp1 = lat[addr1], sp2 = lat[addr2];
     bool acc;
     if (sp1 == sp2) {
@@ -181,8 +196,8 @@ __device__ __forceinline__ void metro_seq_run(const SeqArgs &A, uint8_t *smem) {
     } else {
       // energies before and after the swap from one gather (sites marked)
       double e_un, e_sw;
-      put_pair(lat, g, s1, sub1, kMarkA, s2, sub2, kMarkB);
-      pair_energy_swap_lanes<KEY>(lat, g, offl, vex, pl, addr1, sub1, addr2, sub2, sp1, sp2, sumbuf, e_un, e_sw);
+      put_pair(lat, sg, s1, sub1, kMarkA, s2, sub2, kMarkB);
+      pair_energy_swap_lanes<KEY>(lat, sg, offl, vex, pl, addr1, sub1, addr2, sub2, sp1, sp2, sumbuf, e_un, e_sw);
       double de = e_sw - e_un;
       if (de < 0.0) {
         acc = true;
@@ -191,10 +206,10 @@ __device__ __forceinline__ void metro_seq_run(const SeqArgs &A, uint8_t *smem) {
         double u = (double)r * (1.0 / 4294967296.0);
         acc = lt_exp_dev(u, -(beta * de));
       }
-      put_pair(lat, g, s1, sub1, (int8_t)(acc ? sp2 : sp1), s2, sub2, (int8_t)(acc ? sp1 : sp2));
+      put_pair(lat, sg, s1, sub1, (int8_t)(acc ? sp2 : sp1), s2, sub2, (int8_t)(acc ? sp1 : sp2));
     }
     if (acc) accepted++;
-    if (A.energies && A.sample_every > 0 && trial % A.sample_every == 0) {
+    if (sampling && trial % sample_every == 0) {
       const double e = ref_total_energy(lat, g, offl, vex);
       if (lane == 0) A.energies[sample_k] = e;
       sample_k++;
@@ -343,8 +358,9 @@ __global__ __launch_bounds__(64) void k_metro_seq_batch(const SeqArgs *__restric
 }
 
 size_t seq_lds_bytes(const Geo &g) {
-  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + kSumBufBytes + sizeof(double) * g.S * g.S * g.R +
-         sizeof(int32_t) * g.nsub * g.z + 64;
+  return seq_geo_region((size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + kSumBufBytes +
+                        sizeof(double) * g.S * g.S * g.R + sizeof(int32_t) * g.nsub * g.z) +
+         64;
 }
 
 namespace {

@@ -374,6 +374,65 @@ __device__ inline void pair_energy_swap_lanes(const int8_t *lat, const GEO &g, c
   pair_sum2_t<KEY>(g, pl, vb, va, sumbuf, e_before, e_after);
 }
 
+// The trial loops' geometry as words in LDS (a region of kSeqGeoWords
+// words at the end of each sequential kernel's carve, plus room for a few
+// kernel-specific loop constants): written by lane 0 at set-up, read back
+// into registers after the block barrier.  Values read from an argument
+// block may be re-read from it at every use when scalar registers run
+// short, each re-read a scalar memory round trip in the middle of a trial;
+// values loaded from LDS are kept in registers instead.
+constexpr int kSeqGeoWords = 16;   // 15 geometry words, rounded
+constexpr int kSeqLoopWords = 16;  // kernel-specific loop constants after them
+
+__host__ __device__ inline size_t seq_geo_region(size_t o) {
+  return ((o + 15) & ~(size_t)15) + sizeof(int32_t) * (kSeqGeoWords + kSeqLoopWords);
+}
+
+__device__ inline void put_seq_geo(int32_t *gw, const Geo &g) {
+  if ((threadIdx.x & 63) == 0) {
+    gw[0] = g.lattice;
+    for (int a = 0; a < 3; ++a) {
+      gw[1 + a] = g.n[a];
+      gw[4 + a] = g.m[a];
+      gw[7 + a] = g.P[a];
+    }
+    gw[10] = g.h;
+    gw[11] = g.sub_stride;
+    gw[12] = g.S;
+    gw[13] = g.R;
+    gw[14] = g.z;
+  }
+}
+
+// A 64-bit loop constant in two of the loop words (lane 0 writes).
+__device__ inline void put_loop64(int32_t *gw, int k, unsigned long long v) {
+  if ((threadIdx.x & 63) == 0) {
+    gw[kSeqGeoWords + k] = (int32_t)(uint32_t)v;
+    gw[kSeqGeoWords + k + 1] = (int32_t)(uint32_t)(v >> 32);
+  }
+}
+
+__device__ inline unsigned long long get_loop64(const int32_t *gw, int k) {
+  return (unsigned long long)(uint32_t)gw[kSeqGeoWords + k] |
+         ((unsigned long long)(uint32_t)gw[kSeqGeoWords + k + 1] << 32);
+}
+
+__device__ inline SeqGeo get_seq_geo(const int32_t *gw) {
+  SeqGeo sg;
+  sg.lattice = gw[0];
+  for (int a = 0; a < 3; ++a) {
+    sg.n[a] = gw[1 + a];
+    sg.m[a] = gw[4 + a];
+    sg.P[a] = gw[7 + a];
+  }
+  sg.h = gw[10];
+  sg.sub_stride = gw[11];
+  sg.S = gw[12];
+  sg.R = gw[13];
+  sg.z = gw[14];
+  return sg;
+}
+
 struct SiteXYZ {
   int x, y, z;  // 0-based doubled-grid coordinates
 };

@@ -76,9 +76,15 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
   double *E = (double *)(smem + o);
   o += sizeof(double) * A.n_walkers;
   int32_t *offl = (int32_t *)(smem + o);
+  o += sizeof(int32_t) * g.nsub * g.z;
+  // the trial loop's geometry (see put_seq_geo)
+  int32_t *gw = (int32_t *)(smem + ((o + 15) & ~(size_t)15));
 
   for (int i = lane; i < 624; i += 64) st[i] = A.mt[i];
   for (int i = lane; i < g.S * g.S * g.R; i += 64) vex[i] = A.vex[i];
+  put_seq_geo(gw, g);
+  if (lane == 0) gw[kSeqGeoWords] = A.n_steps;
+  put_loop64(gw, 2, (unsigned long long)A.max_redraws);
   stage_offsets(offl, A.off, g);
   PairLanes pl;
   init_pair_lanes(g, pl);
@@ -94,6 +100,9 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
   int status = 0;
   int extra = 0, n_acc = 0;
   const int half = (int)((float)nw / 2.0f);
+  const SeqGeo sg = get_seq_geo(gw);
+  const int n_steps = gw[kSeqGeoWords];
+  const long long max_redraws = (long long)get_loop64(gw, 2);
   for (int it = 1; it <= A.n_iter && status == 0; ++it) {
     // maxloc: the first maximum (nested_sampling.f90:117)
     double best = -INFINITY;
@@ -117,7 +126,7 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
     if (lane == 0) A.e_limits[it - 1] = limit;
     // walk-length adaptation (nested_sampling.f90:129-144)
     if (it % half == 0) {
-      if (((float)n_acc < (float)A.n_at * 0.05f) && (extra < A.n_steps * 100)) extra += A.n_steps;
+      if (((float)n_acc < (float)A.n_at * 0.05f) && (extra < n_steps * 100)) extra += n_steps;
     }
     // clone a random walker over the discarded one (nested_sampling.f90:149-152)
     const double rnd = genrand_dev(st, mti);
@@ -130,28 +139,28 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
     double ecur = E[src];
     n_acc = 0;
     // the walk (nested_sampling.f90:157-192)
-    for (int stp = 1; stp <= A.n_steps + extra; ++stp) {
-      const SiteXYZ r1 = random_site_dev(g, st, mti);
-      const int a1 = padded_addr_xyz(g, r1.x, r1.y, r1.z);
+    for (int stp = 1; stp <= n_steps + extra; ++stp) {
+      const SiteXYZ r1 = random_site_dev(sg, st, mti);
+      const int a1 = padded_addr_xyz(sg, r1.x, r1.y, r1.z);
       const int s1 = lat[a1];
       SiteXYZ r2;
       int a2, s2;
       long long redraws = 0;
       do {
-        r2 = random_site_dev(g, st, mti);
-        a2 = padded_addr_xyz(g, r2.x, r2.y, r2.z);
+        r2 = random_site_dev(sg, st, mti);
+        a2 = padded_addr_xyz(sg, r2.x, r2.y, r2.z);
         s2 = lat[a2];
-      } while (s1 == s2 && ++redraws < A.max_redraws);
+      } while (s1 == s2 && ++redraws < max_redraws);
       if (s1 == s2) {
         status = 1;
         break;
       }
-      const int sub1 = sub_of(g.lattice, r1.x & 1, r1.y & 1);
-      const int sub2 = sub_of(g.lattice, r2.x & 1, r2.y & 1);
+      const int sub1 = sub_of(sg.lattice, r1.x & 1, r1.y & 1);
+      const int sub2 = sub_of(sg.lattice, r2.x & 1, r2.y & 1);
       // energies before and after the swap from one gather (sites marked)
       double eu, es;
-      put_pair(lat, g, r1, sub1, kMarkA, r2, sub2, kMarkB);
-      pair_energy_swap_lanes<KEY>(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, sumbuf, eu, es);
+      put_pair(lat, sg, r1, sub1, kMarkA, r2, sub2, kMarkB);
+      pair_energy_swap_lanes<KEY>(lat, sg, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, sumbuf, eu, es);
       const double de = es - eu;
       ++trials;
       const bool acc = ecur + de < limit;
@@ -160,7 +169,7 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
         ++n_acc;
         ++accepted;
       }
-      put_pair(lat, g, r1, sub1, (int8_t)(acc ? s2 : s1), r2, sub2, (int8_t)(acc ? s1 : s2));
+      put_pair(lat, sg, r1, sub1, (int8_t)(acc ? s2 : s1), r2, sub2, (int8_t)(acc ? s1 : s2));
     }
     __syncthreads();
     if (lane == 0) E[imax] = ecur;
@@ -208,29 +217,17 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
   o += sizeof(double) * hb;
   int32_t *offl = (int32_t *)(smem + o);
   o += sizeof(int32_t) * g.nsub * g.z;
-  // the trial loop's geometry and window, copied to LDS and read back into
-  // registers once: values read from the argument block may be re-read from
-  // it at every use when scalar registers run short
+  // the trial loop's geometry and window (see put_seq_geo)
   int32_t *gw = (int32_t *)(smem + ((o + 15) & ~(size_t)15));
 
   for (int i = lane; i < 624; i += 64) st[i] = A.mt[i];
   for (int i = lane; i < g.S * g.S * g.R; i += 64) vex[i] = A.vex[i];
+  put_seq_geo(gw, g);
   if (lane == 0) {
-    gw[0] = g.lattice;
-    for (int a = 0; a < 3; ++a) {
-      gw[1 + a] = g.n[a];
-      gw[4 + a] = g.m[a];
-      gw[7 + a] = g.P[a];
-    }
-    gw[10] = g.h;
-    gw[11] = g.sub_stride;
-    gw[12] = g.S;
-    gw[13] = g.R;
-    gw[14] = g.z;
-    gw[15] = A.bins;
-    gw[16] = A.start_idx;
-    gw[17] = A.end_idx;
-    gw[18] = A.nbr_swap;
+    gw[kSeqGeoWords + 0] = A.bins;
+    gw[kSeqGeoWords + 1] = A.start_idx;
+    gw[kSeqGeoWords + 2] = A.end_idx;
+    gw[kSeqGeoWords + 3] = A.nbr_swap;
   }
   stage_offsets(offl, A.off, g);
   PairLanes pl;
@@ -241,19 +238,9 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
   stage_lattice(lat, A.compact, g, 0);
   __syncthreads();
   int mti = A.mt_index[0];
-  SeqGeo sg;
-  sg.lattice = gw[0];
-  for (int a = 0; a < 3; ++a) {
-    sg.n[a] = gw[1 + a];
-    sg.m[a] = gw[4 + a];
-    sg.P[a] = gw[7 + a];
-  }
-  sg.h = gw[10];
-  sg.sub_stride = gw[11];
-  sg.S = gw[12];
-  sg.R = gw[13];
-  sg.z = gw[14];
-  const int bins = gw[15], start_idx = gw[16], end_idx = gw[17], nbr_swap = gw[18];
+  const SeqGeo sg = get_seq_geo(gw);
+  const int bins = gw[kSeqGeoWords + 0], start_idx = gw[kSeqGeoWords + 1];
+  const int end_idx = gw[kSeqGeoWords + 2], nbr_swap = gw[kSeqGeoWords + 3];
   const double wl_f = A.wl_f;
   const long long cadence = (long long)(int)(0.02 * (double)(float)g.n_atoms);
   double e_un = ref_total_energy(lat, g, offl, vex);
@@ -518,67 +505,81 @@ __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__
   double *vex = (double *)(smem + o);
   o += sizeof(double) * g.S * g.S * g.R;
   int32_t *offl = (int32_t *)(smem + o);
+  o += sizeof(int32_t) * g.nsub * g.z;
+  // the trial loop's geometry (see put_seq_geo)
+  int32_t *gw = (int32_t *)(smem + ((o + 15) & ~(size_t)15));
 
   for (int i = lane; i < 624; i += 64) st[i] = A.mt[i];
   for (int i = lane; i < g.S * g.S * g.R; i += 64) vex[i] = A.vex[i];
+  put_seq_geo(gw, g);
+  put_loop64(gw, 0, (unsigned long long)__double_as_longlong(A.min_e));
+  put_loop64(gw, 2, (unsigned long long)__double_as_longlong(A.max_e));
+  put_loop64(gw, 4, (unsigned long long)__double_as_longlong(A.denom));
+  put_loop64(gw, 6, (unsigned long long)A.period);
+  put_loop64(gw, 8, (unsigned long long)A.budget);
   stage_offsets(offl, A.off, g);
   PairLanes pl;
   init_pair_lanes(g, pl);
   stage_lattice(lat, A.compact, g, 0);
   __syncthreads();
   int mti = A.mt_index[0];
-  const double target = (A.min_e + A.max_e) / 2.0;
-  const double cond = fabs(A.max_e - A.min_e) * 0.1;
+  const double min_e = __longlong_as_double((long long)get_loop64(gw, 0));
+  const double max_e = __longlong_as_double((long long)get_loop64(gw, 2));
+  const double denom = __longlong_as_double((long long)get_loop64(gw, 4));
+  const long long period = (long long)get_loop64(gw, 6), budget = (long long)get_loop64(gw, 8);
+  const double target = (min_e + max_e) / 2.0;
+  const double cond = fabs(max_e - min_e) * 0.1;
   double e_un = A.init_e ? ref_total_energy(lat, g, offl, vex) : A.e_unswapped[0];
   long long isteps = A.i_steps[0];
   long long trials = A.trials[0];
   long long made = 0;
   bool skip = A.resume[0] != 0;
   int status = 0;
+  const SeqGeo sg = get_seq_geo(gw);
   for (;;) {
     if (!skip) {
       ++isteps;
-      if (isteps % A.period == 0) {
+      if (isteps % period == 0) {
         isteps = 0;
         status = 1;  // the host re-runs initial_setup, then resumes here
         break;
       }
     }
     skip = false;
-    if (e_un < A.max_e - cond && e_un > A.min_e + cond) {
+    if (e_un < max_e - cond && e_un > min_e + cond) {
       e_un = ref_total_energy(lat, g, offl, vex);
-      if (e_un < A.max_e - cond && e_un > A.min_e + cond) {
+      if (e_un < max_e - cond && e_un > min_e + cond) {
         status = 0;
         break;
       }
       continue;
     }
-    if (made >= A.budget) {
+    if (made >= budget) {
       status = 2;
       skip = true;  // resume before this iteration's trial
       break;
     }
     ++made;
     ++trials;
-    const SiteXYZ r1 = random_site_dev(g, st, mti);
-    const SiteXYZ r2 = random_site_dev(g, st, mti);
-    const int a1 = padded_addr_xyz(g, r1.x, r1.y, r1.z);
-    const int a2 = padded_addr_xyz(g, r2.x, r2.y, r2.z);
+    const SiteXYZ r1 = random_site_dev(sg, st, mti);
+    const SiteXYZ r2 = random_site_dev(sg, st, mti);
+    const int a1 = padded_addr_xyz(sg, r1.x, r1.y, r1.z);
+    const int a2 = padded_addr_xyz(sg, r2.x, r2.y, r2.z);
     const int s1 = lat[a1], s2 = lat[a2];
     if (s1 != s2) {
-      const int sub1 = sub_of(g.lattice, r1.x & 1, r1.y & 1);
-      const int sub2 = sub_of(g.lattice, r2.x & 1, r2.y & 1);
+      const int sub1 = sub_of(sg.lattice, r1.x & 1, r1.y & 1);
+      const int sub2 = sub_of(sg.lattice, r2.x & 1, r2.y & 1);
       // energies before and after the swap from one gather (sites marked)
       double pu, ps;
-      put_pair(lat, g, r1, sub1, kMarkA, r2, sub2, kMarkB);
-      pair_energy_swap_lanes<KEY>(lat, g, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, sumbuf, pu, ps);
+      put_pair(lat, sg, r1, sub1, kMarkA, r2, sub2, kMarkB);
+      pair_energy_swap_lanes<KEY>(lat, sg, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, sumbuf, pu, ps);
       const double e_sw = e_un - pu + ps;
       const double de = ((e_sw - target) * (e_sw - target) - (e_un - target) * (e_un - target)) /
-                        A.denom;
+                        denom;
       const double u = genrand_dev(st, mti);
       const bool acc = log_lt_dev(u, -de);
       if (acc) e_un = e_sw;
-      put_pair(lat, g, r1, sub1, (int8_t)(acc ? s2 : s1), r2, sub2, (int8_t)(acc ? s1 : s2));
+      put_pair(lat, sg, r1, sub1, (int8_t)(acc ? s2 : s1), r2, sub2, (int8_t)(acc ? s1 : s2));
     }
   }
   __syncthreads();
@@ -662,8 +663,9 @@ const void *wl_enter_kernel(int key) { BWL_KERNEL_FOR_PAIR_SUM_KEY(k_wl_enter, k
 }  // namespace
 
 size_t ns_lds_bytes(const Geo &g, int n_walkers) {
-  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + kSumBufBytes + sizeof(double) * g.S * g.S * g.R +
-         sizeof(double) * (size_t)n_walkers + sizeof(int32_t) * g.nsub * g.z;
+  return seq_geo_region((size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + kSumBufBytes +
+                        sizeof(double) * g.S * g.S * g.R + sizeof(double) * (size_t)n_walkers +
+                        sizeof(int32_t) * g.nsub * g.z);
 }
 
 hipError_t launch_ns(const NsArgs *args_dev, int n, size_t lds, int max_walkers, size_t lds_energies,
@@ -687,10 +689,9 @@ size_t ns_energies_lds_bytes(const Geo &g) {
 }
 
 size_t wl_lds_bytes(const Geo &g, int bins, int window_bins) {
-  const size_t o = (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + kSumBufBytes +
-                   sizeof(double) * g.S * g.S * g.R + sizeof(double) * (size_t)(2 * bins + 1 + window_bins) +
-                   sizeof(int32_t) * g.nsub * g.z;
-  return ((o + 15) & ~(size_t)15) + sizeof(int32_t) * 20;  // + the loop's geometry words
+  return seq_geo_region((size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + kSumBufBytes +
+                        sizeof(double) * g.S * g.S * g.R +
+                        sizeof(double) * (size_t)(2 * bins + 1 + window_bins) + sizeof(int32_t) * g.nsub * g.z);
 }
 
 hipError_t launch_wl_sweeps(const WlArgs *args_dev, int n, size_t lds, hipStream_t s, int key) {
@@ -737,8 +738,8 @@ hipError_t launch_wl_swap_configs(int8_t *walkers, long long bytes, const int32_
 }
 
 size_t wl_enter_lds_bytes(const Geo &g) {
-  return (size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + kSumBufBytes + sizeof(double) * g.S * g.S * g.R +
-         sizeof(int32_t) * g.nsub * g.z;
+  return seq_geo_region((size_t)((g.lds_bytes + 15) & ~15) + 624 * 4 + kSumBufBytes +
+                        sizeof(double) * g.S * g.S * g.R + sizeof(int32_t) * g.nsub * g.z);
 }
 
 hipError_t launch_wl_enter(const WlEnterArgs *args_dev, int n, size_t lds, hipStream_t s, int key) {
