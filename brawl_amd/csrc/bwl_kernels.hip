@@ -194,10 +194,10 @@ __device__ __forceinline__ void metro_seq_run(const SeqArgs &A, uint8_t *smem) {
     if (sp1 == sp2) {
       acc = true;  // metropolis.F90:774-777: counted as accepted, no draw
     } else {
-      // energies before and after the swap from one gather (sites marked)
+      // energies before and after the swap from one gather of the unswapped lattice
       double e_un, e_sw;
-      put_pair(lat, sg, s1, sub1, kMarkA, s2, sub2, kMarkB);
-      pair_energy_swap_lanes<KEY>(lat, sg, offl, vex, pl, addr1, sub1, addr2, sub2, sp1, sp2, sumbuf, e_un, e_sw);
+      pair_energy_swap_lanes<KEY>(lat, sg, offl, vex, pl, addr1, sub1, s1, addr2, sub2, s2, sp1, sp2, sumbuf, e_un,
+                                  e_sw);
       double de = e_sw - e_un;
       if (de < 0.0) {
         acc = true;
@@ -206,7 +206,7 @@ __device__ __forceinline__ void metro_seq_run(const SeqArgs &A, uint8_t *smem) {
         double u = (double)r * (1.0 / 4294967296.0);
         acc = lt_exp_dev(u, -(beta * de));
       }
-      put_pair(lat, sg, s1, sub1, (int8_t)(acc ? sp2 : sp1), s2, sub2, (int8_t)(acc ? sp1 : sp2));
+      if (acc) put_pair(lat, sg, s1, sub1, (int8_t)sp2, s2, sub2, (int8_t)sp1);
     }
     if (acc) accepted++;
     if (sampling && trial % sample_every == 0) {

@@ -41,9 +41,28 @@ __device__ inline void stage_offsets(int32_t *dst, const int32_t *__restrict__ s
 // shell of each lane's entries and, per 64-entry pass, the mask of entries
 // that end a shell.  Entries 0..z-1 are site A's neighbours in list order,
 // z..2z-1 site B's.
+struct SiteXYZ {
+  int x, y, z;  // 0-based doubled-grid coordinates
+};
+
+// A doubled-coordinate difference (dx, dy, dz) reduced mod the periods m
+// (each below 1024), 10 bits per axis: two sites' differences are equal mod
+// the periodic box exactly when their packed words are equal.
+__device__ inline int pack_dmod(int dx, int dy, int dz, const int32_t *m) {
+  return pmod(dx, m[0]) | (pmod(dy, m[1]) << 10) | (pmod(dz, m[2]) << 20);
+}
+
+// The same for the difference of two sites' coordinates (each in [0, m)).
+__device__ inline int pack_site_diff(SiteXYZ a, SiteXYZ b, const int32_t *m) {
+  const int dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  return (dx < 0 ? dx + m[0] : dx) | ((dy < 0 ? dy + m[1] : dy) << 10) |
+         ((dz < 0 ? dz + m[2] : dz) << 20);
+}
+
 struct PairLanes {
   unsigned long long end_mask[6];
   int shell[6];
+  int dmod[6];  // the lane's neighbour offset mod (m0, m1, m2), 10 bits each
   int passes;
   int fixed;  // compile-time summation variant (0: generic loop)
 };
@@ -185,6 +204,12 @@ __device__ inline void init_pair_lanes(const Geo &g, PairLanes &pl) {
     }
     pl.shell[k] = n;
     pl.end_mask[k] = __ballot(t < 2 * z && last);
+    int d = 0;
+    if (t < 2 * z) {
+      const int8_t *v = g.lattice == 1 ? kFccOffsets[tt] : kBccOffsets[tt];
+      d = pack_dmod(v[0], v[1], v[2], g.m);
+    }
+    pl.dmod[k] = d;
   }
   // straight-line sums for every fcc shell count and bcc up to 6 shells
   pl.fixed = pair_sum_key(g.lattice, g.R);
@@ -336,23 +361,24 @@ __device__ inline double pair_energy_lanes(const int8_t *lat, const Geo &g, cons
   return pair_sum(g, pl, vals, eA_out);
 }
 
-// Marks of the two sites of a pair trial: while their energies before and
-// after the swap are gathered, the LDS lattice holds these codes at both
-// sites (all images) instead of their species.
-constexpr int8_t kMarkA = 0x41, kMarkB = 0x42;
-
-// pair_energy of sites A and B (species spA, spB) before and after the
-// exchange of their species, in the reference's order, from one gather: a
-// neighbour entry that is A or B itself (marked) takes that site's species
-// before or after the swap, every other entry its neighbour's species, so
-// both sums are bit-identical to gathers before and after a real swap.
+// pair_energy of sites A and B (species spA, spB, doubled coordinates sA,
+// sB) before and after the exchange of their species, in the reference's
+// order, from one gather of the lattice as it is: before the swap every
+// entry takes its neighbour's species; after it, an entry whose neighbour
+// is A itself takes spB and one whose neighbour is B takes spA.  A lane of
+// A's list sees B when its offset equals B - A mod the box, and A itself
+// when its offset is 0 mod the box (possible on boxes shorter than the
+// interaction range); likewise for B's list.  Both sums are bit-identical
+// to gathers before and after a real swap.
 template <int KEY = -1, class GEO>
 __device__ inline void pair_energy_swap_lanes(const int8_t *lat, const GEO &g, const int32_t *off,
                                               const double *vex, const PairLanes &pl, int addrA,
-                                              int subA, int addrB, int subB, int spA, int spB,
-                                              double *sumbuf, double &e_before, double &e_after) {
+                                              int subA, SiteXYZ sA, int addrB, int subB, SiteXYZ sB,
+                                              int spA, int spB, double *sumbuf, double &e_before,
+                                              double &e_after) {
   const int lane = threadIdx.x & 63;
   const int z = g.z, S = g.S;
+  const int ab = pack_site_diff(sB, sA, g.m), ba = pack_site_diff(sA, sB, g.m);
   double vb[6], va[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
@@ -364,10 +390,11 @@ __device__ inline void pair_energy_swap_lanes(const int8_t *lat, const GEO &g, c
       const int tt = isA ? t : t - z;
       const int addr = isA ? addrA : addrB, sub = isA ? subA : subB;
       const int nb = lat[addr + off[sub * z + tt]];
-      const int nbb = nb == kMarkA ? spA : (nb == kMarkB ? spB : nb);
-      const int nba = nb == kMarkA ? spB : (nb == kMarkB ? spA : nb);
+      const bool self = pl.dmod[k] == 0, other = pl.dmod[k] == (isA ? ab : ba);
+      const bool nA = isA ? self : other, nB = isA ? other : self;
+      const int nba = nA ? spB : (nB ? spA : nb);
       const int sb = isA ? spA : spB, sa = isA ? spB : spA;
-      vb[k] = vex[(sb - 1) + S * ((nbb - 1) + S * pl.shell[k])];
+      vb[k] = vex[(sb - 1) + S * ((nb - 1) + S * pl.shell[k])];
       va[k] = vex[(sa - 1) + S * ((nba - 1) + S * pl.shell[k])];
     }
   }
@@ -433,9 +460,6 @@ __device__ inline SeqGeo get_seq_geo(const int32_t *gw) {
   return sg;
 }
 
-struct SiteXYZ {
-  int x, y, z;  // 0-based doubled-grid coordinates
-};
 
 // Write species va at site a and vb at site b (every image of each; the
 // whole wavefront helps), between barriers.

@@ -157,10 +157,9 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
       }
       const int sub1 = sub_of(sg.lattice, r1.x & 1, r1.y & 1);
       const int sub2 = sub_of(sg.lattice, r2.x & 1, r2.y & 1);
-      // energies before and after the swap from one gather (sites marked)
+      // energies before and after the swap from one gather of the unswapped lattice
       double eu, es;
-      put_pair(lat, sg, r1, sub1, kMarkA, r2, sub2, kMarkB);
-      pair_energy_swap_lanes<KEY>(lat, sg, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, sumbuf, eu, es);
+      pair_energy_swap_lanes<KEY>(lat, sg, offl, vex, pl, a1, sub1, r1, a2, sub2, r2, s1, s2, sumbuf, eu, es);
       const double de = es - eu;
       ++trials;
       const bool acc = ecur + de < limit;
@@ -169,7 +168,7 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
         ++n_acc;
         ++accepted;
       }
-      put_pair(lat, sg, r1, sub1, (int8_t)(acc ? s2 : s1), r2, sub2, (int8_t)(acc ? s1 : s2));
+      if (acc) put_pair(lat, sg, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
     }
     __syncthreads();
     if (lane == 0) E[imax] = ecur;
@@ -259,10 +258,9 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
     // are equal; neither changes the state, so both are skipped here.
     double e_sw = e_un;
     if (s1 != s2) {
-      // energies before and after the swap from one gather (sites marked)
+      // energies before and after the swap from one gather of the unswapped lattice
       double pu, ps;
-      put_pair(lat, sg, r1, sub1, kMarkA, r2, sub2, kMarkB);
-      pair_energy_swap_lanes<KEY>(lat, sg, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, sumbuf, pu, ps);
+      pair_energy_swap_lanes<KEY>(lat, sg, offl, vex, pl, a1, sub1, r1, a2, sub2, r2, s1, s2, sumbuf, pu, ps);
       e_sw = e_un - pu + ps;
     }
     bool acc = false;
@@ -283,7 +281,7 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
     } else {
       jbin = ibin;
     }
-    if (s1 != s2) put_pair(lat, sg, r1, sub1, (int8_t)(acc ? s2 : s1), r2, sub2, (int8_t)(acc ? s1 : s2));
+    if (s1 != s2 && acc) put_pair(lat, sg, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
     // a walker outside its window would index outside the reference's
     // arrays; such updates are dropped
     if (lane == 0) {
@@ -569,17 +567,16 @@ __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__
     if (s1 != s2) {
       const int sub1 = sub_of(sg.lattice, r1.x & 1, r1.y & 1);
       const int sub2 = sub_of(sg.lattice, r2.x & 1, r2.y & 1);
-      // energies before and after the swap from one gather (sites marked)
+      // energies before and after the swap from one gather of the unswapped lattice
       double pu, ps;
-      put_pair(lat, sg, r1, sub1, kMarkA, r2, sub2, kMarkB);
-      pair_energy_swap_lanes<KEY>(lat, sg, offl, vex, pl, a1, sub1, a2, sub2, s1, s2, sumbuf, pu, ps);
+      pair_energy_swap_lanes<KEY>(lat, sg, offl, vex, pl, a1, sub1, r1, a2, sub2, r2, s1, s2, sumbuf, pu, ps);
       const double e_sw = e_un - pu + ps;
       const double de = ((e_sw - target) * (e_sw - target) - (e_un - target) * (e_un - target)) /
                         denom;
       const double u = genrand_dev(st, mti);
       const bool acc = log_lt_dev(u, -de);
       if (acc) e_un = e_sw;
-      put_pair(lat, sg, r1, sub1, (int8_t)(acc ? s2 : s1), r2, sub2, (int8_t)(acc ? s1 : s2));
+      if (acc) put_pair(lat, sg, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
     }
   }
   __syncthreads();
