@@ -192,6 +192,52 @@ __device__ inline void mt_next3(uint32_t *st, int &mti, uint32_t &r1, uint32_t &
   r3 = mt_next(st, mti);
 }
 
+// A window of 64 consecutive tempered draws held across the wavefront's
+// lanes (lane i: state word base + i), so a draw is one cross-lane read
+// instead of an LDS round trip; the window is refilled (one LDS read per
+// lane) when the draws leave it and after every twist.  The draws and the
+// state are those of mt_next.  A kernel that uses a window takes every draw
+// of that state through it (a twist elsewhere would leave it stale).
+struct MtWin {
+  uint32_t w;
+  int base;  // state index of lane 0's word; kMtWinEmpty before the first fill
+};
+constexpr int kMtWinEmpty = -4096;
+
+__device__ inline void mt_win_fill(const uint32_t *st, int from, MtWin &win) {
+  const int k = from + (int)(threadIdx.x & 63);
+  win.w = mt_temper(st[k < 624 ? k : 623]);  // lanes past the end are never read
+  win.base = from;
+}
+
+__device__ inline uint32_t mt_next(uint32_t *st, int &mti, MtWin &win) {
+  if (mti >= 624) {
+    mt_twist(st);
+    mti = 0;
+    mt_win_fill(st, 0, win);
+  } else if ((unsigned)(mti - win.base) >= 64u) {
+    mt_win_fill(st, mti, win);
+  }
+  const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)win.w, mti - win.base);
+  ++mti;
+  return r;
+}
+
+__device__ inline void mt_next3(uint32_t *st, int &mti, MtWin &win, uint32_t &r1, uint32_t &r2,
+                                uint32_t &r3) {
+  if (mti <= 621 && (unsigned)(mti - win.base) <= 61u) {
+    const int i = mti - win.base;
+    r1 = (uint32_t)__builtin_amdgcn_readlane((int)win.w, i);
+    r2 = (uint32_t)__builtin_amdgcn_readlane((int)win.w, i + 1);
+    r3 = (uint32_t)__builtin_amdgcn_readlane((int)win.w, i + 2);
+    mti += 3;
+    return;
+  }
+  r1 = mt_next(st, mti, win);
+  r2 = mt_next(st, mti, win);
+  r3 = mt_next(st, mti, win);
+}
+
 // ------------------------------------------------------------------
 // Double-double arithmetic (Dekker/Knuth error-free transforms) for a
 // correctly-rounded exp and log, used to decide u < exp(x) and

@@ -174,6 +174,7 @@ __device__ __forceinline__ void metro_seq_run(const SeqArgs &A, uint8_t *smem) {
   stage_lattice(lat, A.compact, g, 0);
   __syncthreads();
   int mti = A.mt_index[0];
+  MtWin win{0u, kMtWinEmpty};
   long long accepted = 0;
   long long sample_k = 0;
   const SeqGeo sg = get_seq_geo(gw);
@@ -183,8 +184,8 @@ __device__ __forceinline__ void metro_seq_run(const SeqArgs &A, uint8_t *smem) {
   const double beta = __longlong_as_double((long long)get_loop64(gw, 6));
 
   for (long long trial = 1; trial <= n_trials; ++trial) {
-    SiteXYZ s1 = random_site_dev(sg, st, mti);
-    SiteXYZ s2 = nbr_swap ? random_nbr_dev(sg, s1, st, mti) : random_site_dev(sg, st, mti);
+    SiteXYZ s1 = random_site_dev(sg, st, mti, win);
+    SiteXYZ s2 = nbr_swap ? random_nbr_dev(sg, s1, st, mti, win) : random_site_dev(sg, st, mti, win);
     int addr1 = padded_addr_xyz(sg, s1.x, s1.y, s1.z);
     int addr2 = padded_addr_xyz(sg, s2.x, s2.y, s2.z);
     int sub1 = sub_of(sg.lattice, s1.x & 1, s1.y & 1);
@@ -202,7 +203,7 @@ __device__ __forceinline__ void metro_seq_run(const SeqArgs &A, uint8_t *smem) {
       if (de < 0.0) {
         acc = true;
       } else {
-        uint32_t r = mt_next(st, mti);
+        uint32_t r = mt_next(st, mti, win);
         double u = (double)r * (1.0 / 4294967296.0);
         acc = lt_exp_dev(u, -(beta * de));
       }

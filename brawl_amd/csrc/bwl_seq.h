@@ -476,9 +476,7 @@ __device__ inline void put_pair(int8_t *lat, const GEO &g, SiteXYZ a, int suba, 
 // bcc_random_site / fcc_random_site (random_site.f90:127-139, 182-193):
 // floor(c*genrand()*n) equals (r*c*n) >> 32 exactly since r*c*n < 2^53.
 template <class GEO>
-__device__ inline SiteXYZ random_site_dev(const GEO &g, uint32_t *st, int &mti) {
-  uint32_t r1, r2, r3;
-  mt_next3(st, mti, r1, r2, r3);
+__device__ inline SiteXYZ site_of_draws(const GEO &g, uint32_t r1, uint32_t r2, uint32_t r3) {
   int k = (int)(((uint64_t)r1 * (uint64_t)(2 * g.n[2])) >> 32) + 1;
   SiteXYZ s;
   if (g.lattice == 2) {
@@ -493,6 +491,20 @@ __device__ inline SiteXYZ random_site_dev(const GEO &g, uint32_t *st, int &mti) 
   return s;
 }
 
+template <class GEO>
+__device__ inline SiteXYZ random_site_dev(const GEO &g, uint32_t *st, int &mti) {
+  uint32_t r1, r2, r3;
+  mt_next3(st, mti, r1, r2, r3);
+  return site_of_draws(g, r1, r2, r3);
+}
+
+template <class GEO>
+__device__ inline SiteXYZ random_site_dev(const GEO &g, uint32_t *st, int &mti, MtWin &win) {
+  uint32_t r1, r2, r3;
+  mt_next3(st, mti, win, r1, r2, r3);
+  return site_of_draws(g, r1, r2, r3);
+}
+
 // random_site.f90:38-61 nearest-neighbour tables (one draw)
 static __constant__ int8_t kRsBcc[8][3] = {{1, 1, 1},   {1, 1, -1},  {1, -1, 1},  {1, -1, -1},
                                     {-1, 1, 1},  {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
@@ -501,8 +513,7 @@ static __constant__ int8_t kRsFcc[12][3] = {{0, 1, 1},  {0, 1, -1},  {0, -1, 1},
                                      {-1, 1, 0}, {-1, -1, 0}, {-1, 0, 1}, {-1, 0, -1}};
 
 template <class GEO>
-__device__ inline SiteXYZ random_nbr_dev(const GEO &g, SiteXYZ s, uint32_t *st, int &mti) {
-  uint32_t r = mt_next(st, mti);
+__device__ inline SiteXYZ nbr_of_draw(const GEO &g, SiteXYZ s, uint32_t r) {
   const int8_t *d;
   if (g.lattice == 2)
     d = kRsBcc[((uint64_t)r * 8u) >> 32];
@@ -513,6 +524,16 @@ __device__ inline SiteXYZ random_nbr_dev(const GEO &g, SiteXYZ s, uint32_t *st, 
   t.y = pmod(s.y + d[1], g.m[1]);
   t.z = pmod(s.z + d[2], g.m[2]);
   return t;
+}
+
+template <class GEO>
+__device__ inline SiteXYZ random_nbr_dev(const GEO &g, SiteXYZ s, uint32_t *st, int &mti) {
+  return nbr_of_draw(g, s, mt_next(st, mti));
+}
+
+template <class GEO>
+__device__ inline SiteXYZ random_nbr_dev(const GEO &g, SiteXYZ s, uint32_t *st, int &mti, MtWin &win) {
+  return nbr_of_draw(g, s, mt_next(st, mti, win));
 }
 
 // Whole-lattice energy by the wavefront: 0.5 * sum of site energies, each

@@ -23,8 +23,41 @@ __device__ inline int wl_bin_index(double e, const double *edges, int bins) {
   return (int)(((e - edges[0]) / range) * (double)bins) + 1;
 }
 
+// bin_index with its constants in registers.  x = ((e - e0) * (1/range)) *
+// bins differs from the reference's ((e - e0) / range) * bins by at most a
+// few units in the last place (under 1e-15 |x|); when x is further than
+// 1e-9 max(1, |x|) from every integer both truncate to the same integer, and
+// otherwise the reference's expression is evaluated.  The result is always
+// that of wl_bin_index.
+struct BinMap {
+  double e0, range, rinv, nb;
+};
+
+__device__ inline BinMap bin_map(const double *edges, int bins) {
+  BinMap b;
+  b.e0 = edges[0];
+  b.range = edges[bins] - edges[0];
+  b.rinv = 1.0 / b.range;
+  b.nb = (double)bins;
+  return b;
+}
+
+__device__ inline int wl_bin_index(double e, const BinMap &b) {
+  const double a = e - b.e0;
+  const double x = (a * b.rinv) * b.nb;
+  const double t = trunc(x);
+  const double f = fabs(x - t);
+  const double tol = 1e-9 * fmax(1.0, fabs(x));
+  if (f > tol && 1.0 - f > tol) return (int)t + 1;
+  return (int)((a / b.range) * b.nb) + 1;
+}
+
 __device__ inline double genrand_dev(uint32_t *st, int &mti) {
   return (double)mt_next(st, mti) * (1.0 / 4294967296.0);
+}
+
+__device__ inline double genrand_dev(uint32_t *st, int &mti, MtWin &win) {
+  return (double)mt_next(st, mti, win) * (1.0 / 4294967296.0);
 }
 
 }  // namespace
@@ -89,6 +122,7 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
   PairLanes pl;
   init_pair_lanes(g, pl);
   int mti = A.mt_index[0];
+  MtWin win{0u, kMtWinEmpty};
   const int nw = A.n_walkers;
   const size_t cb = (size_t)g.compact_bytes;
 
@@ -129,7 +163,7 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
       if (((float)n_acc < (float)A.n_at * 0.05f) && (extra < n_steps * 100)) extra += n_steps;
     }
     // clone a random walker over the discarded one (nested_sampling.f90:149-152)
-    const double rnd = genrand_dev(st, mti);
+    const double rnd = genrand_dev(st, mti, win);
     int irnd = (int)ceil(rnd * (double)nw);
     if (irnd < 1) irnd = 1;
     const int src = irnd - 1;
@@ -140,14 +174,14 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
     n_acc = 0;
     // the walk (nested_sampling.f90:157-192)
     for (int stp = 1; stp <= n_steps + extra; ++stp) {
-      const SiteXYZ r1 = random_site_dev(sg, st, mti);
+      const SiteXYZ r1 = random_site_dev(sg, st, mti, win);
       const int a1 = padded_addr_xyz(sg, r1.x, r1.y, r1.z);
       const int s1 = lat[a1];
       SiteXYZ r2;
       int a2, s2;
       long long redraws = 0;
       do {
-        r2 = random_site_dev(sg, st, mti);
+        r2 = random_site_dev(sg, st, mti, win);
         a2 = padded_addr_xyz(sg, r2.x, r2.y, r2.z);
         s2 = lat[a2];
       } while (s1 == s2 && ++redraws < max_redraws);
@@ -237,18 +271,24 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
   stage_lattice(lat, A.compact, g, 0);
   __syncthreads();
   int mti = A.mt_index[0];
+  MtWin win{0u, kMtWinEmpty};
   const SeqGeo sg = get_seq_geo(gw);
   const int bins = gw[kSeqGeoWords + 0], start_idx = gw[kSeqGeoWords + 1];
   const int end_idx = gw[kSeqGeoWords + 2], nbr_swap = gw[kSeqGeoWords + 3];
   const double wl_f = A.wl_f;
   const long long cadence = (long long)(int)(0.02 * (double)(float)g.n_atoms);
   double e_un = ref_total_energy(lat, g, offl, vex);
+  const BinMap bmap = bin_map(edges, bins);
   int ibin = wl_bin_index(e_un, edges, bins);  // bin of e_un, kept with it
+  // ln g of ibin (0 outside 1..bins), kept with it: lane 0 is the only
+  // writer of lng and writes this value back after every trial
+  double lcur = (ibin >= 1 && ibin <= bins) ? lng[ibin - 1] : 0.0;
   long long accepted = 0;
   const long long total = (long long)A.mc_sweeps * g.n_atoms;
   for (long long i = 1; i <= total; ++i) {
-    const SiteXYZ r1 = random_site_dev(sg, st, mti);
-    const SiteXYZ r2 = nbr_swap ? random_nbr_dev(sg, r1, st, mti) : random_site_dev(sg, st, mti);
+    const SiteXYZ r1 = random_site_dev(sg, st, mti, win);
+    const SiteXYZ r2 =
+        nbr_swap ? random_nbr_dev(sg, r1, st, mti, win) : random_site_dev(sg, st, mti, win);
     const int a1 = padded_addr_xyz(sg, r1.x, r1.y, r1.z);
     const int a2 = padded_addr_xyz(sg, r2.x, r2.y, r2.z);
     const int sub1 = sub_of(sg.lattice, r1.x & 1, r1.y & 1);
@@ -265,16 +305,17 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
     }
     bool acc = false;
     // e_sw == e_un when the species are equal (no swap): same bin
-    int jbin = s1 != s2 ? wl_bin_index(e_sw, edges, bins) : ibin;
+    int jbin = s1 != s2 ? wl_bin_index(e_sw, bmap) : ibin;
     const bool hit = cadence > 0 && (i % cadence) == 0;
     if (jbin > start_idx - 1 && jbin < end_idx + 1) {
-      const double li = (ibin >= 1 && ibin <= bins) ? lng[ibin - 1] : 0.0;
-      const double u = genrand_dev(st, mti);
-      if (log_lt_dev(u, li - lng[jbin - 1])) {
+      const double lj = lng[jbin - 1];
+      const double u = genrand_dev(st, mti, win);
+      if (log_lt_dev(u, lcur - lj)) {
         acc = true;
         ++accepted;
         e_un = e_sw;
         ibin = jbin;
+        lcur = lj;
       } else {
         jbin = ibin;
       }
@@ -284,9 +325,11 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
     if (s1 != s2 && acc) put_pair(lat, sg, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
     // a walker outside its window would index outside the reference's
     // arrays; such updates are dropped
+    const bool in_range = jbin >= 1 && jbin <= bins;
+    if (in_range) lcur = lcur + wl_f;
     if (lane == 0) {
       if (hit && jbin >= start_idx && jbin <= end_idx) hist[jbin - start_idx] += 1.0;
-      if (jbin >= 1 && jbin <= bins) lng[jbin - 1] = lng[jbin - 1] + wl_f;
+      if (in_range) lng[jbin - 1] = lcur;
     }
     __syncthreads();
   }
@@ -521,6 +564,7 @@ __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__
   stage_lattice(lat, A.compact, g, 0);
   __syncthreads();
   int mti = A.mt_index[0];
+  MtWin win{0u, kMtWinEmpty};
   const double min_e = __longlong_as_double((long long)get_loop64(gw, 0));
   const double max_e = __longlong_as_double((long long)get_loop64(gw, 2));
   const double denom = __longlong_as_double((long long)get_loop64(gw, 4));
@@ -559,8 +603,8 @@ __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__
     }
     ++made;
     ++trials;
-    const SiteXYZ r1 = random_site_dev(sg, st, mti);
-    const SiteXYZ r2 = random_site_dev(sg, st, mti);
+    const SiteXYZ r1 = random_site_dev(sg, st, mti, win);
+    const SiteXYZ r2 = random_site_dev(sg, st, mti, win);
     const int a1 = padded_addr_xyz(sg, r1.x, r1.y, r1.z);
     const int a2 = padded_addr_xyz(sg, r2.x, r2.y, r2.z);
     const int s1 = lat[a1], s2 = lat[a2];
@@ -573,7 +617,7 @@ __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__
       const double e_sw = e_un - pu + ps;
       const double de = ((e_sw - target) * (e_sw - target) - (e_un - target) * (e_un - target)) /
                         denom;
-      const double u = genrand_dev(st, mti);
+      const double u = genrand_dev(st, mti, win);
       const bool acc = log_lt_dev(u, -de);
       if (acc) e_un = e_sw;
       if (acc) put_pair(lat, sg, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
