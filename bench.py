@@ -256,10 +256,10 @@ def cpu_walkers(budget_s: float):
     return out
 
 
-def nested_sampling_runs(n_runs: int = 64, n_iter: int = 200):
+def nested_sampling_runs(n_runs: int = 64, n_iter: int = 200, fill_runs: int = 256):
     """Nested-sampling runs of the served stream's NS shape on the device:
     one run alone (one wavefront), then n_runs runs (their own seeds) in
-    one batched launch."""
+    one batched launch, then fill_runs runs (one per CU of the MI355X)."""
     from brawl_amd import Lattice, nested_sampling
     q, walkers = ns_request_shape()
     p = dict(n_walkers=walkers, n_steps=100, n_iter=n_iter, species_numbers=q.counts)
@@ -269,22 +269,25 @@ def nested_sampling_runs(n_runs: int = 64, n_iter: int = 200):
         L.seed(110190)
         _, st = nested_sampling.nested_sampling(L, **p)
     one = st.trials / (st.kernel_ms / 1e3)
-    lats = []
-    for k in range(n_runs):
-        Lk = Lattice(q.lattice, q.n, q.v_ex)
-        Lk.seed(110179 + 11 * k)
-        lats.append(Lk)
-    res, mk = nested_sampling.nested_sampling_batch(lats, [p] * n_runs)
-    for Lk in lats:
-        Lk.close()
-    trials = sum(s.trials for _, s in res)
-    log(f"[bench] nested sampling: one run {one:.3e} trials/s; {n_runs} runs {trials} trials in "
-        f"{mk:.1f} ms")
+    def batch(runs):
+        lats = []
+        for k in range(runs):
+            Lk = Lattice(q.lattice, q.n, q.v_ex)
+            Lk.seed(110179 + 11 * k)
+            lats.append(Lk)
+        res, mk = nested_sampling.nested_sampling_batch(lats, [p] * runs)
+        for Lk in lats:
+            Lk.close()
+        trials = sum(s.trials for _, s in res)
+        log(f"[bench] nested sampling: {runs} runs {trials} trials in {mk:.1f} ms")
+        return {"runs": runs, "trials": trials, "makespan_ms": mk, "value": trials / (mk / 1e3),
+                "unit": "swap trials/s"}
+
+    log(f"[bench] nested sampling: one run {one:.3e} trials/s")
     return {"workload": f"quinary fcc 6^3 (864 sites), 4 shells, {walkers} walkers, walk length 100, "
                         f"{n_iter} iterations",
             "one_run": {"value": one, "unit": "swap trials/s", "us_per_trial": 1e6 / one},
-            "batched": {"runs": n_runs, "trials": trials, "makespan_ms": mk,
-                        "value": trials / (mk / 1e3), "unit": "swap trials/s"}}
+            "batched": batch(n_runs), "batched_filled": batch(fill_runs)}
 
 
 def batched(steps: int, warmup: int, tune: bool = True):
