@@ -341,12 +341,20 @@ def batched(steps: int, warmup: int, tune: bool = True):
         tune_s = time.perf_counter() - t0
         table.apply(lats)
         picks = list(table.choices.values())
-        t_rates, t_mks, t_trials, _ = run("autotuned layouts")
+        # shapes whose tuned thread count differs from the rule's for the tuned layout
+        off_rule, seen = 0, set()
+        for L in lats:
+            k = autotune.shape_key(L)
+            if k in seen or k not in table.choices:
+                continue
+            seen.add(k)
+            off_rule += table.threads[k] != L.layout_info(table.choices[k])[0]
+        t_rates, t_mks, t_trials, _ = run("autotuned layouts and threads")
         out["autotuned"] = {"value": float(np.mean(t_rates)),
                             "device_rate": t_trials / (np.mean(t_mks) / 1e3),
                             "makespan_ms": float(np.mean(t_mks)), "tuning_s": tune_s,
                             "shapes": len(picks), "natural": picks.count("natural"),
-                            "fixed": picks.count("fixed")}
+                            "fixed": picks.count("fixed"), "threads_not_rule": off_rule}
     for L in lats:
         L.close()
     return out

@@ -37,7 +37,7 @@ EXPORTS = (
     "bwl_server_submit_anneal",
     "bwl_server_submit_ns", "bwl_server_submit_wl", "bwl_server_progress", "bwl_server_poll",
     "bwl_server_info", "bwl_server_drain", "bwl_server_memory", "bwl_lds_bandwidth",
-    "bwl_set_layout", "bwl_layouts", "bwl_layout_info", "bwl_mc_steps_batch",
+    "bwl_set_layout", "bwl_layouts", "bwl_layout_info", "bwl_set_threads", "bwl_mc_steps_batch",
     "bwl_random_site", "bwl_random_nbr", "bwl_select_sites", "bwl_metropolis_anneal_noalloc",
 )
 
@@ -163,6 +163,7 @@ def load():
         "bwl_server_memory": (C.c_int, [vp, i64p, i64p]),
         "bwl_lds_bandwidth": (C.c_int, [C.c_int32, dp, i32p]),
         "bwl_set_layout": (C.c_int, [vp, C.c_int32]),
+        "bwl_set_threads": (C.c_int, [vp, C.c_int32]),
         "bwl_layouts": (C.c_int, [vp, i32p]),
         "bwl_layout_info": (C.c_int, [vp, C.c_int32, i32p, i64p]),
         "bwl_mc_steps_batch": (C.c_int, [C.c_int32, P(vp), dp, C.c_int32, C.c_int64, C.c_int64,

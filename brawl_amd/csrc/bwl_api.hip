@@ -457,6 +457,14 @@ int bwl_set_layout(bwl_ctx *c, int32_t layout) {
   return BWL_OK;
 }
 
+int bwl_set_threads(bwl_ctx *c, int32_t threads) {
+  if (!c) return fail(BWL_ERR_ARG, "null context");
+  if (threads < 0 || threads > kMaxNatThreads || threads % 64 != 0)
+    return fail(BWL_ERR_ARG, "threads per chain must be 0 (by rule) or a multiple of 64 up to 1024");
+  c->threads = threads;
+  return BWL_OK;
+}
+
 int bwl_layout_info(bwl_ctx *c, int32_t layout, int32_t *threads, int64_t *lds_bytes) {
   if (!c) return fail(BWL_ERR_ARG, "null context");
   if (layout < BWL_LAYOUT_AUTO || layout > BWL_LAYOUT_FIXED) return fail(BWL_ERR_ARG, "unknown layout");
@@ -584,7 +592,7 @@ int ctx_create_borrowed(const bwl_lattice_desc &d, const double *v_ex, char *mem
 namespace bwl {
 
 int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int &threads,
-                  size_t &lds, bool *natural, int layout) {
+                  size_t &lds, bool *natural, int layout, int threads_req) {
   const int lim = lds_limit();
   int rc = plan_parallel(g, plan, g_err);
   if (rc) return rc;
@@ -617,7 +625,9 @@ int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int
     if (mode < 0 || (layout != 0 && mode != layout - 1)) continue;
     const bool use_nat = mode != 0, fixed = mode == 2;
     const NatGeo *ng = fixed ? &fix : (use_nat ? &per : nullptr);
-    threads = block_size(par_max_threads(g, use_nat, fixed));
+    const int cap = par_max_threads(g, use_nat, fixed);
+    if (threads_req > cap) continue;
+    threads = threads_req > 0 ? threads_req : block_size(cap);
     lds = par_lds_bytes(g, ng, threads);
     if (lds > (size_t)lim) continue;
     if (ng) nat = *ng;
@@ -627,6 +637,8 @@ int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int
     if (natural) *natural = use_nat;
     return BWL_OK;
   }
+  if (threads_req > 0)
+    return fail(BWL_ERR_UNSUPPORTED, "no layout of this lattice runs the requested threads per chain");
   if (layout != 0) return fail(BWL_ERR_UNSUPPORTED, "the requested layout is not available for this lattice");
   return fail(BWL_ERR_UNSUPPORTED, "lattice does not fit in LDS");
 }
@@ -652,7 +664,7 @@ int anneal_plan(int n, bwl_ctx *const *ctxs, const bwl_anneal_params *params, An
     int threads = 0;
     size_t lds = 0;
     bool natural = false;
-    rc = anneal_kernel(g, it.plan, nat, it.fn, threads, lds, &natural, c->layout);
+    rc = anneal_kernel(g, it.plan, nat, it.fn, threads, lds, &natural, c->layout, c->threads);
     if (rc) return rc;
     if (p->asro_every < 0 || p->asro_every > p->n_sweeps)
       return fail(BWL_ERR_ARG, "asro_every must be in 0..n_sweeps");

@@ -48,6 +48,7 @@ struct bwl_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool borrowed = false;  // device memory and stream belong to someone else
   int32_t layout = 0;     // parallel-sampler layout: 0 by rule, else BWL_LAYOUT_*
+  int32_t threads = 0;    // parallel-sampler threads per chain: 0 by rule
 };
 
 namespace bwl {
@@ -116,9 +117,11 @@ struct AnnealPlan {
 };
 // The parallel-sampler kernel variant, block size and LDS for a lattice:
 // layout 0 takes the first that fits of fixed pitch, own pitch, per
-// sublattice; a BWL_LAYOUT_* value takes only that one.
+// sublattice; a BWL_LAYOUT_* value takes only that one.  threads_req > 0
+// asks for that many threads per chain instead of the rule's count (layout
+// 0 then takes the first layout that can run them).
 int anneal_kernel(const Geo &g, ParPlan &plan, NatGeo &nat, const void *&fn, int &threads,
-                  size_t &lds, bool *natural = nullptr, int layout = 0);
+                  size_t &lds, bool *natural = nullptr, int layout = 0, int threads_req = 0);
 int anneal_plan(int n, bwl_ctx *const *ctxs, const bwl_anneal_params *params, AnnealPlan &P);
 // zero_by_kernel: clear the counters with a kernel instead of a memset
 int anneal_upload(AnnealPlan &P, char *scratch_dev, hipStream_t s, bool zero_by_kernel = false);

@@ -362,6 +362,13 @@ module brawl_hip
       integer(c_int32_t), value :: layout
     end function
 
+    ! threads per chain of the parallel sampler (no reference counterpart)
+    integer(c_int) function bwl_set_threads(ctx, threads) bind(C, name="bwl_set_threads")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: threads
+    end function
+
     integer(c_int) function bwl_layouts(ctx, mask) bind(C, name="bwl_layouts")
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx

@@ -92,6 +92,7 @@ class Lattice:
         check(L.bwl_n_atoms(h, C.byref(na)))
         self.n_atoms = int(na.value)
         self.layout = "auto"
+        self.threads = 0
 
     # -- lifetime ---------------------------------------------------------
     def close(self):
@@ -228,6 +229,12 @@ class Lattice:
         code = _lib.LAYOUTS[layout] if isinstance(layout, str) else int(layout)
         check(self._L.bwl_set_layout(self._h, code))
         self.layout = layout if isinstance(layout, str) else {v: k for k, v in _lib.LAYOUTS.items()}[code]
+
+    def set_threads(self, threads: int) -> None:
+        """Threads per chain of the parallel sampler for later anneals: 0 by
+        rule, else a multiple of 64 up to 1024 (see bwl_set_threads)."""
+        check(self._L.bwl_set_threads(self._h, int(threads)))
+        self.threads = int(threads)
 
     def layouts(self) -> list:
         """The layouts available to this lattice, in the rule's order of

@@ -234,6 +234,19 @@ int bwl_set_layout(bwl_ctx *ctx, int32_t layout);
 int bwl_layouts(bwl_ctx *ctx, int32_t *mask);
 int bwl_layout_info(bwl_ctx *ctx, int32_t layout, int32_t *threads, int64_t *lds_bytes);
 
+/* Threads per chain of the parallel sampler for the context's later
+ * anneals (no counterpart in the reference: a tuning knob of this
+ * sampler): 0 (the default) takes the rule's count (the fewest threads
+ * that need the fewest passes over a round's pairs); otherwise a multiple
+ * of 64 up to 1024, else BWL_ERR_ARG.  With BWL_LAYOUT_AUTO the first
+ * layout that can run that many threads is taken; an anneal whose layout
+ * cannot (the fixed-pitch kernels take up to 1024 threads, the others up
+ * to 512) fails with BWL_ERR_UNSUPPORTED.  The thread count is part of a
+ * chain's definition (its random numbers are keyed by thread and step):
+ * equal thread counts give the same chain in either lattice-coordinate
+ * layout, different counts give different, equally exact chains. */
+int bwl_set_threads(bwl_ctx *ctx, int32_t threads);
+
 /* Batched form: n independent chains (contexts on one device, uneven
  * lattice sizes, compositions and schedules) annealed in as few launches
  * as possible -- one workgroup per chain, chains with the same kernel

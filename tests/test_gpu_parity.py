@@ -425,9 +425,10 @@ def test_layouts_same_chain_at_equal_threads_and_exact_otherwise(hip, oracle):
 
 
 def test_autotuned_batch_matches_single_runs(hip, oracle):
-    """Layouts picked by the autotuner per shape, saved and loaded as JSON:
-    the batch gives exactly what each chain gives alone with the same
-    layout, and the tracked energies equal recounts."""
+    """Layouts and thread counts picked by the autotuner per shape, saved
+    and loaded as JSON: the batch gives exactly what each chain gives alone
+    with the same layout and thread count, and the tracked energies equal
+    recounts."""
     from brawl_amd import autotune, workloads
     chains = workloads.config3(n_chains=8, n_sweeps=1)
     lats, params = [], []
@@ -449,6 +450,7 @@ def test_autotuned_batch_matches_single_runs(hip, oracle):
         f = L.get_config()
         L2 = hip.Lattice(L.lattice, L.n, L.v_ex)
         L2.set_layout(table.get(L))
+        L2.set_threads(table.get_threads(L))
         L2.set_config(s0)
         r1 = L2.metropolis_simulated_annealing(**p)
         assert np.array_equal(L2.get_config(), f)
@@ -458,6 +460,71 @@ def test_autotuned_batch_matches_single_runs(hip, oracle):
         assert abs(ref.total_energy(f) - r.energies_of_T[0] * ref.n_atoms) <= 1e-11 * ref.n_atoms
     for L in lats:
         L.close()
+
+
+def test_thread_count_override(hip, oracle):
+    """bwl_set_threads: the rule's count given explicitly reproduces the
+    default chain; another count (here two passes over each round's 108
+    pairs instead of one) gives an exact chain (composition kept, tracked
+    energy equal to a recount), the same in both lattice-coordinate layouts
+    and alone or batched; counts that are not a multiple of 64 up to 1024
+    are argument errors, and a count above what the layout's kernel runs
+    fails at the anneal."""
+    from brawl_amd import workloads
+    lat, n, S = 2, (12, 12, 12), 3
+    v = workloads.random_v(S, [25.0 / k for k in range(1, 4)], 11)
+    counts = workloads.dirichlet_counts(workloads.n_atoms_of(lat, n), S, np.random.default_rng(5))
+    ref = oracle.OracleLattice(lat, n, v)
+    cfg = ref.initial_setup(oracle.Generator(110179), species_numbers=counts)
+    p = dict(T=900.0, n_sweeps=1, burn_in_sweeps=2, seed=77)   # one sample: the final state
+
+    def run(layout, threads):
+        with hip.Lattice(lat, n, v) as L:
+            L.set_layout(layout)
+            L.set_threads(threads)
+            L.set_config(cfg)
+            r = L.metropolis_simulated_annealing(**p)
+            return L.get_config(), r
+
+    rule = None
+    with hip.Lattice(lat, n, v) as L:
+        names = L.layouts()
+        assert "natural" in names and "fixed" in names
+        rule = L.layout_info("fixed")[0]
+        for bad in (-64, 32, 100, 1088):
+            with pytest.raises(hip.BrawlError):
+                L.set_threads(bad)
+    _say("threads: rule", rule)
+    o0, r0 = run("fixed", 0)
+    o1, r1 = run("fixed", rule)
+    assert np.array_equal(o0, o1) and r0.energies_of_T[0] == r1.energies_of_T[0]
+    other = 64 if rule != 64 else 128
+    o2, r2 = run("fixed", other)
+    o3, r3 = run("natural", other)
+    assert np.array_equal(o2, o3) and r2.energies_of_T[0] == r3.energies_of_T[0]
+    assert r2.accepted == r3.accepted
+    S1 = S + 1
+    assert np.array_equal(np.bincount(o2.ravel(), minlength=S1), np.bincount(cfg.ravel(), minlength=S1))
+    e_ref = ref.total_energy(o2)
+    assert abs(r2.energies_of_T[0] * ref.n_atoms - e_ref) <= 1e-11 * max(1.0, abs(e_ref))
+    # alone or in a batch with a chain at the rule's count
+    lats = [hip.Lattice(lat, n, v), hip.Lattice(lat, n, v)]
+    for L, t in zip(lats, (other, 0)):
+        L.set_layout("fixed")
+        L.set_threads(t)
+        L.set_config(cfg)
+    res, _ = hip.anneal_batch(lats, [p, p])
+    assert np.array_equal(lats[0].get_config(), o2) and res[0].energies_of_T[0] == r2.energies_of_T[0]
+    assert np.array_equal(lats[1].get_config(), o0) and res[1].energies_of_T[0] == r0.energies_of_T[0]
+    for L in lats:
+        L.close()
+    # the compact layout's kernel runs at most 512 threads
+    with hip.Lattice(lat, n, v) as L:
+        L.set_layout("compact")
+        L.set_threads(1024)
+        L.set_config(cfg)
+        with pytest.raises(hip.BrawlError):
+            L.metropolis_simulated_annealing(**p)
 
 
 @pytest.mark.parametrize("mixed", [False, True])
