@@ -659,6 +659,27 @@ def sequential_bit_exact(sweeps: int = 100):
                          "note": "one chain per rank, each the reference's trajectory bit for bit"}}
 
 
+def largest_batch_chains(k: int, sweeps: int = 100):
+    """The k chains of the 256-chain batch with the most trials, each run
+    alone for `sweeps` sweeps at its own temperature: the chains that set
+    the batch's makespan, for timing and profiling them apart."""
+    from brawl_amd import Lattice, workloads
+    chains = sorted(workloads.config3(), key=lambda c: -workloads.n_atoms_of(c.lattice, c.n))[:k]
+    out = []
+    for c in chains:
+        with Lattice(c.lattice, c.n, c.v_ex) as L:
+            L.seed(c.seed & 0xFFFFFFFF)
+            L.initial_setup(species_numbers=c.counts)
+            r = L.metropolis_simulated_annealing(T=c.T, n_sweeps=sweeps, seed=c.seed)
+            out.append({"lattice": "fcc" if c.lattice == 1 else "bcc", "n": list(c.n),
+                        "species": len(c.counts), "T": c.T, "threads": L.layout_info("auto")[0],
+                        "trials": r.trials, "kernel_ms": r.kernel_ms,
+                        "value": r.trials / (r.kernel_ms / 1e3)})
+            log(f"[bench] chain {out[-1]['lattice']} {c.n} S={len(c.counts)}: {r.trials} trials in "
+                f"{r.kernel_ms:.2f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -682,9 +703,15 @@ def main():
     ap.add_argument("--served-budget-mb", type=int, default=1024)
     ap.add_argument("--served-wait-us", type=int, default=5000)
     ap.add_argument("--served-plain-every", type=int, default=4)
+    ap.add_argument("--largest-batch-chains", type=int, default=0, metavar="K",
+                    help="only: the K chains of the batched workload with the most trials, each "
+                         "alone for 100 sweeps (for profiling them)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.largest_batch_chains > 0:
+        print(json.dumps({"largest_batch_chains": largest_batch_chains(args.largest_batch_chains)}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "brawl_amd runs on one GPU; --gpus must be 1"}))
