@@ -12,6 +12,16 @@
 
 namespace bwl {
 
+// Keeps the compiler from moving memory accesses across this point, without
+// a barrier or a wait: in a workgroup of one wavefront the LDS accesses of
+// the wavefront complete in program order (an LDS read issued after an LDS
+// write sees it, whichever lanes made them), so ordering the instructions
+// orders the accesses.
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_wave_barrier();
+  __asm__ __volatile__("" ::: "memory");
+}
+
 // Positive modulo for a small positive divisor.
 __device__ __forceinline__ int pmod(int a, int m) {
   int r = a % m;
@@ -386,5 +396,6 @@ __device__ inline bool log_lt_dev(double u, double t) {
   }
   return log_lt(u, t);
 }
+
 
 }  // namespace bwl

@@ -462,15 +462,17 @@ __device__ inline SeqGeo get_seq_geo(const int32_t *gw) {
 
 
 // Write species va at site a and vb at site b (every image of each; the
-// whole wavefront helps), between barriers.
+// whole wavefront helps).  For kernels of one wavefront per workgroup: the
+// writes are ordered after every earlier LDS access and before every later
+// one (wave_lds_order), with no barrier.
 template <class GEO>
 __device__ inline void put_pair(int8_t *lat, const GEO &g, SiteXYZ a, int suba, int8_t va, SiteXYZ b,
                                 int subb, int8_t vb) {
   const int lane = threadIdx.x & 63;
-  __syncthreads();
+  wave_lds_order();
   write_site(lat, g, suba, a.x >> 1, a.y >> 1, a.z >> 1, va, lane, 64);
   write_site(lat, g, subb, b.x >> 1, b.y >> 1, b.z >> 1, vb, lane, 64);
-  __syncthreads();
+  wave_lds_order();
 }
 
 // bcc_random_site / fcc_random_site (random_site.f90:127-139, 182-193):

@@ -331,8 +331,9 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
       if (hit && jbin >= start_idx && jbin <= end_idx) hist[jbin - start_idx] += 1.0;
       if (in_range) lng[jbin - 1] = lcur;
     }
-    __syncthreads();
+    wave_lds_order();  // one wavefront: the next trial's LDS reads follow these writes
   }
+  __syncthreads();
   unstage_lattice(lat, A.compact, g, 0);
   for (int i = lane; i < A.bins; i += 64) A.lng[i] = lng[i];
   for (int i = lane; i < hb; i += 64) A.hist[i] = hist[i];
