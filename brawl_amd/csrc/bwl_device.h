@@ -22,6 +22,11 @@ __device__ __forceinline__ void wave_lds_order() {
   __asm__ __volatile__("" ::: "memory");
 }
 
+// v as a wavefront-uniform value (read from the first active lane), for a
+// value that every lane computes alike: the compiler keeps it in a scalar
+// register and branches on it without exec-mask bookkeeping.
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // Positive modulo for a small positive divisor.
 __device__ __forceinline__ int pmod(int a, int m) {
   int r = a % m;
@@ -395,6 +400,16 @@ __device__ inline bool log_lt_dev(double u, double t) {
     if (lf > t + 1e-4) return false;
   }
   return log_lt(u, t);
+}
+
+// log_lt_dev for a u and t that every lane holds alike.
+__device__ inline bool log_lt_dev_uniform(double u, double t) {
+  if (wave_uniform(u >= 0x1p-32 && t > -1e30 && t < 1e30)) {
+    const double lf = (double)__logf((float)u);
+    if (wave_uniform(lf < t - 1e-4)) return true;
+    if (wave_uniform(lf > t + 1e-4)) return false;
+  }
+  return wave_uniform(log_lt(u, t));
 }
 
 

@@ -23,7 +23,8 @@ __device__ inline int wl_bin_index(double e, const double *edges, int bins) {
   return (int)(((e - edges[0]) / range) * (double)bins) + 1;
 }
 
-// bin_index with its constants in registers.  x = ((e - e0) * (1/range)) *
+// bin_index with its constants in registers, for an energy that every lane
+// holds alike (the result is wavefront-uniform).  x = ((e - e0) * (1/range)) *
 // bins differs from the reference's ((e - e0) / range) * bins by at most a
 // few units in the last place (under 1e-15 |x|); when x is further than
 // 1e-9 max(1, |x|) from every integer both truncate to the same integer, and
@@ -48,8 +49,8 @@ __device__ inline int wl_bin_index(double e, const BinMap &b) {
   const double t = trunc(x);
   const double f = fabs(x - t);
   const double tol = 1e-9 * fmax(1.0, fabs(x));
-  if (f > tol && 1.0 - f > tol) return (int)t + 1;
-  return (int)((a / b.range) * b.nb) + 1;
+  if (wave_uniform(f > tol && 1.0 - f > tol)) return wave_uniform((int)t + 1);
+  return wave_uniform((int)((a / b.range) * b.nb) + 1);
 }
 
 __device__ inline double genrand_dev(uint32_t *st, int &mti) {
@@ -293,7 +294,7 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
     const int a2 = padded_addr_xyz(sg, r2.x, r2.y, r2.z);
     const int sub1 = sub_of(sg.lattice, r1.x & 1, r1.y & 1);
     const int sub2 = sub_of(sg.lattice, r2.x & 1, r2.y & 1);
-    const int s1 = lat[a1], s2 = lat[a2];
+    const int s1 = wave_uniform(lat[a1]), s2 = wave_uniform(lat[a2]);
     // The reference also evaluates pair_energy and swaps when the species
     // are equal; neither changes the state, so both are skipped here.
     double e_sw = e_un;
@@ -310,7 +311,7 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
     if (jbin > start_idx - 1 && jbin < end_idx + 1) {
       const double lj = lng[jbin - 1];
       const double u = genrand_dev(st, mti, win);
-      if (log_lt_dev(u, lcur - lj)) {
+      if (log_lt_dev_uniform(u, lcur - lj)) {
         acc = true;
         ++accepted;
         e_un = e_sw;
