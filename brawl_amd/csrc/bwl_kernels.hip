@@ -183,6 +183,7 @@ __device__ __forceinline__ void metro_seq_run(const SeqArgs &A, uint8_t *smem) {
   const long long n_trials = (long long)get_loop64(gw, 2), sample_every = (long long)get_loop64(gw, 4);
   const double beta = __longlong_as_double((long long)get_loop64(gw, 6));
 
+  long long to_sample = sample_every;  // trial % sample_every == 0, counted down
   for (long long trial = 1; trial <= n_trials; ++trial) {
     SiteXYZ s1 = random_site_dev(sg, st, mti, win);
     SiteXYZ s2 = nbr_swap ? random_nbr_dev(sg, s1, st, mti, win) : random_site_dev(sg, st, mti, win);
@@ -210,7 +211,8 @@ __device__ __forceinline__ void metro_seq_run(const SeqArgs &A, uint8_t *smem) {
       if (acc) put_pair(lat, sg, s1, sub1, (int8_t)sp2, s2, sub2, (int8_t)sp1);
     }
     if (acc) accepted++;
-    if (sampling && trial % sample_every == 0) {
+    if (sampling && --to_sample == 0) {
+      to_sample = sample_every;
       const double e = ref_total_energy(lat, g, offl, vex);
       if (lane == 0) A.energies[sample_k] = e;
       sample_k++;

@@ -286,6 +286,9 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
   double lcur = (ibin >= 1 && ibin <= bins) ? lng[ibin - 1] : 0.0;
   long long accepted = 0;
   const long long total = (long long)A.mc_sweeps * g.n_atoms;
+  // trials until the next histogram update: i % cadence == 0, counted down
+  // (a 64-bit remainder per trial costs more than the rest of the decision)
+  long long to_hit = cadence;
   for (long long i = 1; i <= total; ++i) {
     const SiteXYZ r1 = random_site_dev(sg, st, mti, win);
     const SiteXYZ r2 =
@@ -307,7 +310,8 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
     bool acc = false;
     // e_sw == e_un when the species are equal (no swap): same bin
     int jbin = s1 != s2 ? wl_bin_index(e_sw, bmap) : ibin;
-    const bool hit = cadence > 0 && (i % cadence) == 0;
+    const bool hit = cadence > 0 && --to_hit == 0;
+    if (hit) to_hit = cadence;
     if (jbin > start_idx - 1 && jbin < end_idx + 1) {
       const double lj = lng[jbin - 1];
       const double u = genrand_dev(st, mti, win);
@@ -580,10 +584,12 @@ __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__
   bool skip = A.resume[0] != 0;
   int status = 0;
   const SeqGeo sg = get_seq_geo(gw);
+  // iterations until isteps is next a multiple of period, counted down
+  long long to_period = period - isteps % period;
   for (;;) {
     if (!skip) {
       ++isteps;
-      if (isteps % period == 0) {
+      if (--to_period == 0) {
         isteps = 0;
         status = 1;  // the host re-runs initial_setup, then resumes here
         break;
