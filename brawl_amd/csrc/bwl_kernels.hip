@@ -150,6 +150,7 @@ __device__ __forceinline__ void metro_seq_run(const SeqArgs &A, uint8_t *smem) {
   uint32_t *st = (uint32_t *)(smem + o);
   o += 624 * 4;
   double *sumbuf = (double *)(smem + o);
+  clear_sum_buf(sumbuf);
   o += kSumBufBytes;
   double *vex = (double *)(smem + o);
   o += sizeof(double) * g.S * g.S * g.R;

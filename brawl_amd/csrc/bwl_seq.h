@@ -62,7 +62,8 @@ __device__ inline int pack_site_diff(SiteXYZ a, SiteXYZ b, const int32_t *m) {
 struct PairLanes {
   unsigned long long end_mask[6];
   int shell[6];
-  int dmod[6];  // the lane's neighbour offset mod (m0, m1, m2), 10 bits each
+  int dmod[6];   // the lane's neighbour offset mod (m0, m1, m2), 10 bits each
+  int chain[6];  // the lane's slot in the shell-chain layout (fixed_pair_sum2_chains)
   int passes;
   int fixed;  // compile-time summation variant (0: generic loop)
 };
@@ -73,6 +74,16 @@ __host__ __device__ constexpr int shell_first(int n) {
   int s = 0;
   for (int k = 0; k < n; ++k) s += LAT == 1 ? kFccShellSize[k] : kBccShellSize[k];
   return s;
+}
+
+// Largest shell of the first r shells of a lattice (lattice 1: fcc).
+__host__ __device__ constexpr int shell_max(int lattice, int r) {
+  int m = 0;
+  for (int k = 0; k < r; ++k) {
+    const int v = lattice == 1 ? kFccShellSize[k] : kBccShellSize[k];
+    m = v > m ? v : m;
+  }
+  return m;
 }
 
 // The ordered sum of pair_energy_lanes with the shell structure known at
@@ -134,55 +145,10 @@ __device__ __forceinline__ void fixed_pair_sum2(const double (&v1)[6], const dou
   e2 = a2 + b2;
 }
 
-// Bytes of LDS for fixed_pair_sum2_lds: two sums of up to 3 passes of 64
-// entries (the straight-line variants need at most 172 entries per sum).
-constexpr int kSumBufBytes = 2 * 3 * 64 * 8;
+// Bytes of LDS for the sums' entries (fixed_pair_sum2_chains: 4R chains of
+// a stride of shell_max | 1 doubles, at most 24 x 25).
+constexpr int kSumBufBytes = 4800;
 
-// fixed_pair_sum2 with the lanes' values passed through LDS (`buf`,
-// kSumBufBytes) instead of lane reads: every lane stores its entries, then
-// reads them back in list order, one 64-bit add per entry instead of two
-// lane reads and an add.  The same values are added in the same order, so
-// the results are bit-identical.  One wavefront per workgroup: its LDS
-// accesses complete in program order, so no barrier is needed.
-template <int LAT, int R>
-__device__ __forceinline__ void fixed_pair_sum2_lds(const double (&v1)[6], const double (&v2)[6],
-                                                    double *buf, double &e1, double &e2) {
-  constexpr int z = shell_first<LAT>(R);
-  constexpr int P = (2 * z + 63) / 64;
-  static_assert(2 * P * 64 * 8 <= kSumBufBytes, "sum buffer too small");
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int k = 0; k < P; ++k) {
-    buf[k * 64 + lane] = v1[k];
-    buf[(P + k) * 64 + lane] = v2[k];
-  }
-  const double *b1 = buf, *b2 = buf + P * 64;
-  double a1 = 0.0, bb1 = 0.0, a2 = 0.0, bb2 = 0.0;
-#pragma unroll
-  for (int n = 0; n < R; ++n) {
-    double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-    for (int t = shell_first<LAT>(n); t < shell_first<LAT>(n + 1); ++t) {
-      s1 = s1 + b1[t];
-      s2 = s2 + b2[t];
-    }
-    a1 = n == 0 ? s1 : a1 + s1;
-    a2 = n == 0 ? s2 : a2 + s2;
-  }
-#pragma unroll
-  for (int n = 0; n < R; ++n) {
-    double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-    for (int t = z + shell_first<LAT>(n); t < z + shell_first<LAT>(n + 1); ++t) {
-      s1 = s1 + b1[t];
-      s2 = s2 + b2[t];
-    }
-    bb1 = n == 0 ? s1 : bb1 + s1;
-    bb2 = n == 0 ? s2 : bb2 + s2;
-  }
-  e1 = a1 + bb1;
-  e2 = a2 + bb2;
-}
 
 __device__ inline void init_pair_lanes(const Geo &g, PairLanes &pl) {
   const int lane = threadIdx.x & 63;
@@ -210,6 +176,9 @@ __device__ inline void init_pair_lanes(const Geo &g, PairLanes &pl) {
       d = pack_dmod(v[0], v[1], v[2], g.m);
     }
     pl.dmod[k] = d;
+    // chain slot: (site * R + shell) * (shell_max | 1) + index within the shell
+    const int st = shell_max(g.lattice, g.R) | 1;
+    pl.chain[k] = t < 2 * z ? ((t < z ? 0 : g.R) + n) * st + (tt - bounds[n]) : 0;
   }
   // straight-line sums for every fcc shell count and bcc up to 6 shells
   pl.fixed = pair_sum_key(g.lattice, g.R);
@@ -296,6 +265,53 @@ __device__ inline void pair_sum2(const GEO &g, const PairLanes &pl, const double
   e2 = pair_sum(g, pl, v2, nullptr);
 }
 
+// fixed_pair_sum2 with the shells' sums formed side by side in lanes: the
+// 4R chains (before / after the swap, site A / B, shell n) each run in a
+// lane of their own over the shell's entries in list order, from 0.0, then
+// every lane adds each site's shell sums left to right and e_A + e_B.  The
+// entries are passed through `buf` (kSumBufBytes, its chain slots past each
+// shell's length zero: adding +0.0 to a sum that started at +0.0 leaves it
+// unchanged, since such a sum is never -0.0), so every addition is the one
+// of the reference's order, on the same values, and the results are
+// bit-identical, with 4R-fold fewer additions per wavefront.  One wavefront
+// per workgroup (its LDS accesses complete in program order).
+template <int LAT, int R>
+__device__ __forceinline__ void fixed_pair_sum2_chains(const PairLanes &pl, const double (&v1)[6],
+                                                       const double (&v2)[6], double *buf, double &e1,
+                                                       double &e2) {
+  constexpr int z = shell_first<LAT>(R);
+  constexpr int P = (2 * z + 63) / 64;
+  constexpr int ML = shell_max(LAT, R), ST = ML | 1, NC = 4 * R;
+  static_assert(NC * ST * 8 <= kSumBufBytes && NC <= 64, "sum buffer too small");
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    if (64 * k + lane < 2 * z) {
+      buf[pl.chain[k]] = v1[k];
+      buf[2 * R * ST + pl.chain[k]] = v2[k];
+    }
+  }
+  const double *cb = buf + (lane < NC ? lane : NC - 1) * ST;
+  double c = 0.0;
+#pragma unroll
+  for (int j = 0; j < ML; ++j) c = c + cb[j];
+  double a[4];
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {  // (before, A), (before, B), (after, A), (after, B)
+    a[w] = readlane_d(c, w * R);
+#pragma unroll
+    for (int n = 1; n < R; ++n) a[w] = a[w] + readlane_d(c, w * R + n);
+  }
+  e1 = a[0] + a[1];
+  e2 = a[2] + a[3];
+}
+
+// Clears the chain slots of the sum buffer (at kernel start: the chains'
+// padding slots are never written afterwards).
+__device__ inline void clear_sum_buf(double *buf) {
+  for (int i = threadIdx.x; i < kSumBufBytes / 8; i += blockDim.x) buf[i] = 0.0;
+}
+
 // pair_sum2 for a variant fixed at compile time (KEY = pair_sum_key, which
 // must match pl.fixed; the straight-line sums read the entries back from
 // `sumbuf` in LDS), or chosen at run time (KEY < 0): a kernel compiled for
@@ -304,7 +320,7 @@ template <int KEY, class GEO>
 __device__ inline void pair_sum2_t(const GEO &g, const PairLanes &pl, const double (&v1)[6],
                                    const double (&v2)[6], double *sumbuf, double &e1, double &e2) {
   if constexpr (KEY > 0) {
-    fixed_pair_sum2_lds<KEY / 16, KEY % 16>(v1, v2, sumbuf, e1, e2);
+    fixed_pair_sum2_chains<KEY / 16, KEY % 16>(pl, v1, v2, sumbuf, e1, e2);
   } else if constexpr (KEY == 0) {
     e1 = pair_sum_loop(g, pl, v1, nullptr);
     e2 = pair_sum_loop(g, pl, v2, nullptr);

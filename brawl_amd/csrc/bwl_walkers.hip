@@ -104,6 +104,7 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
   uint32_t *st = (uint32_t *)(smem + o);
   o += 624 * 4;
   double *sumbuf = (double *)(smem + o);
+  clear_sum_buf(sumbuf);
   o += kSumBufBytes;
   double *vex = (double *)(smem + o);
   o += sizeof(double) * g.S * g.S * g.R;
@@ -240,6 +241,7 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
   uint32_t *st = (uint32_t *)(smem + o);
   o += 624 * 4;
   double *sumbuf = (double *)(smem + o);
+  clear_sum_buf(sumbuf);
   o += kSumBufBytes;
   double *vex = (double *)(smem + o);
   o += sizeof(double) * g.S * g.S * g.R;
@@ -548,6 +550,7 @@ __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__
   uint32_t *st = (uint32_t *)(smem + o);
   o += 624 * 4;
   double *sumbuf = (double *)(smem + o);
+  clear_sum_buf(sumbuf);
   o += kSumBufBytes;
   double *vex = (double *)(smem + o);
   o += sizeof(double) * g.S * g.S * g.R;
