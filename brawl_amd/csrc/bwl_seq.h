@@ -112,43 +112,9 @@ __device__ __forceinline__ double fixed_pair_sum(const double (&vals)[6], double
   return eA + eB;
 }
 
-// Two ordered sums of fixed_pair_sum at once (independent chains, so the
-// adds of one fill the latency of the other), bit-identical to two calls.
-template <int LAT, int R>
-__device__ __forceinline__ void fixed_pair_sum2(const double (&v1)[6], const double (&v2)[6],
-                                                double &e1, double &e2) {
-  constexpr int z = shell_first<LAT>(R);
-  double a1 = 0.0, b1 = 0.0, a2 = 0.0, b2 = 0.0;
-#pragma unroll
-  for (int n = 0; n < R; ++n) {
-    double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-    for (int t = shell_first<LAT>(n); t < shell_first<LAT>(n + 1); ++t) {
-      s1 = s1 + readlane_d(v1[t >> 6], t & 63);
-      s2 = s2 + readlane_d(v2[t >> 6], t & 63);
-    }
-    a1 = n == 0 ? s1 : a1 + s1;
-    a2 = n == 0 ? s2 : a2 + s2;
-  }
-#pragma unroll
-  for (int n = 0; n < R; ++n) {
-    double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-    for (int t = z + shell_first<LAT>(n); t < z + shell_first<LAT>(n + 1); ++t) {
-      s1 = s1 + readlane_d(v1[t >> 6], t & 63);
-      s2 = s2 + readlane_d(v2[t >> 6], t & 63);
-    }
-    b1 = n == 0 ? s1 : b1 + s1;
-    b2 = n == 0 ? s2 : b2 + s2;
-  }
-  e1 = a1 + b1;
-  e2 = a2 + b2;
-}
-
 // Bytes of LDS for the sums' entries (fixed_pair_sum2_chains: 4R chains of
 // a stride of shell_max | 1 doubles, at most 24 x 25).
 constexpr int kSumBufBytes = 4800;
-
 
 __device__ inline void init_pair_lanes(const Geo &g, PairLanes &pl) {
   const int lane = threadIdx.x & 63;
@@ -242,29 +208,6 @@ __device__ inline double pair_sum_loop(const GEO &g, const PairLanes &pl,
   return eA + e;
 }
 
-// Both ordered sums of pair_energy_swap_lanes.
-template <class GEO>
-__device__ inline void pair_sum2(const GEO &g, const PairLanes &pl, const double (&v1)[6],
-                                 const double (&v2)[6], double &e1, double &e2) {
-  switch (pl.fixed) {
-    case 16 + 1: fixed_pair_sum2<1, 1>(v1, v2, e1, e2); return;
-    case 16 + 2: fixed_pair_sum2<1, 2>(v1, v2, e1, e2); return;
-    case 16 + 3: fixed_pair_sum2<1, 3>(v1, v2, e1, e2); return;
-    case 16 + 4: fixed_pair_sum2<1, 4>(v1, v2, e1, e2); return;
-    case 16 + 5: fixed_pair_sum2<1, 5>(v1, v2, e1, e2); return;
-    case 16 + 6: fixed_pair_sum2<1, 6>(v1, v2, e1, e2); return;
-    case 32 + 1: fixed_pair_sum2<2, 1>(v1, v2, e1, e2); return;
-    case 32 + 2: fixed_pair_sum2<2, 2>(v1, v2, e1, e2); return;
-    case 32 + 3: fixed_pair_sum2<2, 3>(v1, v2, e1, e2); return;
-    case 32 + 4: fixed_pair_sum2<2, 4>(v1, v2, e1, e2); return;
-    case 32 + 5: fixed_pair_sum2<2, 5>(v1, v2, e1, e2); return;
-    case 32 + 6: fixed_pair_sum2<2, 6>(v1, v2, e1, e2); return;
-    default: break;
-  }
-  e1 = pair_sum(g, pl, v1, nullptr);
-  e2 = pair_sum(g, pl, v2, nullptr);
-}
-
 // fixed_pair_sum2 with the shells' sums formed side by side in lanes: the
 // 4R chains (before / after the swap, site A / B, shell n) each run in a
 // lane of their own over the shell's entries in list order, from 0.0, then
@@ -306,6 +249,31 @@ __device__ __forceinline__ void fixed_pair_sum2_chains(const PairLanes &pl, cons
   e2 = a[2] + a[3];
 }
 
+// Both ordered sums of pair_energy_swap_lanes, the shell structure chosen
+// at run time (the chains for every fcc shell count and bcc up to 6
+// shells, the generic loop beyond).
+template <class GEO>
+__device__ inline void pair_sum2(const GEO &g, const PairLanes &pl, const double (&v1)[6],
+                                 const double (&v2)[6], double *buf, double &e1, double &e2) {
+  switch (pl.fixed) {
+    case 16 + 1: fixed_pair_sum2_chains<1, 1>(pl, v1, v2, buf, e1, e2); return;
+    case 16 + 2: fixed_pair_sum2_chains<1, 2>(pl, v1, v2, buf, e1, e2); return;
+    case 16 + 3: fixed_pair_sum2_chains<1, 3>(pl, v1, v2, buf, e1, e2); return;
+    case 16 + 4: fixed_pair_sum2_chains<1, 4>(pl, v1, v2, buf, e1, e2); return;
+    case 16 + 5: fixed_pair_sum2_chains<1, 5>(pl, v1, v2, buf, e1, e2); return;
+    case 16 + 6: fixed_pair_sum2_chains<1, 6>(pl, v1, v2, buf, e1, e2); return;
+    case 32 + 1: fixed_pair_sum2_chains<2, 1>(pl, v1, v2, buf, e1, e2); return;
+    case 32 + 2: fixed_pair_sum2_chains<2, 2>(pl, v1, v2, buf, e1, e2); return;
+    case 32 + 3: fixed_pair_sum2_chains<2, 3>(pl, v1, v2, buf, e1, e2); return;
+    case 32 + 4: fixed_pair_sum2_chains<2, 4>(pl, v1, v2, buf, e1, e2); return;
+    case 32 + 5: fixed_pair_sum2_chains<2, 5>(pl, v1, v2, buf, e1, e2); return;
+    case 32 + 6: fixed_pair_sum2_chains<2, 6>(pl, v1, v2, buf, e1, e2); return;
+    default: break;
+  }
+  e1 = pair_sum(g, pl, v1, nullptr);
+  e2 = pair_sum(g, pl, v2, nullptr);
+}
+
 // Clears the chain slots of the sum buffer (at kernel start: the chains'
 // padding slots are never written afterwards).
 __device__ inline void clear_sum_buf(double *buf) {
@@ -325,7 +293,7 @@ __device__ inline void pair_sum2_t(const GEO &g, const PairLanes &pl, const doub
     e1 = pair_sum_loop(g, pl, v1, nullptr);
     e2 = pair_sum_loop(g, pl, v2, nullptr);
   } else {
-    pair_sum2(g, pl, v1, v2, e1, e2);
+    pair_sum2(g, pl, v1, v2, sumbuf, e1, e2);
   }
 }
 
