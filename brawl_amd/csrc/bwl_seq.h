@@ -445,10 +445,63 @@ __device__ inline SeqGeo get_seq_geo(const int32_t *gw) {
 }
 
 
+// A cell's periodic images in the padded box, one per lane: with a halo no
+// wider than the box there are at most 3 images per axis, and lane
+// 9 iz + 3 iy + ix (< 27) takes image (ix, iy, iz), which lies ox = ix n0,
+// oy = iy n1, oz = iz n2 past the first image (p0 = (c + h) mod n) and
+// exists when p0 + o stays inside the padded extent on every axis.  The
+// lane's address step dq is fixed for a geometry, so a write is a few
+// compares and one add per lane.
+struct ImageLanes {
+  int ox, oy, oz, dq;
+  bool fast;  // halo no wider than the box
+};
+
+template <class GEO>
+__device__ inline ImageLanes image_lanes(const GEO &g) {
+  const int lane = threadIdx.x & 63;
+  const int ix = lane % 3, iy = (lane / 3) % 3, iz = lane / 9;  // iz >= 3: no image
+  ImageLanes il;
+  il.fast = g.h <= g.n[0] && g.h <= g.n[1] && g.h <= g.n[2];
+  il.ox = ix * g.n[0];
+  il.oy = iy * g.n[1];
+  il.oz = lane < 27 ? iz * g.n[2] : 1 << 29;  // past every extent
+  il.dq = lane < 27 ? (il.oz * g.P[1] + il.oy) * g.P[0] + il.ox : 0;
+  return il;
+}
+
+// Store v at interior cell (s, cx, cy, cz) and all its images (the same
+// cells as write_site), one image per lane.
+template <class GEO>
+__device__ __forceinline__ void write_site_lanes(int8_t *lat, const GEO &g, const ImageLanes &il, int s,
+                                                 int cx, int cy, int cz, int8_t v) {
+  if (!il.fast) {
+    write_site(lat, g, s, cx, cy, cz, v, threadIdx.x & 63, 64);
+    return;
+  }
+  const int p0x = cx + g.h >= g.n[0] ? cx + g.h - g.n[0] : cx + g.h;
+  const int p0y = cy + g.h >= g.n[1] ? cy + g.h - g.n[1] : cy + g.h;
+  const int p0z = cz + g.h >= g.n[2] ? cz + g.h - g.n[2] : cz + g.h;
+  if (p0x + il.ox < g.P[0] && p0y + il.oy < g.P[1] && p0z + il.oz < g.P[2])
+    lat[s * g.sub_stride + (p0z * g.P[1] + p0y) * g.P[0] + p0x + il.dq] = v;
+}
+
 // Write species va at site a and vb at site b (every image of each; the
 // whole wavefront helps).  For kernels of one wavefront per workgroup: the
 // writes are ordered after every earlier LDS access and before every later
-// one (wave_lds_order), with no barrier.
+// one (wave_lds_order), with no barrier.  With ImageLanes each lane writes
+// one fixed image (the walker kernels); without, the lanes split the images
+// of write_site (the sequential Metropolis kernel, where the image lanes
+// measured slower).
+template <class GEO>
+__device__ inline void put_pair(int8_t *lat, const GEO &g, const ImageLanes &il, SiteXYZ a, int suba,
+                                int8_t va, SiteXYZ b, int subb, int8_t vb) {
+  wave_lds_order();
+  write_site_lanes(lat, g, il, suba, a.x >> 1, a.y >> 1, a.z >> 1, va);
+  write_site_lanes(lat, g, il, subb, b.x >> 1, b.y >> 1, b.z >> 1, vb);
+  wave_lds_order();
+}
+
 template <class GEO>
 __device__ inline void put_pair(int8_t *lat, const GEO &g, SiteXYZ a, int suba, int8_t va, SiteXYZ b,
                                 int subb, int8_t vb) {

@@ -137,6 +137,7 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
   int extra = 0, n_acc = 0;
   const int half = (int)((float)nw / 2.0f);
   const SeqGeo sg = get_seq_geo(gw);
+  const ImageLanes il = image_lanes(sg);
   const int n_steps = gw[kSeqGeoWords];
   const long long max_redraws = (long long)get_loop64(gw, 2);
   for (int it = 1; it <= A.n_iter && status == 0; ++it) {
@@ -204,7 +205,7 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
         ++n_acc;
         ++accepted;
       }
-      if (acc) put_pair(lat, sg, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
+      if (acc) put_pair(lat, sg, il, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
     }
     __syncthreads();
     if (lane == 0) E[imax] = ecur;
@@ -276,6 +277,7 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
   int mti = A.mt_index[0];
   MtWin win{0u, kMtWinEmpty};
   const SeqGeo sg = get_seq_geo(gw);
+  const ImageLanes il = image_lanes(sg);
   const int bins = gw[kSeqGeoWords + 0], start_idx = gw[kSeqGeoWords + 1];
   const int end_idx = gw[kSeqGeoWords + 2], nbr_swap = gw[kSeqGeoWords + 3];
   const double wl_f = A.wl_f;
@@ -329,7 +331,7 @@ __global__ __launch_bounds__(64) void k_wl_sweeps(const WlArgs *__restrict__ arg
     } else {
       jbin = ibin;
     }
-    if (s1 != s2 && acc) put_pair(lat, sg, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
+    if (s1 != s2 && acc) put_pair(lat, sg, il, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
     // a walker outside its window would index outside the reference's
     // arrays; such updates are dropped
     const bool in_range = jbin >= 1 && jbin <= bins;
@@ -587,6 +589,7 @@ __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__
   bool skip = A.resume[0] != 0;
   int status = 0;
   const SeqGeo sg = get_seq_geo(gw);
+  const ImageLanes il = image_lanes(sg);
   // iterations until isteps is next a multiple of period, counted down
   long long to_period = period - isteps % period;
   for (;;) {
@@ -631,7 +634,7 @@ __global__ __launch_bounds__(64) void k_wl_enter(const WlEnterArgs *__restrict__
       const double u = genrand_dev(st, mti, win);
       const bool acc = log_lt_dev(u, -de);
       if (acc) e_un = e_sw;
-      if (acc) put_pair(lat, sg, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
+      if (acc) put_pair(lat, sg, il, r1, sub1, (int8_t)s2, r2, sub2, (int8_t)s1);
     }
   }
   __syncthreads();
