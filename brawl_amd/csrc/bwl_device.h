@@ -33,6 +33,51 @@ __device__ __forceinline__ int pmod(int a, int m) {
   return r < 0 ? r + m : r;
 }
 
+// A cell index of a box of extents (E0, E1, E2) x sublattices, held as its
+// digits (s, z, y, x) and advanced by a fixed step with carries: the digits
+// of a thread's strided walk through the box without a division per cell.
+struct BoxWalk {
+  int x, y, z, s;
+  int dx, dy, dz, ds;  // the step's digits
+};
+
+__device__ inline BoxWalk box_walk(int start, int step, int e0, int e1, int e2) {
+  const int plane = e0 * e1, vol = plane * e2;
+  BoxWalk w;
+  w.s = start / vol;
+  int r = start - w.s * vol;
+  w.z = r / plane;
+  r -= w.z * plane;
+  w.y = r / e0;
+  w.x = r - w.y * e0;
+  w.ds = step / vol;
+  r = step - w.ds * vol;
+  w.dz = r / plane;
+  r -= w.dz * plane;
+  w.dy = r / e0;
+  w.dx = r - w.dy * e0;
+  return w;
+}
+
+__device__ __forceinline__ void box_step(BoxWalk &w, int e0, int e1, int e2) {
+  w.x += w.dx;
+  int c = w.x >= e0;
+  w.x -= c ? e0 : 0;
+  w.y += w.dy + c;
+  c = w.y >= e1;
+  w.y -= c ? e1 : 0;
+  w.z += w.dz + c;
+  c = w.z >= e2;
+  w.z -= c ? e2 : 0;
+  w.s += w.ds + c;
+}
+
+// (p - h) mod n for a padded coordinate p in [0, n + 2h), h <= n.
+__device__ __forceinline__ int unpad(int p, int h, int n) {
+  const int c = p - h;
+  return c < 0 ? c + n : (c >= n ? c - n : c);
+}
+
 // Stage a compact lattice (HBM) into the padded LDS layout, halos included.
 // `shift` > 0 stores shift*(species-1) (a bit position for packed species
 // counting) instead of the species itself.
@@ -69,6 +114,40 @@ __device__ inline void unstage_lattice(const int8_t *lat, int8_t *__restrict__ c
     int cx = r - cy * g.n[0];
     int v = lat[padded_addr(g, s, cx, cy, cz)];
     compact[idx] = (int8_t)(shift ? v / shift + 1 : v);
+  }
+}
+
+// stage_lattice for a species lattice staged many times in one kernel (the
+// nested-sampling clone at every iteration): the same cells, each thread's
+// cell digits carried along its strided walk instead of divided out per
+// cell.  (Kernels that stage once keep stage_lattice: there the walk's code
+// measured slower in the trial loop around it.)
+__device__ inline void stage_lattice_walk(int8_t *lat, const int8_t *__restrict__ compact,
+                                          const Geo &g) {
+  const int total = g.nsub * g.sub_stride;
+  if (g.h <= g.n[0] && g.h <= g.n[1] && g.h <= g.n[2]) {
+    BoxWalk w = box_walk(threadIdx.x, blockDim.x, g.P[0], g.P[1], g.P[2]);
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+      const int cx = unpad(w.x, g.h, g.n[0]), cy = unpad(w.y, g.h, g.n[1]), cz = unpad(w.z, g.h, g.n[2]);
+      const int v = compact[compact_addr(g, w.s, cx, cy, cz)];
+      lat[idx] = (int8_t)v;
+      box_step(w, g.P[0], g.P[1], g.P[2]);
+    }
+    return;
+  }
+  stage_lattice(lat, compact, g, 0);
+}
+
+// unstage_lattice by the same carried walk.
+__device__ inline void unstage_lattice_walk(const int8_t *lat, int8_t *__restrict__ compact,
+                                            const Geo &g) {
+  const int cells = g.n[0] * g.n[1] * g.n[2];
+  const int total = g.nsub * cells;
+  BoxWalk w = box_walk(threadIdx.x, blockDim.x, g.n[0], g.n[1], g.n[2]);
+  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const int v = lat[padded_addr(g, w.s, w.x, w.y, w.z)];
+    compact[idx] = (int8_t)v;
+    box_step(w, g.n[0], g.n[1], g.n[2]);
   }
 }
 

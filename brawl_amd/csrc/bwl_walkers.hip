@@ -171,7 +171,7 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
     if (irnd < 1) irnd = 1;
     const int src = irnd - 1;
     __syncthreads();
-    stage_lattice(lat, A.walkers + cb * src, g, 0);
+    stage_lattice_walk(lat, A.walkers + cb * src, g);
     __syncthreads();
     double ecur = E[src];
     n_acc = 0;
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
     }
     __syncthreads();
     if (lane == 0) E[imax] = ecur;
-    unstage_lattice(lat, A.walkers + cb * imax, g, 0);
+    unstage_lattice_walk(lat, A.walkers + cb * imax, g);
     __syncthreads();
   }
   for (int w = lane; w < nw; w += 64) A.E[w] = E[w];
