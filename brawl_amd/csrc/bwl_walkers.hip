@@ -177,17 +177,20 @@ __global__ __launch_bounds__(64) void k_ns(const NsArgs *__restrict__ args) {
     n_acc = 0;
     // the walk (nested_sampling.f90:157-192)
     for (int stp = 1; stp <= n_steps + extra; ++stp) {
+      // rdm2 is redrawn while its species equals rdm1's; the first rdm2 is
+      // drawn before either species is read, so both reads go out together
       const SiteXYZ r1 = random_site_dev(sg, st, mti, win);
+      SiteXYZ r2 = random_site_dev(sg, st, mti, win);
       const int a1 = padded_addr_xyz(sg, r1.x, r1.y, r1.z);
+      int a2 = padded_addr_xyz(sg, r2.x, r2.y, r2.z);
       const int s1 = lat[a1];
-      SiteXYZ r2;
-      int a2, s2;
+      int s2 = lat[a2];
       long long redraws = 0;
-      do {
+      while (s1 == s2 && ++redraws < max_redraws) {
         r2 = random_site_dev(sg, st, mti, win);
         a2 = padded_addr_xyz(sg, r2.x, r2.y, r2.z);
         s2 = lat[a2];
-      } while (s1 == s2 && ++redraws < max_redraws);
+      }
       if (s1 == s2) {
         status = 1;
         break;
