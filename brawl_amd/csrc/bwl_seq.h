@@ -353,7 +353,9 @@ __device__ inline double pair_energy_lanes(const int8_t *lat, const Geo &g, cons
 // A's list sees B when its offset equals B - A mod the box, and A itself
 // when its offset is 0 mod the box (possible on boxes shorter than the
 // interaction range); likewise for B's list.  Both sums are bit-identical
-// to gathers before and after a real swap.
+// to gathers before and after a real swap.  For kernels of one wavefront
+// per workgroup whose `sumbuf` (kSumBufBytes of LDS) was cleared with
+// clear_sum_buf at kernel start.
 template <int KEY = -1, class GEO>
 __device__ inline void pair_energy_swap_lanes(const int8_t *lat, const GEO &g, const int32_t *off,
                                               const double *vex, const PairLanes &pl, int addrA,
