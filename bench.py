@@ -597,9 +597,14 @@ def wang_landau_run(tolerance: float):
     tr, entered = wl.enter_windows(10**8)
     t1 = time.perf_counter()
     log(f"[bench] Wang-Landau run: window entry {t1 - t0:.1f} s, {int(entered.sum())} walkers in")
-    res = wl.run(wl_f=c["wl_f"], tolerance=tolerance, flatness=c["flatness"], max_sweeps_calls=50000,
-                 max_pre_sampling_calls=50000, enter_max_trials=10**8)
+    def progress(halvings, ln_f, calls):
+        log(f"[bench] Wang-Landau run: {halvings} halvings, ln f {ln_f:.3g}, {calls} sweeps calls, "
+            f"{time.perf_counter() - t1:.1f} s")
+
+    res = wl.run(wl_f=c["wl_f"], tolerance=tolerance, flatness=c["flatness"], max_sweeps_calls=200000,
+                 max_pre_sampling_calls=50000, enter_max_trials=10**8, progress=progress)
     t2 = time.perf_counter()
+    n_atoms = wl.n_atoms
     wl.close()
     log(f"[bench] Wang-Landau run: {res.iterations} halvings, {res.sweeps_calls} sweeps calls, "
         f"{t2 - t1:.1f} s")
@@ -610,7 +615,53 @@ def wang_landau_run(tolerance: float):
             "pre_sampling_calls": res.pre_sampling_calls, "trials": res.trials,
             "device_ms": res.kernel_ms, "value": res.trials / (t2 - t1), "unit": "swap trials/s",
             "exchanges": res.exchanges, "exchange_pairs": res.exchange_pairs,
-            "lng_span": float(res.lng.max() - res.lng.min())}
+            "lng_span": float(res.lng.max() - res.lng.min()),
+            "canonical_vs_metropolis": wl_canonical_check(c, res, n_atoms)}
+
+
+def wl_canonical_check(c, res, n_atoms, chains: int = 8, sweeps: int = 2000):
+    """The run's ln g(E) against an independent sampler on the same lattice:
+    the canonical <E>/N from ln g at 1800 -> 1000 K in steps of 100 K (where
+    the canonical distribution lies inside the windows' energy range; the
+    random alloy's energy lies above it), and the parallel Metropolis
+    sampler's <E>/N over `chains` independent anneals through the same
+    temperatures (`sweeps` burn-in and `sweeps` sampled sweeps at each), with
+    the standard error over the chains."""
+    from brawl_amd import Lattice, wang_landau, workloads
+    from brawl_amd.lattice import RY_TO_EV, anneal_batch
+    temps = [1800.0 - 100.0 * k for k in range(9)]
+    e_wl, outer = wang_landau.canonical_energy(res.lng, res.bin_edges, n_atoms, temps)
+    lats = []
+    try:
+        for k in range(chains):
+            L = Lattice(c["lattice"], c["n"], c["v_ex"])
+            L.seed(110179 + 11 * k)
+            L.initial_setup(species_numbers=c["counts"])
+            lats.append(L)
+        params = [dict(T=temps[0], delta_T=-100.0, T_steps=len(temps), n_sweeps=sweeps,
+                       burn_in_sweeps=sweeps, seed=workloads.derive_seed(workloads.MASTER_SEED, 4000 + k))
+                  for k in range(chains)]
+        out, mk = anneal_batch(lats, params)
+    finally:
+        for L in lats:
+            L.close()
+    e = np.array([r.energies_of_T for r in out])
+    mean, err = e.mean(axis=0), e.std(axis=0, ddof=1) / np.sqrt(chains)
+    to_mev = RY_TO_EV * 1000.0           # the bins' conversion (constants.f90:49)
+    rows = []
+    for i, T in enumerate(temps):
+        rows.append({"T": T, "E_wl_meV_per_atom": e_wl[i] * to_mev,
+                     "E_mc_meV_per_atom": float(mean[i]) * to_mev,
+                     "E_mc_err_meV_per_atom": float(err[i]) * to_mev,
+                     "difference_meV_per_atom": (e_wl[i] - float(mean[i])) * to_mev,
+                     "wl_weight_in_outer_5pct": float(outer[i])})
+        log(f"[bench] {T:.0f} K: <E>/N from ln g {e_wl[i] * to_mev:.4f}, Metropolis "
+            f"{mean[i] * to_mev:.4f} +- {err[i] * to_mev:.4f} meV/atom (outer weight {outer[i]:.2g})")
+    diffs = np.array([r["difference_meV_per_atom"] for r in rows])
+    return {"sampler": f"{chains} parallel-sampler anneals, {sweeps} burn-in + {sweeps} sampled "
+                       f"sweeps per temperature, makespan {mk:.0f} ms",
+            "bin_width_meV_per_atom": 50.0 / 800, "rows": rows,
+            "max_abs_difference_meV_per_atom": float(np.abs(diffs).max())}
 
 
 def sequential_bit_exact(sweeps: int = 100):

@@ -265,12 +265,14 @@ class WangLandau:
 
     def run(self, *, wl_f=1.0, tolerance=1e-6, flatness=0.8, max_sweeps_calls=None,
             enter_max_trials=10**9, pre_sampling=True, replica_exchange=True,
-            max_pre_sampling_calls=None):
+            max_pre_sampling_calls=None, progress=None):
         """wl_main (see module docstring for what is left out).  With
         pre_sampling, ln f stays at wl_f through the pre-sampling loop, whose
         ln g (averaged and stitched) starts the main loop; with
         replica_exchange, the main loop exchanges after every sweeps call
-        (both as the reference runs by default, performance = 0)."""
+        (both as the reference runs by default, performance = 0).
+        progress, if given, is called as progress(halvings, ln_f, calls)
+        after every halving of ln f and every 2000 sweeps calls."""
         _, entered = self.enter_windows(enter_max_trials)
         if not entered.all():
             raise RuntimeError("some walkers did not reach their energy window")
@@ -296,6 +298,8 @@ class WangLandau:
             calls += 1
             ms += k_ms
             acc_total += int(acc.sum())
+            if progress is not None and calls % 2000 == 0:
+                progress(iters, wl_f, calls)
             if replica_exchange:
                 partner, ex, x_ms = self.replica_exchange()
                 ms += x_ms
@@ -308,9 +312,36 @@ class WangLandau:
                 wl_f = wl_f * 0.5
                 self._halve_setup(lng, hist)
                 iters += 1
+                if progress is not None:
+                    progress(iters, wl_f, calls)
         lng, _ = self.lng_hist()
         return WlRunResult(lng[0].copy(), self.bin_edges.copy(), iters, calls + pre_calls,
                            (calls + pre_calls) * trials_per_call, ms, pre_calls, exch, pairs)
 
 
-__all__ = ["WangLandau", "WlRunResult"]
+def canonical_energy(lng, bin_edges, n_atoms, temps):
+    """Canonical averages from a binned density of states: for each
+    temperature (K) the mean energy per atom <E>/N (Ry) and the share of the
+    Boltzmann weight that falls in the outer 5 % of the energy range (large
+    when the range cuts the distribution off, and the mean is then biased).
+    lng holds ln g per bin up to a constant and bin_edges the bins' total
+    energies (Ry), as WlRunResult gives them.  Each bin's states are placed
+    at its centre; the error this makes shrinks with the bins' width
+    relative to kT (for a Gaussian density of states about 0.06 of the
+    distribution's spread with bins 7 kT wide, 0.007 with bins 1 kT wide)."""
+    from .lattice import beta_of
+    edges = np.asarray(bin_edges, dtype=np.float64)
+    lng = np.asarray(lng, dtype=np.float64)
+    centres = 0.5 * (edges[:-1] + edges[1:])
+    span = edges[-1] - edges[0]
+    outer = (centres < edges[0] + 0.05 * span) | (centres > edges[-1] - 0.05 * span)
+    means, edge_share = [], []
+    for T in temps:
+        w = lng - beta_of(T) * centres
+        p = np.exp(w - w.max())
+        means.append(float((p * centres).sum() / p.sum()) / n_atoms)
+        edge_share.append(float(p[outer].sum() / p.sum()))
+    return np.array(means), np.array(edge_share)
+
+
+__all__ = ["WangLandau", "WlRunResult", "canonical_energy"]
