@@ -54,11 +54,11 @@ def bytes_per_trial(z: int) -> int:
 
 
 # HBM bytes per launch of the plain anneal kernel, from rocprofv3 --pmc
-# FETCH_SIZE (114.8125 KiB, x2.0 copy-kernel scale) and WRITE_SIZE
-# (229.25 KiB, x1.0): the lattice is read once into LDS and written back
+# FETCH_SIZE (106.125 KiB, x2.0 copy-kernel scale) and WRITE_SIZE
+# (221.25 KiB, x1.0): the lattice is read once into LDS and written back
 # once, plus the per-T-step outputs and the register spills of the
 # radial-density reduction.
-TRAFFIC_BYTES = int(round((114.8125 * 2.0 + 229.25) * 1024))
+TRAFFIC_BYTES = int(round((106.125 * 2.0 + 221.25) * 1024))
 PLAIN_KERNEL = "k_metro_nat<bcc, R=4, 32-bit counts, 32 cells>"
 # radial densities (alpha^(1..4)) sampled every 10 sweeps of each T step
 ASRO_EVERY = 10
