@@ -333,6 +333,21 @@ def batched(steps: int, warmup: int, tune: bool = True):
            "value": float(np.mean(rates)), "unit": "swap trials/s",
            "device_rate": trials / (np.mean(mks) / 1e3), "makespan_ms": float(np.mean(mks)),
            "bytes_per_step": int(alg_bytes)}
+    # the CUs' busy share over the makespan: each chain run alone from its
+    # start (the same trajectory as in the batch, bit for bit) for its own
+    # device time, summed, against every CU for the batch's makespan
+    from brawl_amd import _lib
+    _, n_cus = _lib.lds_bandwidth()
+    solo_ms = []
+    for L, s0, p in zip(lats, starts, params):
+        L.set_config(s0)
+        solo_ms.append(L.metropolis_simulated_annealing(**p).kernel_ms)
+    out["cu_busy_share"] = float(sum(solo_ms) / (n_cus * out["makespan_ms"]))
+    out["cu_busy_note"] = (f"sum of the {len(solo_ms)} chains' device times alone "
+                           f"({sum(solo_ms):.0f} ms; longest {max(solo_ms):.1f} ms) over {n_cus} CUs "
+                           f"x the makespan")
+    log(f"[bench] batched: CUs busy {out['cu_busy_share']:.2f} of the makespan "
+        f"(longest chain alone {max(solo_ms):.1f} ms)")
     if tune:
         for L, s0 in zip(lats, starts):
             L.set_config(s0)
