@@ -8,7 +8,9 @@ and bcc and fcc with 3 shells (the batched workload's kernels).
 
 Both samplers run 16 chains from independent random starts at the same
 temperature, composition and V, with the same burn-in and sampled sweeps,
-and the energy of every sampled sweep is averaged per chain.  The two
+and the energy of every sampled sweep is averaged per chain; the heat
+capacity per chain is the variance of those energies over k_B T^2 N, as
+the reference forms it (metropolis.F90).  For <E>/N and for C the two
 samplers' means agree within 2 sigma of their difference (sigma from the
 chain-to-chain spread of each).  The seeds are fixed, so every run gives
 the same numbers."""
@@ -38,7 +40,7 @@ def _say(*a):
 @pytest.mark.parametrize("lat,n,S,sig,T", CASES)
 def test_parallel_sampler_matches_reference_algorithm(lat, n, S, sig, T):
     from brawl_amd import Lattice, _lib, beta_of, workloads
-    from brawl_amd.lattice import anneal_batch, mc_steps_batch
+    from brawl_amd.lattice import K_B_IN_RY, anneal_batch, mc_steps_batch
     _lib.load()
     v = workloads.random_v(S, sig, workloads.derive_seed(workloads.MASTER_SEED, 700 + lat * 10 + S))
     par, seq = [], []
@@ -54,16 +56,19 @@ def test_parallel_sampler_matches_reference_algorithm(lat, n, S, sig, T):
                                          seed=workloads.derive_seed(workloads.MASTER_SEED, 9000 + k))
                                     for k in range(CHAINS)])
         e_par = np.array([r.energies_of_T[0] for r in res]) * RY_TO_MEV
+        c_par = np.array([r.C_of_T[0] for r in res])
         betas = [beta_of(T)] * CHAINS
         mc_steps_batch(seq, betas, BURN * N)
         _, E, _ = mc_steps_batch(seq, betas, SAMPLED * N, sample_every=N)
         e_seq = np.array([e.mean() for e in E]) / N * RY_TO_MEV
+        c_seq = np.array([e.var() for e in E]) / (K_B_IN_RY * T * T) / N
     finally:
         for L in par + seq:
             L.close()
-    d = e_par.mean() - e_seq.mean()
-    sd = np.sqrt(e_par.var(ddof=1) / CHAINS + e_seq.var(ddof=1) / CHAINS)
-    _say(f"{'fcc' if lat == 1 else 'bcc'} {n} S={S} R={len(sig)} {T:.0f} K: parallel "
-         f"{e_par.mean():.4f}, sequential {e_seq.mean():.4f} meV/atom, difference {d:+.4f} "
-         f"= {d / sd:+.2f} sigma")
-    assert abs(d) <= 2.0 * sd
+    name = f"{'fcc' if lat == 1 else 'bcc'} {n} S={S} R={len(sig)} {T:.0f} K"
+    for what, a, b in (("<E>/N (meV/atom)", e_par, e_seq), ("C (Ry/K per atom)", c_par, c_seq)):
+        d = a.mean() - b.mean()
+        sd = np.sqrt(a.var(ddof=1) / CHAINS + b.var(ddof=1) / CHAINS)
+        _say(f"{name} {what}: parallel {a.mean():.6g}, sequential {b.mean():.6g}, difference "
+             f"{d:+.3g} = {d / sd:+.2f} sigma")
+        assert abs(d) <= 2.0 * sd, what
